@@ -1,0 +1,378 @@
+"""SpMM benchmark on one MI355X.
+
+Headline step (`value`): the FBF SpMM the reference harness launches --
+B2SR-4 bit tiles x FP32 B -> FP32 C (spmm4_full_full_new2,
+backend/spmm.cu:877-956, test_spmm_full.cu:344-369) -- on the seeded
+89,250-row power-law 0/1 graph (~900k nnz) with hidden N = 64, inputs
+resident in HBM. Metric: algorithmic GB/s (gather model: every stored bit
+reads one full B row), with GFLOP/s = 2 nnz N / t.
+
+`extra` holds the other workloads (skipped with --quick): CSR FP32 on the
+16,384-row graph (N=32) and on the 232,965-row ~114.6M-nnz graph (N=64 and
+128, FP32 and BF16 B); GCN-style fwd+bwd (Y = A X, dX = A^T dY,
+dval = SDDMM) in FP32 and BF16 features; 512 uneven graphs batched in one
+captured hipGraph vs 512 separate FP32 launches. Errors there are measured
+against an FP64 numpy recomputation on sampled rows.
+
+`cpu_baseline`: the OpenMP FP32 CSR SpMM of the oracle library on the same
+graph and B (3 warm-ups, median of 15), same byte count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ bytes
+def csr_bytes(M, nnz, N, s_val=4, s_b=4):
+    return 4 * (M + 1) + nnz * (4 + s_val) + nnz * N * s_b + M * N * 4
+
+
+def b2sr_bytes(M, T, nb, nnz, N, s_b=4):
+    MB = (M + T - 1) // T
+    tile = {4: 4, 8: 8, 16: 32, 32: 128}[T]
+    return 4 * (MB + 1) + nb * (4 + tile) + nnz * N * s_b + M * N * 4
+
+
+def sddmm_bytes(M, nnz, N, s=4):
+    return 4 * (M + 1) + 8 * nnz + nnz * N * s + M * N * s
+
+
+# ----------------------------------------------------------------- timing
+def time_launches(fn, steps, warmup):
+    """Runs fn() warmup times, then `steps` times between two events on the
+    current stream; also brackets every launch with its own event pair.
+    Returns (ms per step over the loop, mean ms per launch)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(s)
+    for a, b in pairs:
+        a.record(s)
+        fn()
+        b.record(s)
+    t1.record(s)
+    torch.cuda.synchronize()
+    total = t0.elapsed_time(t1) / steps
+    per = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
+    return total, per
+
+
+def time_loop(fn, steps, warmup):
+    """Back-to-back launches between two events (no per-launch events)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(s)
+    for _ in range(steps):
+        fn()
+    t1.record(s)
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps
+
+
+def sampled_rel_err(g, B64, C, rows):
+    """Relative Frobenius error of C on `rows` against an FP64 recompute."""
+    num = den = 0.0
+    for i in rows:
+        s, e = g.rowptr[i], g.rowptr[i + 1]
+        ref = g.val[s:e].astype(np.float64) @ B64[g.colind[s:e]]
+        num += float(((C[i] - ref) ** 2).sum())
+        den += float((ref ** 2).sum())
+    return (num / den) ** 0.5 if den > 0 else 0.0
+
+
+# ------------------------------------------------------------------ parts
+def headline(args, h, dev):
+    import torch
+
+    from spmm_eval_amd import api, graphs
+
+    T, N = args.tile, args.hidden
+    g = graphs.workload_b2sr_graph()
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    t = time.time()
+    brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
+    torch.cuda.synchronize()
+    conv_ms = (time.time() - t) * 1e3
+    nb = bci.numel()
+    B = graphs.uniform_pm1((g.K, N), seed=graphs.SEED_BASE + 2)
+    Bd = torch.from_numpy(B).to(dev)
+    C = torch.empty((g.M, N), device=dev)
+
+    def step():
+        api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=C)
+
+    ms_step, ms_launch = time_launches(step, args.steps, args.warmup)
+    nbytes = b2sr_bytes(g.M, T, nb, g.nnz, N)
+    flops = 2.0 * g.nnz * N
+    # exactness spot check: +-1 B against an integer recompute of 2000 rows
+    S = np.random.default_rng(3).choice([-1, 1], size=(g.K, N)).astype(np.float32)
+    Cs = api.spmm_b2sr(h, T, g.M, brp, bci, words, torch.from_numpy(S).to(dev)).cpu().numpy()
+    rows = np.random.default_rng(4).choice(g.M, 2000, replace=False)
+    exact = all(np.array_equal(Cs[i], S[g.colind[g.rowptr[i]:g.rowptr[i + 1]]].sum(axis=0)) for i in rows)
+    err = sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows)
+    return dict(g=g, B=B, T=T, N=N, nb=nb, nbytes=nbytes, flops=flops, ms_step=ms_step,
+                ms_launch=ms_launch, conv_ms=conv_ms, exact=exact, err=err)
+
+
+def cpu_baseline(hd):
+    import oracle
+
+    g, B = hd["g"], hd["B"]
+    C = np.empty((g.M, B.shape[1]), np.float32)
+    for _ in range(3):
+        oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B, out=C)
+    ts = []
+    for _ in range(15):
+        t = time.perf_counter()
+        oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B, out=C)
+        ts.append(time.perf_counter() - t)
+    ms = float(np.median(ts)) * 1e3
+    return {
+        "value": round(hd["nbytes"] / (ms * 1e-3) / 1e9, 2),
+        "unit": "GB/s (algorithmic, same byte count)",
+        "ms_per_step": round(ms, 3),
+        "cores": oracle.omp_threads(),
+        "kind": "port",
+        "sample": f"OpenMP FP32 CSR SpMM (oracle/spmm_oracle.c orc_csr_spmm_f32_omp) on the full "
+                  f"89,250-row graph, N={B.shape[1]}, values 1; 3 warm-ups, median of 15 runs",
+    }
+
+
+def extra_csr(args, h, dev):
+    import torch
+
+    from spmm_eval_amd import api, graphs
+
+    out = {}
+    # workload 1: 16,384 rows, N=32
+    g, B = graphs.workload_csr_small()
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    plan = api.CsrPlan(h, rp, g.nnz)
+    Bd = torch.from_numpy(B).to(dev)
+    C = torch.empty((g.M, 32), device=dev)
+    ms = time_loop(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=C, plan=plan), 200, 20)
+    nbytes = csr_bytes(g.M, g.nnz, 32)
+    rows = np.random.default_rng(5).choice(g.M, 2000, replace=False)
+    out["csr_f32_16k_n32"] = {
+        "nnz": g.nnz, "ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1),
+        "pct_hbm_peak": round(100 * nbytes / ms / 1e6 / HBM_PEAK_GBS, 1),
+        "GFLOPs": round(2 * g.nnz * 32 / ms / 1e6, 1),
+        "rel_err_fp64_sampled": sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows)}
+    del plan
+    # workload 3: 232,965 rows, ~114.6M nnz
+    t = time.time()
+    g = graphs.workload_large_graph()
+    log(f"large graph: {g.M} rows, {g.nnz} nnz, max degree {g.degrees().max()} ({time.time() - t:.1f}s)")
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    t = time.time()
+    plan = api.CsrPlan(h, rp, g.nnz)
+    analyze_ms = (time.time() - t) * 1e3
+    rows = np.random.default_rng(6).choice(g.M, 1500, replace=False)
+    for N in (64, 128):
+        B = graphs.uniform_pm1((g.K, N), seed=graphs.SEED_BASE + 3 + N)
+        B64 = B.astype(np.float64)
+        Bd = torch.from_numpy(B).to(dev)
+        Bh = api.to_bf16_bits(Bd)
+        C = torch.empty((g.M, N), device=dev)
+        for name, BB, sb in (("f32", Bd, 4), ("bf16", Bh, 2)):
+            ms = time_loop(lambda: api.spmm_csr(h, rp, ci, v, BB, C_out=C, plan=plan), args.big_steps, 3)
+            nbytes = csr_bytes(g.M, g.nnz, N, s_b=sb)
+            out[f"csr_{name}_233k_n{N}"] = {
+                "nnz": g.nnz, "ms": round(ms, 3), "GBps": round(nbytes / ms / 1e6, 1),
+                "pct_hbm_peak": round(100 * nbytes / ms / 1e6 / HBM_PEAK_GBS, 1),
+                "GFLOPs": round(2 * g.nnz * N / ms / 1e6, 1),
+                "rel_err_fp64_sampled": sampled_rel_err(g, B64, C.cpu().numpy(), rows)}
+        del Bd, Bh, C
+    out["csr_233k_analyze_ms"] = round(analyze_ms, 2)
+    del plan, rp, ci, v
+    torch.cuda.empty_cache()
+    return out
+
+
+def extra_fwd_bwd(args, h, dev):
+    import torch
+
+    from spmm_eval_amd import api, graphs
+
+    g, X, dY = graphs.workload_gcn(N=128)
+    N = X.shape[1]
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    trp, tci, tval, _ = api.csr_transpose(h, rp, ci, v, g.K)
+    pf, pb = api.CsrPlan(h, rp, g.nnz), api.CsrPlan(h, trp, g.nnz)
+    Xd, dYd = torch.from_numpy(X).to(dev), torch.from_numpy(dY).to(dev)
+    Xh, dYh = api.to_bf16_bits(Xd), api.to_bf16_bits(dYd)
+    Y = torch.empty((g.M, N), device=dev)
+    dX = torch.empty((g.K, N), device=dev)
+    dval = torch.empty(g.nnz, device=dev)
+
+    def step(XX, GG):
+        api.spmm_csr(h, rp, ci, v, XX, C_out=Y, plan=pf)
+        api.spmm_csr(h, trp, tci, tval, GG, K=g.M, C_out=dX, plan=pb)
+        api.sddmm_csr(h, rp, ci, GG, XX, out=dval)
+
+    res = {}
+    for name, XX, GG, s in (("f32", Xd, dYd, 4), ("bf16", Xh, dYh, 2)):
+        ms = time_loop(lambda: step(XX, GG), 100, 10)
+        nbytes = 2 * csr_bytes(g.M, g.nnz, N, s_b=s) + sddmm_bytes(g.M, g.nnz, N, s)
+        rows = np.random.default_rng(7).choice(g.M, 1500, replace=False)
+        Yc, dvc = Y.cpu().numpy(), dval.cpu().numpy()
+        e_y = sampled_rel_err(g, X.astype(np.float64), Yc, rows)
+        num = den = 0.0
+        for i in rows:
+            s0, e0 = g.rowptr[i], g.rowptr[i + 1]
+            ref = X[g.colind[s0:e0]].astype(np.float64) @ dY[i].astype(np.float64)
+            num += float(((dvc[s0:e0] - ref) ** 2).sum())
+            den += float((ref ** 2).sum())
+        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1),
+                     "rel_err_Y_fp64": e_y, "rel_err_dval_fp64": (num / den) ** 0.5}
+    res["bf16_speedup_vs_f32"] = round(res["f32"]["ms"] / res["bf16"]["ms"], 3)
+    return {"gcn_fwd_bwd_89k_n128": res}
+
+
+def extra_batched(args, h, dev):
+    import torch
+
+    from spmm_eval_amd import api, graphs
+    from spmm_eval_amd.batched import BatchPlan
+
+    gs = graphs.workload_batched(512)
+    N = 64
+    plan = BatchPlan(h, gs, N=N, bf16=True, device=dev)
+    M = plan.M
+    X = torch.from_numpy(graphs.normal((M, N), 21, 1)).to(dev)
+    dY = torch.from_numpy(graphs.normal((M, N), 21, 2)).to(dev)
+    Xh, dYh = api.to_bf16_bits(X), api.to_bf16_bits(dY)
+    ms_graph = time_loop(lambda: plan.step(Xh, dYh, use_graph=True), 50, 5)
+    # 512 separate plain FP32 launches per operation
+    per = []
+    ro = plan.row_off
+    for i, g in enumerate(gs):
+        rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+        trp, tci, tval, _ = api.csr_transpose(h, rp, ci, v, g.K)
+        per.append((g, rp, ci, v, trp, tci, tval, api.CsrPlan(h, rp, g.nnz), api.CsrPlan(h, trp, g.nnz),
+                    int(ro[i]), int(ro[i + 1]), int(plan.nnz_off[i]), int(plan.nnz_off[i + 1])))
+    Y = torch.empty((M, N), device=dev)
+    dX = torch.empty((M, N), device=dev)
+    dval = torch.empty(plan.nnz, device=dev)
+
+    def separate():
+        for g, rp, ci, v, trp, tci, tval, pf, pb, r0, r1, e0, e1 in per:
+            api.spmm_csr(h, rp, ci, v, X[r0:r1], C_out=Y[r0:r1], plan=pf)
+            api.spmm_csr(h, trp, tci, tval, dY[r0:r1], K=g.M, C_out=dX[r0:r1], plan=pb)
+            api.sddmm_csr(h, rp, ci, dY[r0:r1], X[r0:r1], out=dval[e0:e1])
+
+    ms_sep = time_loop(separate, 5, 1)
+    plan.close()
+    return {"batched_512_fwd_bwd_n64": {
+        "graphs": len(gs), "rows": M, "nnz": plan.nnz,
+        "bf16_one_graph_ms": round(ms_graph, 4), "f32_separate_launches_ms": round(ms_sep, 3),
+        "speedup": round(ms_sep / ms_graph, 2)}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--tile", type=int, default=4)
+    ap.add_argument("--hidden", type=int, default=64)
+    ap.add_argument("--big-steps", type=int, default=20)
+    ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="run only a device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    dev = torch.device("cuda:0")
+    if args.copy_calib:
+        n = args.copy_calib * (1 << 20) // 4
+        a = torch.ones(n, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_MiB": args.copy_calib}))
+        return
+
+    from spmm_eval_amd import api
+
+    h = api.Handle()
+    log(f"headline: B2SR-{args.tile} FBF SpMM, N={args.hidden}")
+    hd = headline(args, h, dev)
+    gbs = hd["nbytes"] / (hd["ms_step"] * 1e-3) / 1e9
+    ach = hd["nbytes"] / (hd["ms_launch"] * 1e-3) / 1e9
+    result = {
+        "metric": "SpMM HBM GB/s (algorithmic) -- B2SR-4 FBF SpMM, FP32 B",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(hd["ms_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Chung-Lu power-law graph, uniform[-1,1] B)",
+        "config": {"workload": "b2sr_fbf_spmm", "rows": hd["g"].M, "nnz": hd["g"].nnz,
+                   "tile": hd["T"], "nnzb": hd["nb"], "hidden": hd["N"], "parallelism": "single GPU"},
+        "gflops": round(hd["flops"] / (hd["ms_step"] * 1e-3) / 1e9, 1),
+        "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "b2sr_spmm_kernel<4,float,16,1>",
+                     "bytes_per_launch": hd["nbytes"], "mean_launch_ms": round(hd["ms_launch"], 5)},
+        "checks": {"pm1_exact_2000_rows": bool(hd["exact"]), "rel_err_fp64_sampled": hd["err"]},
+        "csr2b2sr_ms": round(hd["conv_ms"], 2),
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(hd)
+    if not args.quick:
+        extra = {}
+        for fn in (extra_csr, extra_fwd_bwd, extra_batched):
+            log(fn.__name__)
+            try:
+                extra.update(fn(args, h, dev))
+            except Exception as e:  # report and keep the headline line
+                extra[fn.__name__] = {"error": repr(e)}
+        result["extra"] = extra
+    h.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

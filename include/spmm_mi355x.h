@@ -1,0 +1,212 @@
+/*
+ * spmm_mi355x.h -- C-ABI of the MI355X (gfx950) SpMM library.
+ *
+ * Plain C types only: device pointers are raw pointers into HIP device
+ * memory, sizes are int, streams are hipStream_t (passed as void* so the
+ * header does not need HIP). Every call returns an int status (SPMM_OK == 0);
+ * spmm_last_error() gives the message of the last failure on the calling
+ * thread. All kernels run asynchronously on the given stream unless the
+ * comment says the call synchronises.
+ *
+ * Matrix conventions (the reference's FBF kernels use the same, see
+ * backend/spmm.cu:938,951): dense B is row-major [K][ldb], dense C is
+ * row-major [M][ldc]; CSR is zero-based int32 (rowptr[M+1], colind[nnz]).
+ * C is fully written (beta = 0), a superset of the reference kernels'
+ * "store nonzero results only" behaviour on a pre-zeroed C
+ * (test_spmm_full.cu:347-348).
+ *
+ * B2SR ("bit-block sparse row") layout, as produced by the reference's
+ * csr2bsr_batch_{4,8,16,32} + ToBit{4,8,16,32}Col
+ * (backend/csr2bsr_batch_bsrbmv.cu:207-397,403-579,654-830,894-1070;
+ * backend/spmm.cu:165-194,226-246,272-291,318-335):
+ *   bsrRowPtr[MB+1], bsrColInd[nb] ascending within a block row, and
+ *   bsrVal[nb*T] words; word bsrVal[T*b + r] has bit (T-1-c) set iff
+ *   A[T*br + r][T*bc + c] is a stored entry. Word type: uint8 for T=4 (low
+ *   nibble) and T=8, uint16 for T=16, uint32 for T=32.  MB = ceil(M/T).
+ *
+ * BF16 operands are uint16 bit patterns (round-to-nearest-even from FP32).
+ */
+#ifndef SPMM_MI355X_H
+#define SPMM_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPMM_OK 0
+#define SPMM_ERR_INVALID_ARG 1   /* null pointer, negative size, bad ld */
+#define SPMM_ERR_UNSUPPORTED 2   /* tile size / N / alignment not supported */
+#define SPMM_ERR_HIP 3           /* a HIP runtime call failed */
+#define SPMM_ERR_ALLOC 4         /* device allocation failed */
+#define SPMM_ERR_MISMATCH 5      /* plan does not match the operands given */
+
+typedef struct spmm_handle_s *spmm_handle_t;       /* workspace + device id */
+typedef struct spmm_csr_plan_s *spmm_csr_plan_t;   /* row schedule of a CSR */
+typedef struct spmm_batch_plan_s *spmm_batch_plan_t;
+
+/* Launch geometry override (the autotuner's knobs). Zero fields = default.
+ *   lanes_per_row : lanes that share one output row (power of two, 4..64)
+ *   vec           : 16-byte vectors of B loaded per lane per gather (1 or 2)
+ *   rows_scale    : multiplier on rows handled per subgroup for short rows */
+typedef struct spmm_tune_s {
+  int lanes_per_row;
+  int vec;
+  int rows_scale;
+} spmm_tune_t;
+
+/* ---------------------------------------------------------------- handle */
+int spmm_create(spmm_handle_t *out);
+int spmm_destroy(spmm_handle_t h);
+const char *spmm_status_string(int status);
+const char *spmm_last_error(void);
+int spmm_version(void);
+
+/* ------------------------------------------------------------ CSR SpMM
+ * Replaces the cuSPARSE CSR SpMM of evalCSRSpmmFloatCuSPARSE
+ * (test_spmm_full.cu:212-342: cusparseCreateCsr :281-284, cusparseSpMM
+ * :303-307, alpha = 1, beta = 0, op N/N, 32-bit indices, R_32F), with B and
+ * C row-major instead of the column-major layout the harness converts to.
+ *
+ * spmm_csr_analyze builds the row schedule (rows binned by degree, longest
+ * bins first). It launches two small kernels and SYNCHRONISES the stream
+ * once to read the bin counts. The plan keeps no pointer to rowptr. */
+int spmm_csr_analyze(spmm_handle_t h, int M, int nnz, const int *rowptr,
+                     spmm_csr_plan_t *plan, void *stream);
+int spmm_csr_plan_destroy(spmm_csr_plan_t plan);
+/* counts[32]: number of rows in each degree bin (bin 0: empty rows,
+ * bin b >= 1: degree in [2^(b-1), 2^b)). */
+int spmm_csr_plan_bins(spmm_csr_plan_t plan, int *counts32);
+
+/* Drop-in form: analyses then computes (synchronises once, in analysis). */
+int spmm_csr_f32(spmm_handle_t h, int M, int K, int N, int nnz,
+                 const int *rowptr, const int *colind, const float *val,
+                 const float *B, int ldb, float *C, int ldc, void *stream);
+int spmm_csr_bf16(spmm_handle_t h, int M, int K, int N, int nnz,
+                  const int *rowptr, const int *colind, const float *val,
+                  const uint16_t *B, int ldb, float *C, int ldc, void *stream);
+
+/* Planned form: fully asynchronous, no allocation. `tune` may be NULL. */
+int spmm_csr_f32_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
+                         const int *rowptr, const int *colind,
+                         const float *val, const float *B, int ldb, float *C,
+                         int ldc, const spmm_tune_t *tune, void *stream);
+int spmm_csr_bf16_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
+                          const int *rowptr, const int *colind,
+                          const float *val, const uint16_t *B, int ldb,
+                          float *C, int ldc, const spmm_tune_t *tune,
+                          void *stream);
+
+/* ---------------------------------------------------------------- B2SR
+ * csr2b2sr_nnzb + csr2b2sr replace cusparseXcsr2bsrNnz
+ * (test_spmm_full.cu:148-151) and csr2bsr_batch_{4,8,16,32}
+ * (backend/csr2bsr_batch_bsrbmv.cu:207,403,654,894). CSR input is on the
+ * device with column indices ascending within each row (as coo2csr emits,
+ * backend/readMtx.hpp:246-304); values are ignored: every stored entry is a
+ * set bit (the reference forces values to 1.0 before packing,
+ * csr2bsr_batch_bsrbmv.cu:296-297). T in {4, 8, 16, 32}.
+ * csr2b2sr_nnzb writes bsrRowPtr[MB+1] and SYNCHRONISES to return nnzb. */
+int csr2b2sr_nnzb(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
+                  const int *colind, int T, int *bsrRowPtr, int *nnzb,
+                  void *stream);
+int csr2b2sr(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
+             const int *colind, int T, const int *bsrRowPtr, int *bsrColInd,
+             void *bsrVal, void *stream);
+
+/* FBF SpMM: C[M][N] = A_bin * B, replacing spmm4_full_full_new2 and its
+ * siblings (backend/spmm.cu:877-956, launched at test_spmm_full.cu:351-361).
+ * Any N >= 1 (the reference requires 32 | N and N | 1024). */
+int spmm_b2sr_f32(spmm_handle_t h, int T, int M, int K, int N,
+                  const int *bsrRowPtr, const int *bsrColInd,
+                  const void *bsrVal, const float *B, int ldb, float *C,
+                  int ldc, void *stream);
+int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N,
+                   const int *bsrRowPtr, const int *bsrColInd,
+                   const void *bsrVal, const uint16_t *B, int ldb, float *C,
+                   int ldc, void *stream);
+
+/* Binary A x sign-packed B -> int32: C[i][k] = sum over stored (i,j) of
+ * (+1 if B[j][k] >= 0 else -1), with B packed by pack_sign_rows_u32 into
+ * Bw[K][ldbw] (ldbw >= ceil(N/32) words). No reference counterpart (its
+ * bsrbmv.cu is absent, csr2bsr_batch_bsrbmv.cu:1). */
+int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N,
+                             const int *bsrRowPtr, const int *bsrColInd,
+                             const void *bsrVal, const uint32_t *Bw, int ldbw,
+                             int32_t *C, int ldc, void *stream);
+
+/* Dense bit packers (backend/spmm.cu:57-130).
+ * pack_sign_rows_u32 / pack_bin_rows_u32: X[H][W] (row stride ldx) ->
+ *   out[H][ceil(W/32)], bit (31-i) of word w = X[h][32w+i] >= 0 (sign) or
+ *   > 0 (bin); bits past W are 0 (ToBit32RowDenseSign/Bin :75-91, :57-73).
+ * pack_sign_cols_u32: X[H][W] -> out[ceil(W/32)][32*ceil(H/32)], word
+ *   out[w][h] has bit (31-i) = X[h][32w+i] >= 0; rows past H are 0
+ *   (ToBit32ColUd :93-111 with grid (ceil(H/32), ceil(W/32))).
+ * unpack_cols_u32_f32: inverse of pack_sign_cols_u32 to 0/1 floats
+ *   (Bit32ColTo :116-130). */
+int pack_sign_rows_u32(int H, int W, const float *X, int ldx, uint32_t *out,
+                       void *stream);
+int pack_bin_rows_u32(int H, int W, const float *X, int ldx, uint32_t *out,
+                      void *stream);
+int pack_sign_cols_u32(int H, int W, const float *X, int ldx, uint32_t *out,
+                       void *stream);
+int unpack_cols_u32_f32(int H, int W, const uint32_t *in, float *X, int ldx,
+                        void *stream);
+
+/* ------------------------------------------------------- binary matmul
+ * XNOR/popcount matmul of backend/mm.cu. A is word-major [Kw][M]
+ * (A[i*M + m]), B is word-major [Kw][N].
+ * bmm32_xnor_f32: C[m][n] = 32*Kw - 2*sum_i popc(A[i][m] ^ B[i][n]), row-major
+ *   with stride ldc (BMM32_Arow_Brow :17-53 and BMM32_MT_M_S :56-75; bits
+ *   past K in the last word count as matches exactly as there).
+ * bmm32_xnor_bin: bit (31-j%32) of C[j/32][m] = (Kbits - 2*popcount >= 0),
+ *   word-major [ceil(N/32)][M] (BMM32_BIN :77-105). */
+int bmm32_xnor_f32(int M, int N, int Kw, const uint32_t *A, const uint32_t *B,
+                   float *C, int ldc, void *stream);
+int bmm32_xnor_bin(int M, int N, int Kbits, const uint32_t *A,
+                   const uint32_t *B, uint32_t *C, void *stream);
+
+/* ------------------------------------------------------------- adjoint
+ * csr_transpose: stable device transpose (entries of each output row in
+ * ascending source-row order), the device form of csr2csc
+ * (backend/readMtx.hpp:319-345). Outputs tRowPtr[K+1], tColInd[nnz],
+ * tVal[nnz] (may be NULL to skip values) and, if perm != NULL, perm[e] =
+ * the CSR position of transposed entry e. Synchronises the stream. */
+int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
+                  const int *colind, const float *val, int *tRowPtr,
+                  int *tColInd, float *tVal, int *perm, void *stream);
+
+/* SDDMM at the stored positions: dval[e] = sum_k dY[i][k] * X[j][k] for
+ * stored (i, j) = entry e, FP32 accumulation. dY [M][lddy], X [K][ldx]. */
+int sddmm_csr_f32(spmm_handle_t h, int M, int K, int N, int nnz,
+                  const int *rowptr, const int *colind, const float *dY,
+                  int lddy, const float *X, int ldx, float *dval,
+                  void *stream);
+int sddmm_csr_bf16(spmm_handle_t h, int M, int K, int N, int nnz,
+                   const int *rowptr, const int *colind, const uint16_t *dY,
+                   int lddy, const uint16_t *X, int ldx, float *dval,
+                   void *stream);
+
+/* ------------------------------------------------------------- batched
+ * Many CSR graphs run as one block-diagonal problem. Graph g has rows
+ * [row_off[g], row_off[g+1]) and entries [nnz_off[g], nnz_off[g+1]) of the
+ * concatenated arrays; its column indices are local (0-based in g), and its
+ * feature rows follow the same row offsets (square graphs). The plan holds
+ * the analysed forward and transposed schedules and a captured hipGraph of
+ * one fwd+bwd step (Y = A X, dX = A^T dY, dval = SDDMM(dY, X)). */
+int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off,
+                           const int *nnz_off, const int *rowptr,
+                           const int *colind, const float *val, int N,
+                           int bf16, spmm_batch_plan_t *plan, void *stream);
+int spmm_batch_plan_destroy(spmm_batch_plan_t plan);
+/* One fwd+bwd step; features are uint16 BF16 when the plan is bf16, else
+ * float. Y, dX are FP32 [total_rows][N]; dval FP32 [total_nnz]. */
+int spmm_batch_fwd_bwd(spmm_batch_plan_t plan, const void *X, const void *dY,
+                       float *Y, float *dX, float *dval, int use_graph,
+                       void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPMM_MI355X_H */

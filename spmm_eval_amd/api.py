@@ -1,0 +1,215 @@
+"""Python host side over the C-ABI (torch tensors are device-memory and
+stream plumbing only; every computation runs in libspmm_mi355x.so).
+
+Names follow the reference's host entry points where one exists:
+``spmm_csr`` is the CSR SpMM of evalCSRSpmmFloatCuSPARSE
+(test_spmm_full.cu:212-342), ``csr2b2sr`` is CSR2B2SR + csr2bsr_batch_T
+(test_spmm_full.cu:127-169, backend/csr2bsr_batch_bsrbmv.cu:207-1070),
+``spmm_b2sr`` is the FBF kernel launch of evalB2SRSpmmFull32
+(test_spmm_full.cu:344-369), ``bmm32_*`` are the kernels of backend/mm.cu.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import Tune, call
+
+WORD_DTYPE = {4: torch.uint8, 8: torch.uint8, 16: torch.uint16, 32: torch.uint32}
+
+
+def _p(t):
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+def _dev(t: torch.Tensor, dtype=None) -> torch.Tensor:
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    return t
+
+
+def _is_bf16(B: torch.Tensor) -> bool:
+    return B.dtype in (torch.bfloat16, torch.uint16, torch.int16)
+
+
+class Handle:
+    """spmm_handle_t: owns a device workspace; one per host thread."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("spmm_create", C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.lib().spmm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CsrPlan:
+    """Row schedule of one CSR structure (spmm_csr_analyze)."""
+
+    def __init__(self, h: Handle, rowptr: torch.Tensor, nnz: int, stream=None):
+        self.M = rowptr.numel() - 1
+        self.nnz = int(nnz)
+        p = C.c_void_p()
+        call("spmm_csr_analyze", h.h, self.M, self.nnz, _p(rowptr), C.byref(p), _stream(stream))
+        self.p = p
+
+    def bins(self) -> list[int]:
+        arr = (C.c_int * 32)()
+        call("spmm_csr_plan_bins", self.p, arr)
+        return list(arr)
+
+    def close(self):
+        if self.p:
+            _lib.lib().spmm_csr_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spmm_csr(h: Handle, rowptr, colind, val, B, K: int | None = None, C_out=None,
+             plan: CsrPlan | None = None, tune: tuple | None = None, stream=None):
+    """C = A * B for CSR A (FP32 values) and row-major B (FP32, or BF16
+    given as torch.bfloat16 / uint16 bits); FP32 C."""
+    M = rowptr.numel() - 1
+    nnz = colind.numel()
+    K = B.shape[0] if K is None else K
+    N = B.shape[1]
+    if C_out is None:
+        C_out = torch.empty((M, N), dtype=torch.float32, device=B.device)
+    bf16 = _is_bf16(B)
+    ldb = B.stride(0) if B.dim() == 2 else N
+    ldc = C_out.stride(0)
+    args = (M, K, N, nnz, _p(rowptr), _p(colind), _p(val), _p(B), ldb, _p(C_out), ldc)
+    if plan is None:
+        call("spmm_csr_bf16" if bf16 else "spmm_csr_f32", h.h, *args, _stream(stream))
+    else:
+        t = C.byref(Tune(*tune)) if tune else None
+        call("spmm_csr_bf16_planned" if bf16 else "spmm_csr_f32_planned", plan.p, *args, t,
+             _stream(stream))
+    return C_out
+
+
+def csr2b2sr(h: Handle, rowptr, colind, M: int, K: int, T: int, stream=None):
+    """Device CSR -> B2SR-T (bsrRowPtr, bsrColInd, bsrVal words)."""
+    MB = (M + T - 1) // T
+    dev = rowptr.device
+    brp = torch.empty(MB + 1, dtype=torch.int32, device=dev)
+    nb = C.c_int()
+    call("csr2b2sr_nnzb", h.h, M, K, colind.numel(), _p(rowptr), _p(colind), T, _p(brp),
+         C.byref(nb), _stream(stream))
+    bci = torch.empty(max(nb.value, 1), dtype=torch.int32, device=dev)
+    words = torch.empty(max(nb.value, 1) * T, dtype=WORD_DTYPE[T], device=dev)
+    call("csr2b2sr", h.h, M, K, colind.numel(), _p(rowptr), _p(colind), T, _p(brp), _p(bci),
+         _p(words), _stream(stream))
+    return brp, bci[: nb.value], words[: nb.value * T]
+
+
+def spmm_b2sr(h: Handle, T: int, M: int, brp, bci, words, B, C_out=None, stream=None):
+    K, N = B.shape
+    if C_out is None:
+        C_out = torch.empty((M, N), dtype=torch.float32, device=B.device)
+    fn = "spmm_b2sr_bf16" if _is_bf16(B) else "spmm_b2sr_f32"
+    call(fn, h.h, T, M, K, N, _p(brp), _p(bci), _p(words), _p(B), B.stride(0), _p(C_out),
+         C_out.stride(0), _stream(stream))
+    return C_out
+
+
+def spmm_b2sr_sign(h: Handle, T: int, M: int, K: int, N: int, brp, bci, words, Bw,
+                   C_out=None, stream=None):
+    if C_out is None:
+        C_out = torch.empty((M, N), dtype=torch.int32, device=Bw.device)
+    call("spmm_b2sr_signpacked_i32", h.h, T, M, K, N, _p(brp), _p(bci), _p(words), _p(Bw),
+         Bw.stride(0), _p(C_out), C_out.stride(0), _stream(stream))
+    return C_out
+
+
+def _pack(fn: str, X, cols: bool, stream=None):
+    H, W = X.shape
+    Wc = (W + 31) // 32
+    shape = (Wc, 32 * ((H + 31) // 32)) if cols else (H, Wc)
+    out = torch.empty(shape, dtype=torch.uint32, device=X.device)
+    call(fn, H, W, _p(X), X.stride(0), _p(out), _stream(stream))
+    return out
+
+
+def pack_sign_rows(X, stream=None):
+    return _pack("pack_sign_rows_u32", X, False, stream)
+
+
+def pack_bin_rows(X, stream=None):
+    return _pack("pack_bin_rows_u32", X, False, stream)
+
+
+def pack_sign_cols(X, stream=None):
+    return _pack("pack_sign_cols_u32", X, True, stream)
+
+
+def unpack_cols(words, H: int, W: int, stream=None):
+    X = torch.zeros((H, W), dtype=torch.float32, device=words.device)
+    call("unpack_cols_u32_f32", H, W, _p(words), _p(X), W, _stream(stream))
+    return X
+
+
+def bmm32_xnor_f32(A, B, M: int, N: int, Kw: int, stream=None):
+    Cm = torch.empty((M, N), dtype=torch.float32, device=A.device)
+    call("bmm32_xnor_f32", M, N, Kw, _p(A), _p(B), _p(Cm), N, _stream(stream))
+    return Cm
+
+
+def bmm32_xnor_bin(A, B, M: int, N: int, Kbits: int, stream=None):
+    Cw = torch.empty(((N + 31) // 32, M), dtype=torch.uint32, device=A.device)
+    call("bmm32_xnor_bin", M, N, Kbits, _p(A), _p(B), _p(Cw), _stream(stream))
+    return Cw
+
+
+def csr_transpose(h: Handle, rowptr, colind, val, K: int, stream=None):
+    """Returns (tRowPtr[K+1], tColInd, tVal, perm)."""
+    M = rowptr.numel() - 1
+    nnz = colind.numel()
+    dev = rowptr.device
+    trp = torch.empty(K + 1, dtype=torch.int32, device=dev)
+    tci = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    tv = torch.empty(max(nnz, 1), dtype=torch.float32, device=dev) if val is not None else None
+    perm = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    call("csr_transpose", h.h, M, K, nnz, _p(rowptr), _p(colind), _p(val), _p(trp), _p(tci),
+         _p(tv), _p(perm), _stream(stream))
+    return trp, tci[:nnz], (tv[:nnz] if tv is not None else None), perm[:nnz]
+
+
+def sddmm_csr(h: Handle, rowptr, colind, dY, X, out=None, stream=None):
+    M = rowptr.numel() - 1
+    K, N = X.shape
+    nnz = colind.numel()
+    if out is None:
+        out = torch.empty(max(nnz, 1), dtype=torch.float32, device=X.device)[:nnz]
+    fn = "sddmm_csr_bf16" if _is_bf16(X) else "sddmm_csr_f32"
+    call(fn, h.h, M, K, N, nnz, _p(rowptr), _p(colind), _p(dY), dY.stride(0), _p(X),
+         X.stride(0), _p(out), _stream(stream))
+    return out
+
+
+def to_bf16_bits(x: torch.Tensor) -> torch.Tensor:
+    """FP32 -> BF16 (round to nearest even) as a uint16 view."""
+    return x.to(torch.bfloat16).view(torch.uint16)

@@ -1,0 +1,313 @@
+// Adjoint of Y = A X for CSR A: the stable device transpose (so dX = A^T dY
+// runs through the same CSR SpMM kernels) and SDDMM for the value gradient
+// dval[e] = <dY[i], X[j]> at each stored (i, j).
+//
+// Transpose (the device form of csr2csc, backend/readMtx.hpp:319-345):
+// column counts by atomics, device scan to column pointers, scatter of the
+// source positions through atomic per-column cursors, then every column's
+// segment of source positions is sorted ascending (one workgroup per
+// column, bitonic in LDS, or in global memory for segments longer than the
+// LDS tile). Sorting restores the stable order, so the output is unique and
+// identical to the sequential counting sort. Row indices and values are then
+// gathered through the sorted positions.
+//
+// SDDMM: one subgroup of G lanes per row keeps its dY row segment in
+// registers; for each stored entry it gathers the X row segment (16-byte
+// loads, several entries in flight), forms the partial dot products and
+// reduces them across the G lanes with xor-shuffles.
+#include "common.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace spmm {
+
+constexpr int kAdjThreads = 256;
+constexpr int kSortTile = 4096;  // LDS bitonic tile (ints)
+
+__global__ void __launch_bounds__(kAdjThreads)
+col_count_kernel(int nnz, const int *__restrict__ colind, int *__restrict__ cnt) {
+  for (int e = blockIdx.x * kAdjThreads + threadIdx.x; e < nnz; e += gridDim.x * kAdjThreads)
+    atomicAdd(&cnt[colind[e]], 1);
+}
+
+__global__ void __launch_bounds__(kAdjThreads)
+row_of_entry_kernel(int M, const int *__restrict__ rowptr, int *__restrict__ rowidx) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + threadIdx.x / 64;
+  if (row >= M) return;
+  for (int e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) rowidx[e] = row;
+}
+
+__global__ void __launch_bounds__(kAdjThreads)
+scatter_kernel(int nnz, const int *__restrict__ colind, int *__restrict__ cursor,
+               int *__restrict__ pos) {
+  for (int e = blockIdx.x * kAdjThreads + threadIdx.x; e < nnz; e += gridDim.x * kAdjThreads)
+    pos[atomicAdd(&cursor[colind[e]], 1)] = e;
+}
+
+// One step of the all-ascending bitonic network ("flip" form): comparator
+// (i, i ^ mask) puts the smaller value at the lower index. Partners at or
+// past `len` stand for +infinity and never move into range, so skipping
+// them sorts any length without padding.
+template <typename Seg>
+__device__ __forceinline__ void bitonic_step(Seg seg, int len, int n2, int mask) {
+  for (int i = threadIdx.x; i < n2; i += kAdjThreads) {
+    const int p = i ^ mask;
+    if (p > i && p < len) {
+      const int a = seg[i], b = seg[p];
+      if (a > b) { seg[i] = b; seg[p] = a; }
+    }
+  }
+  __syncthreads();
+}
+
+template <typename Seg>
+__device__ void bitonic_sort(Seg seg, int len) {
+  int n2 = 1;
+  while (n2 < len) n2 <<= 1;
+  for (int k = 2; k <= n2; k <<= 1) {
+    bitonic_step(seg, len, n2, k - 1);  // mirror compare within blocks of k
+    for (int j = k >> 2; j > 0; j >>= 1) bitonic_step(seg, len, n2, j);
+  }
+}
+
+// Sorts pos[colptr[c] .. colptr[c+1]) ascending; one workgroup per column.
+__global__ void __launch_bounds__(kAdjThreads)
+segment_sort_kernel(const int *__restrict__ colptr, int *__restrict__ pos) {
+  __shared__ int tile[kSortTile];
+  const int c = blockIdx.x;
+  const int b = colptr[c], len = colptr[c + 1] - b;
+  if (len <= 1) return;
+  int *seg = pos + b;
+  if (len <= 32) {
+    if (threadIdx.x == 0) {  // insertion sort
+      for (int i = 1; i < len; ++i) {
+        int v = seg[i], j = i - 1;
+        while (j >= 0 && seg[j] > v) { seg[j + 1] = seg[j]; --j; }
+        seg[j + 1] = v;
+      }
+    }
+    return;
+  }
+  if (len <= kSortTile) {
+    for (int i = threadIdx.x; i < len; i += kAdjThreads) tile[i] = seg[i];
+    __syncthreads();
+    bitonic_sort<int *>(tile, len);
+    for (int i = threadIdx.x; i < len; i += kAdjThreads) seg[i] = tile[i];
+    return;
+  }
+  bitonic_sort<int *>(seg, len);  // long segment: same network in global memory
+}
+
+__global__ void __launch_bounds__(kAdjThreads)
+gather_kernel(int nnz, const int *__restrict__ pos, const int *__restrict__ rowidx,
+              const float *__restrict__ val, int *__restrict__ tcol, float *__restrict__ tval,
+              int *__restrict__ perm) {
+  for (int d = blockIdx.x * kAdjThreads + threadIdx.x; d < nnz; d += gridDim.x * kAdjThreads) {
+    const int e = pos[d];
+    tcol[d] = rowidx[e];
+    if (tval) tval[d] = val[e];
+    if (perm) perm[d] = e;
+  }
+}
+
+// ---------------------------------------------------------------- SDDMM
+template <typename TB, int G>
+__global__ void __launch_bounds__(kAdjThreads)
+sddmm_kernel(int M, const int *__restrict__ rowptr, const int *__restrict__ colind,
+             const TB *__restrict__ dY, int lddy, const TB *__restrict__ X, int ldx,
+             float *__restrict__ out, int ntile) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int U = 8;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int lane_sub = lane & (G - 1);
+  const int gsub = (threadIdx.x / kWave) * S + lane / G;
+  const int row = blockIdx.x * (kAdjThreads / G) + gsub;
+  if (row >= M) return;
+  const int e0 = rowptr[row], e1 = rowptr[row + 1];
+  if (e0 == e1) return;
+  // ntile column tiles of G*EPL each; partial dots are summed tile by tile.
+  float y[4][EPL];
+  for (int tt = 0; tt < ntile; tt += 4) {
+    const int nt = min(4, ntile - tt);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if (t < nt)
+        Vec16<TB>::load(dY + static_cast<size_t>(row) * lddy + (tt + t) * G * EPL + lane_sub * EPL,
+                        y[t]);
+    for (int e = e0; e < e1; e += U) {
+      float part[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) part[u] = 0.f;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + u < e1) {
+          const TB *xr = X + static_cast<size_t>(colind[e + u]) * ldx + lane_sub * EPL;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            if (t < nt) {
+              float x[EPL];
+              Vec16<TB>::load(xr + (tt + t) * G * EPL, x);
+#pragma unroll
+              for (int q = 0; q < EPL; ++q) part[u] = fmaf(y[t][q], x[q], part[u]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int m = 1; m < G; m <<= 1) part[u] += __shfl_xor(part[u], m, G);
+      }
+      if (lane_sub == 0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (e + u < e1) {
+            if (tt == 0) out[e + u] = part[u];
+            else out[e + u] += part[u];
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename TB>
+__global__ void __launch_bounds__(kAdjThreads)
+sddmm_generic_kernel(int M, int N, const int *__restrict__ rowptr, const int *__restrict__ colind,
+                     const TB *__restrict__ dY, int lddy, const TB *__restrict__ X, int ldx,
+                     float *__restrict__ out) {
+  const int row = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (row >= M) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) {
+    const TB *x = X + static_cast<size_t>(colind[e]) * ldx;
+    const TB *y = dY + static_cast<size_t>(row) * lddy;
+    float s = 0.f;
+    for (int k = lane; k < N; k += kWave) s = fmaf(load_scalar<TB>(y + k), load_scalar<TB>(x + k), s);
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) s += __shfl_xor(s, m);
+    if (lane == 0) out[e] = s;
+  }
+}
+
+template <typename TB, int G>
+static void launch_sddmm(int M, const int *rowptr, const int *colind, const TB *dY, int lddy,
+                         const TB *X, int ldx, float *out, int ntile, hipStream_t s) {
+  hipLaunchKernelGGL((sddmm_kernel<TB, G>), dim3(ceil_div(M, kAdjThreads / G)),
+                     dim3(kAdjThreads), 0, s, M, rowptr, colind, dY, lddy, X, ldx, out, ntile);
+}
+
+template <typename TB>
+static int run_sddmm(int M, int K, int N, int nnz, const int *rowptr, const int *colind,
+                     const TB *dY, int lddy, const TB *X, int ldx, float *out, void *stream) {
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0 && nnz >= 0, "negative size");
+  if (M == 0 || nnz == 0) return SPMM_OK;
+  SPMM_CHECK_ARG(rowptr && colind && out, "null rowptr, colind or out");
+  SPMM_CHECK_ARG(N == 0 || (dY && X && lddy >= N && ldx >= N), "null operand or ld < N");
+  hipStream_t s = as_stream(stream);
+  if (N == 0) {
+    SPMM_HIP(hipMemsetAsync(out, 0, sizeof(float) * static_cast<size_t>(nnz), s));
+    return SPMM_OK;
+  }
+  const int e16 = 16 / static_cast<int>(sizeof(TB));
+  const bool vec = aligned16(dY) && aligned16(X) && lddy % e16 == 0 && ldx % e16 == 0 &&
+                   N % e16 == 0;
+  int G = 0;
+  if (vec) {
+    const int lanes = N / e16;
+    for (int t = 64; t >= 1; t >>= 1)
+      if (t <= lanes && lanes % t == 0) { G = t; break; }
+  }
+  if (G == 0) {
+    hipLaunchKernelGGL((sddmm_generic_kernel<TB>), dim3(ceil_div(M, 4)), dim3(kAdjThreads), 0, s,
+                       M, N, rowptr, colind, dY, lddy, X, ldx, out);
+  } else {
+    const int ntile = N / (G * e16);
+    switch (G) {
+      case 1: launch_sddmm<TB, 1>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 2: launch_sddmm<TB, 2>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 4: launch_sddmm<TB, 4>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 8: launch_sddmm<TB, 8>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 16: launch_sddmm<TB, 16>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 32: launch_sddmm<TB, 32>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      default: launch_sddmm<TB, 64>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+    }
+  }
+  SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
+}  // namespace spmm
+
+using namespace spmm;
+
+extern "C" {
+
+int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const int *colind,
+                  const float *val, int *tRowPtr, int *tColInd, float *tVal, int *perm,
+                  void *stream) {
+  SPMM_CHECK_ARG(h && tRowPtr, "null handle or tRowPtr");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && nnz >= 0, "negative size");
+  SPMM_CHECK_ARG(M == 0 || rowptr, "null rowptr");
+  SPMM_CHECK_ARG(nnz == 0 || (colind && tColInd), "null colind or tColInd");
+  SPMM_CHECK_ARG(!(tVal && !val && nnz), "tVal requested without val");
+  hipStream_t s = as_stream(stream);
+  // Workspace: cursor[K+1] | scan partials | pos[nnz] | rowidx[nnz]
+  const size_t a0 = 0;
+  const size_t a1 = (a0 + sizeof(int) * (static_cast<size_t>(K) + 1) + 255) & ~size_t(255);
+  const size_t a2 = (a1 + scan_workspace_bytes(K + 1) + 255) & ~size_t(255);
+  const size_t a3 = (a2 + sizeof(int) * static_cast<size_t>(nnz) + 255) & ~size_t(255);
+  const size_t total = a3 + sizeof(int) * static_cast<size_t>(nnz) + 256;
+  char *ws = static_cast<char *>(workspace(h, total));
+  if (!ws) return SPMM_ERR_ALLOC;
+  int *cursor = reinterpret_cast<int *>(ws + a0);
+  int *part = reinterpret_cast<int *>(ws + a1);
+  int *pos = reinterpret_cast<int *>(ws + a2);
+  int *rowidx = reinterpret_cast<int *>(ws + a3);
+
+  SPMM_HIP(hipMemsetAsync(tRowPtr, 0, sizeof(int) * (static_cast<size_t>(K) + 1), s));
+  const int grid = std::max(1, std::min(ceil_div(nnz, kAdjThreads), 4096));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(col_count_kernel, dim3(grid), dim3(kAdjThreads), 0, s, nnz, colind,
+                       tRowPtr);
+    SPMM_LAUNCHED();
+  }
+  SPMM_HIP(exclusive_scan(tRowPtr, K + 1, part, s));
+  if (nnz == 0) return SPMM_OK;
+  SPMM_HIP(hipMemcpyAsync(cursor, tRowPtr, sizeof(int) * (static_cast<size_t>(K) + 1),
+                          hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid), dim3(kAdjThreads), 0, s, nnz, colind, cursor, pos);
+  SPMM_LAUNCHED();
+  if (K > 0) {
+    hipLaunchKernelGGL(segment_sort_kernel, dim3(K), dim3(kAdjThreads), 0, s, tRowPtr, pos);
+    SPMM_LAUNCHED();
+  }
+  hipLaunchKernelGGL(row_of_entry_kernel, dim3(ceil_div(M, 4)), dim3(kAdjThreads), 0, s, M, rowptr,
+                     rowidx);
+  SPMM_LAUNCHED();
+  hipLaunchKernelGGL(gather_kernel, dim3(grid), dim3(kAdjThreads), 0, s, nnz, pos, rowidx, val,
+                     tColInd, tVal, perm);
+  SPMM_LAUNCHED();
+  // The workspace is reused by the next call on this handle.
+  SPMM_HIP(hipStreamSynchronize(s));
+  return SPMM_OK;
+}
+
+int sddmm_csr_f32(spmm_handle_t h, int M, int K, int N, int nnz, const int *rowptr,
+                  const int *colind, const float *dY, int lddy, const float *X, int ldx,
+                  float *dval, void *stream) {
+  (void)h;
+  return run_sddmm<float>(M, K, N, nnz, rowptr, colind, dY, lddy, X, ldx, dval, stream);
+}
+
+int sddmm_csr_bf16(spmm_handle_t h, int M, int K, int N, int nnz, const int *rowptr,
+                   const int *colind, const uint16_t *dY, int lddy, const uint16_t *X, int ldx,
+                   float *dval, void *stream) {
+  (void)h;
+  return run_sddmm<uint16_t>(M, K, N, nnz, rowptr, colind, dY, lddy, X, ldx, dval, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,542 @@
+// B2SR (bit-block sparse row): device CSR -> B2SR-T conversion, the FBF
+// SpMM C = A_bin * B (FP32 or BF16 B, FP32 out), the binary x sign-packed
+// popcount SpMM (int32 out) and the dense 32-bit packers.
+//
+// Layout (the reference's, csr2bsr_batch_bsrbmv.cu:207-1070 with
+// spmm.cu:165-335): bsrRowPtr[MB+1], bsrColInd[nb] ascending per block row,
+// word bsrVal[T*b + r] has bit (T-1-c) set iff A[T*br+r][T*bc+c] is stored.
+//
+// Conversion: 64/T block rows per wave, one lane per CSR row of the block
+// row; the T lanes of a group merge their sorted rows (group minimum of the
+// current block column via xor-shuffles), OR their bits for that block
+// column and emit one word each. One pass counts, a device scan makes
+// bsrRowPtr, a second pass fills. No host loop, no per-block-row transfers
+// (the reference does ~10 host<->device round trips per block row,
+// csr2bsr_batch_bsrbmv.cu:236-379).
+//
+// FBF SpMM: one subgroup of G lanes per output row (G lanes x 16-byte
+// vectors span the B row segment). For each chunk of G tiles, lane l reads
+// tile l's word for the subgroup's row, the set bits are counted, prefix
+// summed across the subgroup and decoded into an LDS list of B row indices
+// (ascending column order), and the subgroup then streams that list with
+// several 16-byte B gathers in flight per lane, as the CSR kernel does.
+#include "common.hpp"
+
+#include <climits>
+
+namespace spmm {
+
+template <int T> struct Word;
+template <> struct Word<4> { using type = uint8_t; };
+template <> struct Word<8> { using type = uint8_t; };
+template <> struct Word<16> { using type = uint16_t; };
+template <> struct Word<32> { using type = uint32_t; };
+
+constexpr int kB2Threads = 256;
+constexpr int kB2Waves = kB2Threads / kWave;
+
+static bool valid_T(int T) { return T == 4 || T == 8 || T == 16 || T == 32; }
+
+// ------------------------------------------------------------ conversion
+template <int T, bool kFill>
+__global__ void __launch_bounds__(kB2Threads)
+csr2b2sr_kernel(int M, int K, const int *__restrict__ rowptr, const int *__restrict__ colind,
+                int *__restrict__ bsrRowPtr, int *__restrict__ bsrColInd,
+                typename Word<T>::type *__restrict__ bsrVal, int *__restrict__ err) {
+  using W = typename Word<T>::type;
+  constexpr int GPW = kWave / T;  // block rows per wave
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave_g = (blockIdx.x * kB2Threads + threadIdx.x) / kWave;
+  const int br = wave_g * GPW + lane / T;
+  const int r = lane % T;
+  const int MB = (M + T - 1) / T;
+  const bool live = br < MB;
+  const int row = br * T + r;
+  int pos = 0, end = 0;
+  if (live && row < M) {
+    pos = rowptr[row];
+    end = rowptr[row + 1];
+  }
+  int out = 0;
+  if (kFill && live) out = bsrRowPtr[br];
+  int count = 0, prev = -1, bad = 0;
+  while (true) {
+    int cur = INT_MAX;
+    if (pos < end) {
+      const int c = colind[pos];
+      cur = (c >= 0 && c < K) ? c / T : INT_MAX - 1;
+    }
+    int m = cur;
+#pragma unroll
+    for (int d = 1; d < T; d <<= 1) m = min(m, __shfl_xor(m, d));
+    if (__all(m == INT_MAX)) break;
+    if (m != INT_MAX) {
+      uint32_t bits = 0;
+      while (pos < end) {
+        const int c = colind[pos];
+        if (c < 0 || c >= K || c < prev) { bad = 1; pos = end; break; }
+        if (c / T != m) break;
+        bits |= 1u << (T - 1 - (c % T));
+        prev = c;
+        ++pos;
+      }
+      if (m == INT_MAX - 1) { bad = 1; pos = end; }
+      if (kFill && m < INT_MAX - 1) {
+        if (r == 0) bsrColInd[out + count] = m;
+        bsrVal[static_cast<size_t>(out + count) * T + r] = static_cast<W>(bits);
+      }
+      ++count;
+    }
+  }
+  if (!kFill && live && r == 0) bsrRowPtr[br] = count;
+  if (bad) atomicOr(err, 1);
+}
+
+// --------------------------------------------------- decode to LDS list
+// The subgroup's tiles [t, t+n) for row r: writes the B row indices of the
+// set bits, tile by tile and ascending within a tile, to list[0..total).
+template <int T, int G>
+__device__ __forceinline__ int decode_chunk(int t, int n, int r, int lane_sub,
+                                            const int *__restrict__ bci,
+                                            const typename Word<T>::type *__restrict__ words,
+                                            int *list) {
+  int bc = 0;
+  uint32_t w = 0;
+  if (lane_sub < n) {
+    bc = bci[t + lane_sub];
+    w = words[static_cast<size_t>(t + lane_sub) * T + r];
+  }
+  const int cnt = __popc(w);
+  int inc = cnt;
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int v = __shfl_up(inc, d, G);
+    if (lane_sub >= d) inc += v;
+  }
+  const int total = __shfl(inc, G - 1, G);
+  int off = inc - cnt;
+  const int base = bc * T + T - 1;
+  while (w) {
+    const int p = 31 - __clz(w);
+    w ^= 1u << p;
+    list[off++] = base - p;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return total;
+}
+
+template <int T, typename TB, int G, int VEC>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_spmm_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci,
+                 const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
+                 int ldb, float *__restrict__ C, int ldc) {
+  constexpr int EPL = VEC * Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * EPL;
+  constexpr int U0 = EPL >= 16 ? 4 : 8;
+  constexpr int U = U0;
+  __shared__ int lds[kB2Waves * S][G * T];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1);
+  const int gsub = wave * S + lane / G;
+  const int row = blockIdx.x * (kB2Waves * S) + gsub;
+  if (row >= M) return;
+  const int br = row / T, r = row % T;
+  const int col0 = blockIdx.y * CW + lane_sub * EPL;
+  const TB *Bc = B + col0;
+  int *list = lds[gsub];
+
+  float acc[EPL];
+#pragma unroll
+  for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+  const int t1 = brp[br + 1];
+  for (int t = brp[br]; t < t1; t += G) {
+    const int total = decode_chunk<T, G>(t, min(G, t1 - t), r, lane_sub, bci, words, list);
+    for (int k = 0; k < total; k += U) {
+      float b[U][EPL];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (k + u < total) {
+          const TB *p = Bc + static_cast<size_t>(list[k + u]) * ldb;
+#pragma unroll
+          for (int q = 0; q < VEC; ++q)
+            Vec16<TB>::load(p + q * Vec16<TB>::kElems, &b[u][q * Vec16<TB>::kElems]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (k + u < total) {
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) acc[q] += b[u][q];
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // list is rewritten by the next chunk
+  }
+  float *Crow = C + static_cast<size_t>(row) * ldc + col0;
+#pragma unroll
+  for (int q = 0; q < EPL; q += 4)
+    *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
+}
+
+// Any N / alignment: one wave per output row, lanes stride over columns.
+template <int T, typename TB>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_spmm_generic_kernel(int M, int N, const int *__restrict__ brp, const int *__restrict__ bci,
+                         const typename Word<T>::type *__restrict__ words,
+                         const TB *__restrict__ B, int ldb, float *__restrict__ C, int ldc) {
+  const int row = blockIdx.x * kB2Waves + threadIdx.x / kWave;
+  if (row >= M) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int br = row / T, r = row % T;
+  for (int c = lane; c < N; c += kWave) {
+    float acc = 0.f;
+    for (int t = brp[br]; t < brp[br + 1]; ++t) {
+      uint32_t w = words[static_cast<size_t>(t) * T + r];
+      const int base = bci[t] * T + T - 1;
+      while (w) {
+        const int p = 31 - __clz(w);
+        w ^= 1u << p;
+        acc += load_scalar<TB>(B + static_cast<size_t>(base - p) * ldb + c);
+      }
+    }
+    C[static_cast<size_t>(row) * ldc + c] = acc;
+  }
+}
+
+// Binary A x sign-packed B -> int32. Lane owns 4 consecutive columns and
+// counts the +1 bits; C = 2 * count - (number of stored entries in the row).
+template <int T, int G>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restrict__ bci,
+                 const typename Word<T>::type *__restrict__ words,
+                 const uint32_t *__restrict__ Bw, int ldbw, int32_t *__restrict__ C, int ldc) {
+  constexpr int S = kWave / G;
+  constexpr int U = 8;
+  __shared__ int lds[kB2Waves * S][G * T];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1);
+  const int gsub = wave * S + lane / G;
+  const int row = blockIdx.x * (kB2Waves * S) + gsub;
+  if (row >= M) return;
+  const int br = row / T, r = row % T;
+  const int col0 = (blockIdx.y * G + lane_sub) * 4;
+  const bool has_cols = col0 < N;
+  const int wi = has_cols ? col0 / 32 : 0;
+  const int sh = 28 - (col0 % 32);  // bits (31-c%32) .. for 4 columns
+  int *list = lds[gsub];
+  int cnt[4] = {0, 0, 0, 0};
+  int deg = 0;
+  const int t1 = brp[br + 1];
+  for (int t = brp[br]; t < t1; t += G) {
+    const int total = decode_chunk<T, G>(t, min(G, t1 - t), r, lane_sub, bci, words, list);
+    deg += total;
+    for (int k = 0; k < total; k += U) {
+      uint32_t wv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        wv[u] = (k + u < total && has_cols) ? Bw[static_cast<size_t>(list[k + u]) * ldbw + wi] : 0u;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t nib = (wv[u] >> sh) & 0xfu;  // columns col0..col0+3 at bits 3..0
+        cnt[0] += (nib >> 3) & 1u;
+        cnt[1] += (nib >> 2) & 1u;
+        cnt[2] += (nib >> 1) & 1u;
+        cnt[3] += nib & 1u;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  int32_t *Crow = C + static_cast<size_t>(row) * ldc;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (col0 + q < N) Crow[col0 + q] = 2 * cnt[q] - deg;
+}
+
+// ------------------------------------------------------- dense packers
+// One wave per (row, 64-column chunk): a 64-bit ballot gives two words.
+// cols == false: out[h][w] (row-major, Wc words per row);
+// cols == true : out[w][h] over 32*ceil(H/32) rows (padding rows are 0).
+template <bool kSign, bool kCols>
+__global__ void __launch_bounds__(kB2Threads)
+pack_kernel(int H, int W, const float *__restrict__ X, int ldx, uint32_t *__restrict__ out) {
+  const int Wc = (W + 31) / 32;
+  const int Hp = kCols ? 32 * ((H + 31) / 32) : H;
+  const int chunks = (W + 63) / 64;
+  const long long item = (static_cast<long long>(blockIdx.x) * kB2Threads + threadIdx.x) / kWave;
+  if (item >= static_cast<long long>(Hp) * chunks) return;
+  const int h = static_cast<int>(item / chunks), j = static_cast<int>(item % chunks);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int col = 64 * j + lane;
+  bool pred = false;
+  if (h < H && col < W) {
+    const float x = X[static_cast<size_t>(h) * ldx + col];
+    pred = kSign ? (x >= 0.f) : (x > 0.f);
+  }
+  const unsigned long long m = __ballot(pred);
+  const int w = 2 * j + lane;
+  if (lane < 2 && w < Wc) {
+    const uint32_t word = __brev(static_cast<uint32_t>(lane == 0 ? m : (m >> 32)));
+    if (kCols)
+      out[static_cast<size_t>(w) * Hp + h] = word;
+    else
+      out[static_cast<size_t>(h) * Wc + w] = word;
+  }
+}
+
+__global__ void __launch_bounds__(kB2Threads)
+unpack_cols_kernel(int H, int W, const uint32_t *__restrict__ in, float *__restrict__ X, int ldx) {
+  const long long i = static_cast<long long>(blockIdx.x) * kB2Threads + threadIdx.x;
+  if (i >= static_cast<long long>(H) * W) return;
+  const int h = static_cast<int>(i / W), c = static_cast<int>(i % W);
+  const int Hp = 32 * ((H + 31) / 32);
+  const uint32_t v = in[static_cast<size_t>(c / 32) * Hp + h];
+  X[static_cast<size_t>(h) * ldx + c] = static_cast<float>((v >> (31 - c % 32)) & 1u);
+}
+
+// ------------------------------------------------------------------ host
+template <int T, bool kFill>
+static void launch_convert(int M, int K, const int *rowptr, const int *colind, int *brp,
+                           int *bci, void *val, int *err, hipStream_t s) {
+  const int MB = (M + T - 1) / T;
+  const int waves = ceil_div(MB, kWave / T);
+  const int blocks = ceil_div(waves, kB2Waves);
+  hipLaunchKernelGGL((csr2b2sr_kernel<T, kFill>), dim3(blocks), dim3(kB2Threads), 0, s, M, K,
+                     rowptr, colind, brp, bci, static_cast<typename Word<T>::type *>(val), err);
+}
+
+template <bool kFill>
+static void convert(int T, int M, int K, const int *rowptr, const int *colind, int *brp,
+                    int *bci, void *val, int *err, hipStream_t s) {
+  switch (T) {
+    case 4: launch_convert<4, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
+    case 8: launch_convert<8, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
+    case 16: launch_convert<16, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
+    default: launch_convert<32, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
+  }
+}
+
+struct Geo {
+  int G = 0, VEC = 1, tiles = 1;
+};
+
+static Geo geometry(int N, int elem_bytes, const void *B, int ldb, const void *C, int ldc) {
+  Geo g;
+  const int e16 = 16 / elem_bytes;
+  if (!(aligned16(B) && aligned16(C) && ldb % e16 == 0 && ldc % 4 == 0) || N % e16 != 0) return g;
+  const int lanes = N / e16;
+  for (int t = 64; t >= 1; t >>= 1)
+    if (t <= lanes && lanes % t == 0) { g.G = t; break; }
+  g.tiles = N / (g.G * e16);
+  return g;
+}
+
+template <int T, typename TB, int G>
+static void launch_fbf(int M, const int *brp, const int *bci, const void *val, const TB *B,
+                       int ldb, float *C, int ldc, int tiles, hipStream_t s) {
+  constexpr int S = kWave / G;
+  dim3 grid(ceil_div(M, kB2Waves * S), tiles);
+  hipLaunchKernelGGL((b2sr_spmm_kernel<T, TB, G, 1>), grid, dim3(kB2Threads), 0, s, M, brp, bci,
+                     static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc);
+}
+
+template <int T, typename TB>
+static void dispatch_fbf(const Geo &g, int M, int N, const int *brp, const int *bci,
+                         const void *val, const TB *B, int ldb, float *C, int ldc,
+                         hipStream_t s) {
+  switch (g.G) {
+    case 0:
+      hipLaunchKernelGGL((b2sr_spmm_generic_kernel<T, TB>), dim3(ceil_div(M, kB2Waves)),
+                         dim3(kB2Threads), 0, s, M, N, brp, bci,
+                         static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc);
+      break;
+    case 1: launch_fbf<T, TB, 1>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 2: launch_fbf<T, TB, 2>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 4: launch_fbf<T, TB, 4>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 8: launch_fbf<T, TB, 8>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 16: launch_fbf<T, TB, 16>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 32: launch_fbf<T, TB, 32>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    default: launch_fbf<T, TB, 64>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+  }
+}
+
+template <typename TB>
+static int run_fbf(int T, int M, int K, int N, const int *brp, const int *bci, const void *val,
+                   const TB *B, int ldb, float *C, int ldc, void *stream) {
+  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
+  if (M == 0 || N == 0) return SPMM_OK;
+  SPMM_CHECK_ARG(brp && C, "null bsrRowPtr or C");
+  SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldb >= N), "ldb/ldc smaller than N");
+  SPMM_CHECK_ARG(K == 0 || B, "null B");
+  hipStream_t s = as_stream(stream);
+  Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
+  switch (T) {
+    case 4: dispatch_fbf<4, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 8: dispatch_fbf<8, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 16: dispatch_fbf<16, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, s); break;
+    default: dispatch_fbf<32, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, s); break;
+  }
+  SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
+template <int T, int G>
+static void launch_sign(int M, int N, const int *brp, const int *bci, const void *val,
+                        const uint32_t *Bw, int ldbw, int32_t *C, int ldc, hipStream_t s) {
+  constexpr int S = kWave / G;
+  dim3 grid(ceil_div(M, kB2Waves * S), ceil_div(N, 4 * G));
+  hipLaunchKernelGGL((b2sr_sign_kernel<T, G>), grid, dim3(kB2Threads), 0, s, M, N, brp, bci,
+                     static_cast<const typename Word<T>::type *>(val), Bw, ldbw, C, ldc);
+}
+
+template <int T>
+static void dispatch_sign(int M, int N, const int *brp, const int *bci, const void *val,
+                          const uint32_t *Bw, int ldbw, int32_t *C, int ldc, hipStream_t s) {
+  const int lanes = ceil_div(N, 4);
+  if (lanes <= 4) launch_sign<T, 4>(M, N, brp, bci, val, Bw, ldbw, C, ldc, s);
+  else if (lanes <= 8) launch_sign<T, 8>(M, N, brp, bci, val, Bw, ldbw, C, ldc, s);
+  else if (lanes <= 16) launch_sign<T, 16>(M, N, brp, bci, val, Bw, ldbw, C, ldc, s);
+  else if (lanes <= 32) launch_sign<T, 32>(M, N, brp, bci, val, Bw, ldbw, C, ldc, s);
+  else launch_sign<T, 64>(M, N, brp, bci, val, Bw, ldbw, C, ldc, s);
+}
+
+}  // namespace spmm
+
+using namespace spmm;
+
+extern "C" {
+
+int csr2b2sr_nnzb(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const int *colind,
+                  int T, int *bsrRowPtr, int *nnzb, void *stream) {
+  SPMM_CHECK_ARG(h && bsrRowPtr && nnzb, "null handle, bsrRowPtr or nnzb");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && nnz >= 0, "negative size");
+  SPMM_CHECK_ARG(M == 0 || rowptr, "null rowptr");
+  SPMM_CHECK_ARG(nnz == 0 || colind, "null colind");
+  hipStream_t s = as_stream(stream);
+  const int MB = (M + T - 1) / T;
+  const size_t part_bytes = scan_workspace_bytes(MB + 1);
+  char *ws = static_cast<char *>(workspace(h, 256 + part_bytes));
+  int *host = static_cast<int *>(pinned(h, 2 * sizeof(int)));
+  if (!ws || !host) return SPMM_ERR_ALLOC;
+  int *err = reinterpret_cast<int *>(ws);
+  int *part = reinterpret_cast<int *>(ws + 256);
+  SPMM_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+  SPMM_HIP(hipMemsetAsync(bsrRowPtr + MB, 0, sizeof(int), s));
+  if (MB > 0) {
+    convert<false>(T, M, K, rowptr, colind, bsrRowPtr, nullptr, nullptr, err, s);
+    SPMM_LAUNCHED();
+  }
+  SPMM_HIP(exclusive_scan(bsrRowPtr, MB + 1, part, s));
+  SPMM_HIP(hipMemcpyAsync(host, bsrRowPtr + MB, sizeof(int), hipMemcpyDeviceToHost, s));
+  SPMM_HIP(hipMemcpyAsync(host + 1, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  SPMM_HIP(hipStreamSynchronize(s));
+  if (host[1])
+    SPMM_FAIL(SPMM_ERR_INVALID_ARG,
+              "CSR column indices must lie in [0, K) and ascend within each row");
+  *nnzb = host[0];
+  return SPMM_OK;
+}
+
+int csr2b2sr(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const int *colind, int T,
+             const int *bsrRowPtr, int *bsrColInd, void *bsrVal, void *stream) {
+  SPMM_CHECK_ARG(h && bsrRowPtr, "null handle or bsrRowPtr");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && nnz >= 0, "negative size");
+  SPMM_CHECK_ARG(M == 0 || rowptr, "null rowptr");
+  SPMM_CHECK_ARG(nnz == 0 || (colind && bsrColInd && bsrVal), "null colind or outputs");
+  if (M == 0 || nnz == 0) return SPMM_OK;
+  hipStream_t s = as_stream(stream);
+  int *err = static_cast<int *>(workspace(h, 256));
+  if (!err) return SPMM_ERR_ALLOC;
+  convert<true>(T, M, K, rowptr, colind, const_cast<int *>(bsrRowPtr), bsrColInd, bsrVal, err, s);
+  SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
+int spmm_b2sr_f32(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,
+                  const int *bsrColInd, const void *bsrVal, const float *B, int ldb, float *C,
+                  int ldc, void *stream) {
+  (void)h;
+  return run_fbf<float>(T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc, stream);
+}
+
+int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,
+                   const int *bsrColInd, const void *bsrVal, const uint16_t *B, int ldb,
+                   float *C, int ldc, void *stream) {
+  (void)h;
+  return run_fbf<uint16_t>(T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc, stream);
+}
+
+int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,
+                             const int *bsrColInd, const void *bsrVal, const uint32_t *Bw,
+                             int ldbw, int32_t *C, int ldc, void *stream) {
+  (void)h;
+  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
+  if (M == 0 || N == 0) return SPMM_OK;
+  SPMM_CHECK_ARG(bsrRowPtr && C, "null bsrRowPtr or C");
+  SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldbw >= (N + 31) / 32), "ldbw/ldc too small");
+  SPMM_CHECK_ARG(K == 0 || Bw, "null Bw");
+  hipStream_t s = as_stream(stream);
+  switch (T) {
+    case 4: dispatch_sign<4>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
+    case 8: dispatch_sign<8>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
+    case 16: dispatch_sign<16>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
+    default: dispatch_sign<32>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
+  }
+  SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
+static int run_pack(bool sign, bool cols, int H, int W, const float *X, int ldx, uint32_t *out,
+                    void *stream) {
+  SPMM_CHECK_ARG(H >= 0 && W >= 0, "negative size");
+  if (H == 0 || W == 0) return SPMM_OK;
+  SPMM_CHECK_ARG(X && out && ldx >= W, "null pointer or ldx < W");
+  const int Hp = cols ? 32 * ceil_div(H, 32) : H;
+  const long long items = static_cast<long long>(Hp) * ceil_div(W, 64);
+  const long long blocks = ceil_div_ll(items, kB2Waves);
+  SPMM_CHECK_ARG(blocks < (1ll << 31), "matrix too large");
+  hipStream_t s = as_stream(stream);
+  dim3 grid(static_cast<unsigned>(blocks));
+  if (sign && cols)
+    hipLaunchKernelGGL((pack_kernel<true, true>), grid, dim3(kB2Threads), 0, s, H, W, X, ldx, out);
+  else if (sign)
+    hipLaunchKernelGGL((pack_kernel<true, false>), grid, dim3(kB2Threads), 0, s, H, W, X, ldx, out);
+  else
+    hipLaunchKernelGGL((pack_kernel<false, false>), grid, dim3(kB2Threads), 0, s, H, W, X, ldx, out);
+  SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
+int pack_sign_rows_u32(int H, int W, const float *X, int ldx, uint32_t *out, void *stream) {
+  return run_pack(true, false, H, W, X, ldx, out, stream);
+}
+
+int pack_bin_rows_u32(int H, int W, const float *X, int ldx, uint32_t *out, void *stream) {
+  return run_pack(false, false, H, W, X, ldx, out, stream);
+}
+
+int pack_sign_cols_u32(int H, int W, const float *X, int ldx, uint32_t *out, void *stream) {
+  return run_pack(true, true, H, W, X, ldx, out, stream);
+}
+
+int unpack_cols_u32_f32(int H, int W, const uint32_t *in, float *X, int ldx, void *stream) {
+  SPMM_CHECK_ARG(H >= 0 && W >= 0, "negative size");
+  if (H == 0 || W == 0) return SPMM_OK;
+  SPMM_CHECK_ARG(in && X && ldx >= W, "null pointer or ldx < W");
+  const long long n = static_cast<long long>(H) * W;
+  hipLaunchKernelGGL(unpack_cols_kernel, dim3(static_cast<unsigned>(ceil_div_ll(n, kB2Threads))),
+                     dim3(kB2Threads), 0, as_stream(stream), H, W, in, X, ldx);
+  SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
+}  // extern "C"

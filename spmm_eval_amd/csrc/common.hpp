@@ -1,0 +1,119 @@
+// Shared internals of libspmm_mi355x: status plumbing, the handle and its
+// grow-only device workspace, and small device helpers used by every kernel
+// file. gfx950 only: wave64, 16-byte global loads, 64-bit ballot.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+#include "../../include/spmm_mi355x.h"
+
+namespace spmm {
+
+constexpr int kWave = 64;
+
+// Thread-local message of the last failing call (spmm_last_error()).
+void set_error(const std::string &msg);
+
+#define SPMM_FAIL(code, msg)                                                   \
+  do {                                                                         \
+    ::spmm::set_error(std::string(__func__) + ": " + (msg));                   \
+    return (code);                                                             \
+  } while (0)
+
+#define SPMM_CHECK_ARG(cond, msg)                                              \
+  do {                                                                         \
+    if (!(cond)) SPMM_FAIL(SPMM_ERR_INVALID_ARG, msg);                         \
+  } while (0)
+
+#define SPMM_HIP(call)                                                         \
+  do {                                                                         \
+    hipError_t e__ = (call);                                                   \
+    if (e__ != hipSuccess)                                                     \
+      SPMM_FAIL(SPMM_ERR_HIP,                                                  \
+                std::string(#call) + " -> " + hipGetErrorString(e__));         \
+  } while (0)
+
+// Checks the launch just made (configuration errors surface here).
+#define SPMM_LAUNCHED() SPMM_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline bool aligned16(const void *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+inline long long ceil_div_ll(long long a, long long b) { return (a + b - 1) / b; }
+
+}  // namespace spmm
+
+// Handle: owns one grow-only device workspace used by the synchronous
+// helper paths (analysis, conversions, transposes). Not thread-safe; use one
+// handle per host thread.
+struct spmm_handle_s {
+  int device = 0;
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+  void *pinned = nullptr;  // small pinned host buffer for count read-backs
+  size_t pinned_bytes = 0;
+};
+
+namespace spmm {
+// Returns a workspace of at least `bytes` (device) or nullptr with the
+// error already recorded. Growing frees the previous buffer after a device
+// synchronisation, so pointers from earlier calls must not be held.
+void *workspace(spmm_handle_s *h, size_t bytes);
+void *pinned(spmm_handle_s *h, size_t bytes);
+}  // namespace spmm
+
+// ------------------------------------------------------------------ device
+namespace spmm {
+
+__device__ __forceinline__ float bf16_lo(uint32_t u) {
+  return __uint_as_float(u << 16);
+}
+__device__ __forceinline__ float bf16_hi(uint32_t u) {
+  return __uint_as_float(u & 0xffff0000u);
+}
+
+// 16-byte operand fetch: four floats or eight bf16 widened to float.
+template <typename TB> struct Vec16;
+template <> struct Vec16<float> {
+  static constexpr int kElems = 4;
+  __device__ __forceinline__ static void load(const float *p, float *out) {
+    float4 v = *reinterpret_cast<const float4 *>(p);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+  }
+};
+template <> struct Vec16<uint16_t> {
+  static constexpr int kElems = 8;
+  __device__ __forceinline__ static void load(const uint16_t *p, float *out) {
+    uint4 v = *reinterpret_cast<const uint4 *>(p);
+    out[0] = bf16_lo(v.x); out[1] = bf16_hi(v.x);
+    out[2] = bf16_lo(v.y); out[3] = bf16_hi(v.y);
+    out[4] = bf16_lo(v.z); out[5] = bf16_hi(v.z);
+    out[6] = bf16_lo(v.w); out[7] = bf16_hi(v.w);
+  }
+};
+
+template <typename TB>
+__device__ __forceinline__ float load_scalar(const TB *p);
+template <> __device__ __forceinline__ float load_scalar<float>(const float *p) {
+  return *p;
+}
+template <>
+__device__ __forceinline__ float load_scalar<uint16_t>(const uint16_t *p) {
+  return __uint_as_float(static_cast<uint32_t>(*p) << 16);
+}
+
+}  // namespace spmm
+
+namespace spmm {
+// scan.hip: in-place exclusive prefix sum of d[0..n) on stream s;
+// `part` must hold scan_workspace_bytes(n) bytes of device memory.
+size_t scan_workspace_bytes(int n);
+hipError_t exclusive_scan(int *d, int n, int *part, hipStream_t s);
+}  // namespace spmm

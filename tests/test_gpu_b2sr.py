@@ -1,0 +1,159 @@
+"""B2SR parity on the GPU: device csr2b2sr bit-exact against the oracle and
+the hand-derived vectors; FBF SpMM against FP64 (<= 1e-5 relative) and,
+with +-1 B, bit-exact against integer CSR SpMM (the reference's known
+answer, test_spmm_full.cu:412-431); the sign-packed popcount path and the
+dense packers bit-exact."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from spmm_eval_amd import api, graphs
+from spmm_eval_amd._lib import SpmmError
+
+pytestmark = pytest.mark.gpu
+HAND = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "b2sr_hand.json")))
+
+
+def dev_csr(g, dev):
+    return torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+
+
+def np_words(t):
+    return t.cpu().numpy() if t.dtype != torch.uint16 else t.view(torch.int16).cpu().numpy().view(np.uint16)
+
+
+def np_u32(t):
+    return t.view(torch.int32).cpu().numpy().view(np.uint32)
+
+
+def assert_b2sr_equal(dev_out, ref):
+    brp, bci, words = dev_out
+    np.testing.assert_array_equal(brp.cpu().numpy(), ref[0])
+    np.testing.assert_array_equal(bci.cpu().numpy(), ref[1])
+    w = words.view(torch.int32).cpu().numpy().view(np.uint32) if words.dtype == torch.uint32 else np_words(words)
+    np.testing.assert_array_equal(w, ref[2])
+
+
+@pytest.mark.parametrize("T", [4, 8])
+def test_hand_vectors(handle, dev, T):
+    rp = torch.tensor(HAND["rowptr"], dtype=torch.int32, device=dev)
+    ci = torch.tensor(HAND["colind"], dtype=torch.int32, device=dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, HAND["M"], HAND["K"], T)
+    exp = HAND[f"T{T}"]
+    assert brp.cpu().tolist() == exp["bsrRowPtr"]
+    assert bci.cpu().tolist() == exp["bsrColInd"]
+    assert words.cpu().tolist() == exp["bsrVal"]
+    B = torch.tensor(HAND["B"], dtype=torch.float32, device=dev)
+    Bp = torch.zeros(8, 4, device=dev)
+    Bp[:, :2] = B  # N=4 keeps the vector kernel; N=2 the generic one
+    C = api.spmm_b2sr(handle, T, HAND["M"], brp, bci, words, Bp)
+    assert C[:, :2].cpu().tolist() == HAND["C"]
+    C2 = api.spmm_b2sr(handle, T, HAND["M"], brp, bci, words, B.contiguous())
+    assert C2.cpu().tolist() == HAND["C"]
+
+
+@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("shape", [(1, 1), (37, 101), (300, 300), (1000, 777)])
+def test_csr2b2sr_random(handle, dev, T, shape):
+    M, K = shape
+    g = graphs.random_csr(M, K, 0.03, seed=M + T, long_rows=2, long_len=K)
+    ref = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    rp, ci = dev_csr(g, dev)
+    assert_b2sr_equal(api.csr2b2sr(handle, rp, ci, M, K, T), ref)
+
+
+@pytest.mark.parametrize("T", [4, 8, 16, 32])
+def test_csr2b2sr_workload(handle, dev, T):
+    g = graphs.workload_b2sr_graph()
+    ref = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    rp, ci = dev_csr(g, dev)
+    out = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    assert_b2sr_equal(out, ref)
+
+
+def test_csr2b2sr_rejects_unsorted(handle, dev):
+    rp = torch.tensor([0, 2], dtype=torch.int32, device=dev)
+    ci = torch.tensor([5, 1], dtype=torch.int32, device=dev)
+    with pytest.raises(SpmmError) as ei:
+        api.csr2b2sr(handle, rp, ci, 1, 8, 4)
+    assert ei.value.status == 1
+    ci = torch.tensor([1, 9], dtype=torch.int32, device=dev)  # col >= K
+    with pytest.raises(SpmmError):
+        api.csr2b2sr(handle, rp, ci, 1, 8, 4)
+
+
+@pytest.fixture(scope="module")
+def b2sr_graph():
+    return graphs.workload_b2sr_graph()
+
+
+@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("N", [32, 64, 128])
+def test_fbf_workload(handle, dev, b2sr_graph, T, N):
+    g = b2sr_graph
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    # known answer: +-1 B, exact
+    S = np.random.default_rng(N + T).choice([-1, 1], size=(g.K, N)).astype(np.int32)
+    C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(S.astype(np.float32)).to(dev))
+    np.testing.assert_array_equal(C.cpu().numpy(), oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.float32))
+    # float B vs FP64
+    B = graphs.uniform_pm1((g.K, N), seed=N * T)
+    Cf = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(B).to(dev)).cpu().numpy()
+    ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
+    assert np.linalg.norm(Cf - ref) / np.linalg.norm(ref) <= 1e-5
+    # BF16 B
+    Bh = torch.from_numpy(B).to(dev).to(torch.bfloat16).view(torch.uint16)
+    Ch = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bh).cpu().numpy()
+    refq = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, np_words(Bh))
+    assert np.linalg.norm(Ch - refq) / np.linalg.norm(refq) <= 1e-5
+
+
+@pytest.mark.parametrize("T", [4, 16, 32])
+@pytest.mark.parametrize("N", [1, 7, 100])
+def test_fbf_odd_widths(handle, dev, T, N):
+    g = graphs.random_csr(333, 444, 0.02, seed=N, long_rows=1, long_len=444)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    B = graphs.uniform_pm1((g.K, N), seed=N)
+    C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(B).to(dev)).cpu().numpy()
+    ref = oracle.b2sr_spmm_f64(T, g.M, *oracle.csr2b2sr(g.rowptr, g.colind, T), B)
+    assert np.linalg.norm(C - ref) / max(np.linalg.norm(ref), 1e-30) <= 1e-5
+
+
+@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("N", [32, 64, 70, 128])
+def test_sign_packed_popcount(handle, dev, b2sr_graph, T, N):
+    g = b2sr_graph
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    X = graphs.uniform_pm1((g.K, N), seed=T + N)
+    X[::97, 0] = -0.0
+    Xd = torch.from_numpy(X).to(dev)
+    Bw = api.pack_sign_rows(Xd)
+    np.testing.assert_array_equal(np_u32(Bw), oracle.pack_sign_rows(X))
+    C = api.spmm_b2sr_sign(handle, T, g.M, g.K, N, brp, bci, words, Bw).cpu().numpy()
+    S = np.where(X >= 0, 1, -1).astype(np.int32)
+    np.testing.assert_array_equal(C, oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.int32))
+
+
+@pytest.mark.parametrize("H,W", [(1, 1), (5, 31), (33, 32), (100, 100), (64, 129)])
+def test_dense_packers(dev, H, W):
+    X = graphs.uniform_pm1((H, W), seed=H * W)
+    X[0, 0] = 0.0
+    Xd = torch.from_numpy(X).to(dev)
+    np.testing.assert_array_equal(np_u32(api.pack_sign_rows(Xd)), oracle.pack_sign_rows(X))
+    np.testing.assert_array_equal(np_u32(api.pack_bin_rows(Xd)), oracle.pack_bin_rows(X))
+    cols = api.pack_sign_cols(Xd)
+    np.testing.assert_array_equal(np_u32(cols), oracle.pack_sign_cols(X))
+    if H % 32 == 0:
+        np.testing.assert_array_equal(api.unpack_cols(cols, H, W).cpu().numpy(),
+                                      oracle.unpack_cols(np_u32(cols), H, W))
+    sign = HAND["sign_pack"]
+    Xh = torch.tensor(sign["X"], dtype=torch.float32, device=dev)
+    assert np_u32(api.pack_sign_rows(Xh)).reshape(-1).tolist() == sign["sign"]
+    assert np_u32(api.pack_bin_rows(Xh)).reshape(-1).tolist() == sign["bin"]

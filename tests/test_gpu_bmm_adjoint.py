@@ -1,0 +1,163 @@
+"""Binary matmul (bit-exact vs the oracle and the hand vectors), the device
+transpose (bit-exact vs csr2csc), dX = A^T dY and SDDMM (<= 1e-5 relative
+vs FP64), and the adjoint wrapped as a torch.autograd.Function checked
+against FP64 central finite differences on 256 sampled entries."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from spmm_eval_amd import api, graphs
+
+pytestmark = pytest.mark.gpu
+HAND = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "b2sr_hand.json")))
+
+
+def u32(a, dev):
+    return torch.from_numpy(a.view(np.int32)).to(dev).view(torch.uint32)
+
+
+def np_u32(t):
+    return t.view(torch.int32).cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("M,N,Kw", [(1, 1, 1), (32, 32, 1), (64, 128, 4), (100, 70, 7), (256, 512, 33)])
+def test_bmm(dev, M, N, Kw):
+    r = np.random.default_rng(M * N + Kw)
+    A = r.integers(0, 2 ** 32, (Kw, M), dtype=np.uint64).astype(np.uint32)
+    B = r.integers(0, 2 ** 32, (Kw, N), dtype=np.uint64).astype(np.uint32)
+    C = api.bmm32_xnor_f32(u32(A, dev), u32(B, dev), M, N, Kw).cpu().numpy()
+    np.testing.assert_array_equal(C, oracle.bmm32_xnor_f32(A, B, M, N, Kw))
+    for Kbits in (32 * Kw, 32 * Kw - 5 if Kw > 1 else 32):
+        Cb = np_u32(api.bmm32_xnor_bin(u32(A, dev), u32(B, dev), M, N, Kbits))
+        np.testing.assert_array_equal(Cb, oracle.bmm32_xnor_bin(A, B, M, N, Kbits))
+
+
+def test_bmm_hand(dev):
+    for a, b, c in HAND["bmm"]["cases"]:
+        A = np.array([a], np.uint32)
+        B = np.array([b], np.uint32)
+        assert api.bmm32_xnor_f32(u32(A, dev), u32(B, dev), 1, 1, 1).item() == c
+
+
+@pytest.mark.parametrize("M,K", [(1, 1), (40, 25), (500, 300), (2000, 5000)])
+def test_transpose(handle, dev, M, K):
+    g = graphs.random_csr(M, K, 0.01 if M > 100 else 0.2, seed=M, long_rows=2, long_len=K)
+    # a column with > 4096 entries exercises the global-memory sort
+    cp, ri, cv, perm = oracle.csr2csc(g.rowptr, g.colind, g.val, K)
+    out = api.csr_transpose(handle, torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev),
+                            torch.from_numpy(g.val).to(dev), K)
+    np.testing.assert_array_equal(out[0].cpu().numpy(), cp)
+    np.testing.assert_array_equal(out[1].cpu().numpy(), ri)
+    np.testing.assert_array_equal(out[2].cpu().numpy(), cv)
+    np.testing.assert_array_equal(out[3].cpu().numpy(), perm)
+
+
+def test_transpose_long_column(handle, dev):
+    M, K = 6000, 3
+    rowptr = np.arange(M + 1, dtype=np.int32)
+    colind = np.zeros(M, np.int32)  # one column with 6000 entries (> LDS tile)
+    colind[::5] = 2
+    val = np.arange(M, dtype=np.float32)
+    cp, ri, cv, perm = oracle.csr2csc(rowptr, colind, val, K)
+    out = api.csr_transpose(handle, torch.from_numpy(rowptr).to(dev), torch.from_numpy(colind).to(dev),
+                            torch.from_numpy(val).to(dev), K)
+    np.testing.assert_array_equal(out[1].cpu().numpy(), ri)
+    np.testing.assert_array_equal(out[3].cpu().numpy(), perm)
+
+
+class SpmmFn(torch.autograd.Function):
+    """Y = A X with A's values differentiable; backward through the C-ABI
+    (dX = A^T dY over the device transpose, dval = SDDMM(dY, X)). Test
+    harness only."""
+
+    @staticmethod
+    def forward(ctx, val, X, h, rp, ci, trp, tci, perm):
+        ctx.save_for_backward(val, X, rp, ci, trp, tci, perm)
+        ctx.h = h
+        return api.spmm_csr(h, rp, ci, val, X)
+
+    @staticmethod
+    def backward(ctx, dY):
+        val, X, rp, ci, trp, tci, perm = ctx.saved_tensors
+        dY = dY.contiguous()
+        tval = val[perm.long()]
+        dX = api.spmm_csr(ctx.h, trp, tci, tval, dY, K=dY.shape[0])
+        dval = api.sddmm_csr(ctx.h, rp, ci, dY, X)
+        return dval, dX, None, None, None, None, None, None
+
+
+@pytest.fixture(scope="module")
+def gcn():
+    return graphs.workload_gcn(N=128)
+
+
+def test_adjoint_gcn_workload(handle, dev, gcn):
+    g, X, dY = gcn
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    v = torch.from_numpy(g.val).to(dev)
+    trp, tci, tval, perm = api.csr_transpose(handle, rp, ci, v, g.K)
+    Xd, dYd = torch.from_numpy(X).to(dev), torch.from_numpy(dY).to(dev)
+    dX = api.spmm_csr(handle, trp, tci, tval, dYd, K=g.M).cpu().numpy()
+    cp, ri, cv, _ = oracle.csr2csc(g.rowptr, g.colind, g.val, g.K)
+    ref_dX = oracle.csr_spmm_f64(cp, ri, cv, dY)
+    assert np.linalg.norm(dX - ref_dX) / np.linalg.norm(ref_dX) <= 1e-5
+    dval = api.sddmm_csr(handle, rp, ci, dYd, Xd).cpu().numpy()
+    ref_dval = oracle.sddmm_f64(g.rowptr, g.colind, dY, X)
+    assert np.linalg.norm(dval - ref_dval) / np.linalg.norm(ref_dval) <= 1e-5
+    # BF16 features: dX and dval against FP64 of the rounded operands, and
+    # against the FP32 source at 1e-2
+    Xh, dYh = api.to_bf16_bits(Xd), api.to_bf16_bits(dYd)
+    dXh = api.spmm_csr(handle, trp, tci, tval, dYh, K=g.M).cpu().numpy()
+    assert np.linalg.norm(dXh - ref_dX) / np.linalg.norm(ref_dX) <= 1e-2
+    dvalh = api.sddmm_csr(handle, rp, ci, dYh, Xh).cpu().numpy()
+    Xq = oracle.bf16_to_f32(Xh.view(torch.int16).cpu().numpy().view(np.uint16))
+    dYq = oracle.bf16_to_f32(dYh.view(torch.int16).cpu().numpy().view(np.uint16))
+    refq = oracle.sddmm_f64(g.rowptr, g.colind, dYq, Xq)
+    assert np.linalg.norm(dvalh - refq) / np.linalg.norm(refq) <= 1e-5
+    assert np.linalg.norm(dvalh - ref_dval) / np.linalg.norm(ref_dval) <= 1e-2
+
+
+def test_autograd_finite_differences(handle, dev, gcn):
+    """L = <A X, G>. FP64 central differences of L with respect to 256
+    sampled values and 256 sampled X entries (L is linear in each, so the
+    difference quotient is exact up to rounding) against the gradients
+    produced through the C-ABI."""
+    g, X, G = gcn
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    v = torch.from_numpy(g.val).to(dev).requires_grad_(True)
+    Xd = torch.from_numpy(X).to(dev).requires_grad_(True)
+    trp, tci, _, perm = api.csr_transpose(handle, rp, ci, v.detach(), g.K)
+    Y = SpmmFn.apply(v, Xd, handle, rp, ci, trp, tci, perm)
+    (Y * torch.from_numpy(G).to(dev)).sum().backward()
+    gv, gx = v.grad.cpu().numpy(), Xd.grad.cpu().numpy()
+
+    import scipy.sparse as sp
+    A = sp.csr_matrix((g.val.astype(np.float64), g.colind, g.rowptr), shape=(g.M, g.K))
+    X64, G64 = X.astype(np.float64), G.astype(np.float64)
+    r = np.random.default_rng(0)
+    rows = np.repeat(np.arange(g.M), g.degrees())
+    h = 1e-2
+    for e in r.choice(g.nnz, 256, replace=False):
+        i, j = rows[e], g.colind[e]
+        # perturbing val_e changes row i of A X only
+        s, t = g.rowptr[i], g.rowptr[i + 1]
+        cols, vals = g.colind[s:t], g.val[s:t].astype(np.float64)
+        vp, vm = vals.copy(), vals.copy()
+        vp[e - s] += h
+        vm[e - s] -= h
+        fd = ((vp @ X64[cols] - vm @ X64[cols]) @ G64[i]) / (2 * h)
+        assert abs(fd - gv[e]) <= 1e-5 * (np.abs(G64[i] * X64[j]).sum() + 1e-12)
+    AT = A.T.tocsr()
+    for flat in r.choice(g.K * X.shape[1], 256, replace=False):
+        j, k = divmod(int(flat), X.shape[1])
+        Xp, Xm = X64[:, k].copy(), X64[:, k].copy()
+        Xp[j] += h
+        Xm[j] -= h
+        fd = ((A @ Xp - A @ Xm) * G64[:, k]).sum() / (2 * h)
+        ref = (AT[j] @ G64[:, k]).item() if AT[j].nnz else 0.0
+        assert abs(fd - ref) < 1e-9 + 1e-9 * abs(ref)
+        assert abs(fd - gx[j, k]) <= 1e-5 * (np.abs(AT[j]).sum() * np.abs(G64[:, k]).max() + 1e-12)

@@ -1,0 +1,163 @@
+"""CSR SpMM parity on the GPU through the C-ABI, against the FP64 oracle.
+
+Tolerances (stated per the path's contract): FP32 B -> relative Frobenius
+error <= 1e-5 vs FP64 and every element within nnz_row * 2^-23 * sum|a||b|
+(+1e-30); BF16 B -> the same 1e-5 bound against FP64 of the BF16-rounded B
+(kernel exactness), and <= 1e-2 against FP64 of the FP32 source (precision
+form). Values 1 with +-1 B -> bit-exact (the reference's own known-answer
+check, test_spmm_full.cu:412-431).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from spmm_eval_amd import api, graphs
+from spmm_eval_amd._lib import SpmmError
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(g, dev):
+    return (torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev),
+            torch.from_numpy(g.val).to(dev))
+
+
+def rel_fro(C, ref):
+    n = np.linalg.norm(ref)
+    return float(np.linalg.norm(C.astype(np.float64) - ref) / (n if n > 0 else 1.0))
+
+
+def elementwise_bound(g, B):
+    absA = np.abs(g.val).astype(np.float64)
+    rows = np.repeat(np.arange(g.M), g.degrees())
+    # sum_j |a_ij| |b_j| per row and column, scaled by deg * eps
+    import scipy.sparse as sp
+    A = sp.csr_matrix((absA, g.colind, g.rowptr), shape=(g.M, g.K))
+    mag = A @ np.abs(B.astype(np.float64))
+    deg = np.maximum(g.degrees(), 1)[:, None]
+    return deg * 2.0 ** -23 * mag + 1e-30
+
+
+def check_f32(g, B, C):
+    ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
+    assert rel_fro(C, ref) <= 1e-5
+    assert (np.abs(C - ref) <= elementwise_bound(g, B)).all()
+
+
+CASES = [
+    # (M, K, density, long_rows, long_len, N)
+    (1, 1, 1.0, 0, 0, 1),
+    (50, 40, 0.2, 0, 0, 3),       # N not a multiple of 4 -> generic kernel
+    (300, 200, 0.05, 2, 200, 4),
+    (513, 700, 0.02, 1, 700, 8),
+    (1000, 1000, 0.01, 3, 1000, 32),
+    (777, 3000, 0.003, 2, 3000, 64),
+    (400, 5000, 0.002, 3, 5000, 64),  # rows of 5000 -> one workgroup per row
+    (256, 256, 0.05, 1, 256, 96),     # 96 = 3 column tiles of 32
+    (300, 300, 0.03, 1, 300, 128),
+    (200, 200, 0.05, 1, 200, 256),
+    (100, 100, 0.05, 1, 100, 512),
+]
+
+
+@pytest.mark.parametrize("M,K,dens,nl,ll,N", CASES)
+def test_csr_f32_random(handle, dev, M, K, dens, nl, ll, N):
+    g = graphs.random_csr(M, K, dens, seed=M + N, long_rows=nl, long_len=ll)
+    B = graphs.uniform_pm1((K, N), seed=N)
+    rp, ci, v = to_dev(g, dev)
+    C = api.spmm_csr(handle, rp, ci, v, torch.from_numpy(B).to(dev)).cpu().numpy()
+    check_f32(g, B, C)
+
+
+def test_csr_f32_workload_small_and_known_answer(handle, dev):
+    g, B = graphs.workload_csr_small()
+    rp, ci, v = to_dev(g, dev)
+    Bd = torch.from_numpy(B).to(dev)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    C1 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan)
+    C2 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan)
+    C3 = api.spmm_csr(handle, rp, ci, v, Bd)  # drop-in form
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2) and torch.equal(C1, C3)  # deterministic
+    check_f32(g, B, C1.cpu().numpy())
+    # +-1 B with values 1: integer sums, exact (test_spmm_full.cu:412-431)
+    S = np.random.default_rng(7).choice([-1, 1], size=B.shape).astype(np.int32)
+    Cs = api.spmm_csr(handle, rp, ci, v, torch.from_numpy(S.astype(np.float32)).to(dev), plan=plan)
+    exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S)
+    np.testing.assert_array_equal(Cs.cpu().numpy(), exact.astype(np.float32))
+    assert sum(plan.bins()) == g.M
+
+
+@pytest.mark.parametrize("N", [32, 64, 128])
+def test_csr_bf16(handle, dev, N):
+    g = graphs.random_csr(2000, 2500, 0.004, seed=N, long_rows=2, long_len=2500)
+    B = graphs.uniform_pm1((g.K, N), seed=N + 1)
+    Bh = oracle.f32_to_bf16(B)
+    rp, ci, v = to_dev(g, dev)
+    Bd = torch.from_numpy(Bh.view(np.int16)).to(dev).view(torch.uint16)
+    C = api.spmm_csr(handle, rp, ci, v, Bd).cpu().numpy()
+    ref_q = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, Bh)
+    assert rel_fro(C, ref_q) <= 1e-5
+    ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
+    assert rel_fro(C, ref) <= 1e-2
+
+
+@pytest.mark.parametrize("tune", [(4, 1, 1), (8, 2, 1), (16, 1, 2), (32, 2, 4), (64, 1, 1)])
+def test_csr_tune_variants(handle, dev, tune):
+    g = graphs.chung_lu(5000, 80000, seed=3)
+    g.val = graphs.row_normalised_uniform(g, 3)
+    B = graphs.uniform_pm1((g.K, 128), seed=5)
+    rp, ci, v = to_dev(g, dev)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    C = api.spmm_csr(handle, rp, ci, v, torch.from_numpy(B).to(dev), plan=plan, tune=tune)
+    check_f32(g, B, C.cpu().numpy())
+
+
+def test_csr_strided_and_empty(handle, dev):
+    g = graphs.random_csr(300, 200, 0.05, seed=1)
+    rp, ci, v = to_dev(g, dev)
+    Bfull = torch.from_numpy(graphs.uniform_pm1((200, 70), seed=2)).to(dev)
+    B = Bfull[:, :64]  # ldb = 70: not 16-byte aligned rows -> generic kernel
+    Cfull = torch.full((300, 80), 7.0, device=dev)
+    api.spmm_csr(handle, rp, ci, v, B, C_out=Cfull[:, :64])
+    check_f32(g, B.cpu().numpy(), Cfull[:, :64].cpu().numpy())
+    assert (Cfull[:, 64:] == 7.0).all()  # columns past N untouched
+    # nnz = 0: C is zeroed
+    e = graphs.random_csr(10, 10, 0.0, seed=0)
+    rp, ci, v = to_dev(e, dev)
+    C = torch.full((10, 8), 3.0, device=dev)
+    api.spmm_csr(handle, rp, ci, v, torch.ones(10, 8, device=dev), C_out=C)
+    assert (C == 0).all()
+
+
+def test_plan_mismatch_is_an_error(handle, dev):
+    g = graphs.random_csr(100, 100, 0.05, seed=4)
+    rp, ci, v = to_dev(g, dev)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    g2 = graphs.random_csr(101, 100, 0.05, seed=4)
+    rp2, ci2, v2 = to_dev(g2, dev)
+    with pytest.raises(SpmmError) as ei:
+        api.spmm_csr(handle, rp2, ci2, v2, torch.ones(100, 8, device=dev), plan=plan)
+    assert ei.value.status == 5
+
+
+def test_csr_large_workload_properties(handle, dev):
+    """Workload-3 graph at 1/8 of its nnz (same rows, same degree law): FP32
+    and BF16 against FP64, and linearity C(aB1 + B2) = aC(B1) + C(B2)."""
+    g = graphs.workload_large_graph(scale=0.125)
+    N = 64
+    B = graphs.uniform_pm1((g.K, N), seed=11)
+    rp, ci, v = to_dev(g, dev)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    Bd = torch.from_numpy(B).to(dev)
+    C = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan).cpu().numpy()
+    ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
+    assert rel_fro(C, ref) <= 1e-5
+    Bh = api.to_bf16_bits(Bd)
+    Ch = api.spmm_csr(handle, rp, ci, v, Bh, plan=plan).cpu().numpy()
+    assert rel_fro(Ch, ref) <= 1e-2
+    B2 = torch.from_numpy(graphs.uniform_pm1((g.K, N), seed=12)).to(dev)
+    lhs = api.spmm_csr(handle, rp, ci, v, 0.5 * Bd + B2, plan=plan)
+    rhs = 0.5 * api.spmm_csr(handle, rp, ci, v, Bd, plan=plan) + api.spmm_csr(handle, rp, ci, v, B2, plan=plan)
+    assert torch.linalg.norm(lhs - rhs) / torch.linalg.norm(rhs) < 1e-5

@@ -1,0 +1,281 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+Each oracle function is checked against something independent of its own
+formula: hand-derived vectors (tests/golden/b2sr_hand.json), a textbook or
+library routine (numpy dense matmul, scipy.sparse transpose, torch's BF16
+rounding, numpy.packbits), brute force on tiny inputs, and invariants. Each
+check is chosen so that a plausible slip (wrong bit order, transposed
+operand, dropped padding rule, off-by-one in block rows) fails; the
+`planted` tests show that for a few of them.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from spmm_eval_amd.graphs import Csr, chung_lu, random_csr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HAND = json.load(open(os.path.join(HERE, "golden", "b2sr_hand.json")))
+
+
+def dense(g: Csr, binary=False):
+    A = np.zeros((g.M, g.K), np.float64)
+    rows = np.repeat(np.arange(g.M), g.degrees())
+    A[rows, g.colind] = 1.0 if binary else g.val
+    return A
+
+
+def brute_b2sr(A01: np.ndarray, T: int):
+    """Independent formulation: pad the dense 0/1 matrix to whole tiles,
+    cut it into T x T tiles, keep the tiles with any 1 (row-major tile
+    order), and read each tile row as a big-endian bit string."""
+    M, K = A01.shape
+    MB, KB = -(-M // T), -(-K // T)
+    P = np.zeros((MB * T, KB * T), np.uint8)
+    P[:M, :K] = A01
+    tiles = P.reshape(MB, T, KB, T).transpose(0, 2, 1, 3)  # [MB][KB][T][T]
+    nz = tiles.reshape(MB, KB, -1).any(axis=2)
+    rowptr = np.concatenate([[0], np.cumsum(nz.sum(axis=1))]).astype(np.int32)
+    br, bc = np.nonzero(nz)
+    bits = tiles[br, bc]  # [nb][T][T]
+    weights = (1 << np.arange(T - 1, -1, -1, dtype=np.uint64))
+    words = (bits.astype(np.uint64) * weights).sum(axis=2).reshape(-1)
+    return rowptr, bc.astype(np.int32), words
+
+
+# ------------------------------------------------------------ hand vectors
+def hand_csr():
+    rp = np.array(HAND["rowptr"], np.int32)
+    ci = np.array(HAND["colind"], np.int32)
+    return Csr(rp, ci, np.ones(len(ci), np.float32), HAND["M"], HAND["K"])
+
+
+@pytest.mark.parametrize("T", [4, 8])
+def test_csr2b2sr_hand_vectors(T):
+    g = hand_csr()
+    brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    exp = HAND[f"T{T}"]
+    assert brp.tolist() == exp["bsrRowPtr"]
+    assert bci.tolist() == exp["bsrColInd"]
+    assert words.tolist() == exp["bsrVal"]
+
+
+def test_b2sr_spmm_hand_vectors():
+    g = hand_csr()
+    B = np.array(HAND["B"], np.float32)
+    for T in (4, 8):
+        brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+        C = oracle.b2sr_spmm_f64(T, g.M, brp, bci, words, B)
+        assert C.tolist() == HAND["C"]
+
+
+def test_pack_hand_vectors():
+    X = np.array(HAND["sign_pack"]["X"], np.float32)
+    assert oracle.pack_sign_rows(X).reshape(-1).tolist() == HAND["sign_pack"]["sign"]
+    assert oracle.pack_bin_rows(X).reshape(-1).tolist() == HAND["sign_pack"]["bin"]
+
+
+def test_bmm_hand_vectors():
+    for a, b, c in HAND["bmm"]["cases"]:
+        A = np.array([a], np.uint32)
+        B = np.array([b], np.uint32)
+        assert oracle.bmm32_xnor_f32(A, B, 1, 1, 1)[0, 0] == c
+        bit = oracle.bmm32_xnor_bin(A, B, 1, 1, 32)[0, 0] >> 31
+        assert bit == (1 if c >= 0 else 0)
+
+
+# ------------------------------------------------------------------- CSR
+@pytest.mark.parametrize("M,K,N", [(1, 1, 1), (37, 53, 7), (64, 20, 33), (200, 150, 64)])
+def test_csr_spmm_f64_matches_dense(M, K, N):
+    g = random_csr(M, K, 0.1, seed=M * 7 + N, long_rows=2, long_len=K)
+    B = np.random.default_rng(1).uniform(-1, 1, (K, N)).astype(np.float32)
+    C = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
+    ref = dense(g) @ B.astype(np.float64)
+    np.testing.assert_allclose(C, ref, rtol=1e-12, atol=1e-12)
+    # also against scipy's sparse product
+    A = sp.csr_matrix((g.val.astype(np.float64), g.colind, g.rowptr), shape=(M, K))
+    np.testing.assert_allclose(C, A @ B.astype(np.float64), rtol=1e-12, atol=1e-12)
+
+
+def test_planted_transpose_is_caught():
+    g = random_csr(30, 30, 0.2, seed=3, empty_rows=False)
+    B = np.random.default_rng(2).uniform(-1, 1, (30, 5)).astype(np.float32)
+    C = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
+    wrong = dense(g).T @ B  # a transposed operand must not pass the dense check
+    assert not np.allclose(C, wrong)
+
+
+def test_csr_spmm_bf16_and_f32_forms():
+    g = random_csr(120, 90, 0.15, seed=11)
+    B = np.random.default_rng(3).uniform(-1, 1, (90, 24)).astype(np.float32)
+    Bh = oracle.f32_to_bf16(B)
+    C64 = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, oracle.bf16_to_f32(Bh))
+    np.testing.assert_array_equal(oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, Bh), C64)
+    C32 = oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B)
+    ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
+    assert np.linalg.norm(C32 - ref) / np.linalg.norm(ref) < 1e-6
+    C16 = oracle.csr_spmm_bf16_omp(g.rowptr, g.colind, g.val, Bh)
+    assert np.linalg.norm(C16 - C64) / np.linalg.norm(C64) < 1e-6
+
+
+def test_csr_spmm_i64_integer():
+    g = random_csr(80, 60, 0.2, seed=5)
+    B = np.random.default_rng(4).choice([-1, 1], size=(60, 9)).astype(np.int32)
+    C = oracle.csr_spmm_i64(g.rowptr, g.colind, B)
+    np.testing.assert_array_equal(C, (dense(g, binary=True) @ B).astype(np.int64))
+
+
+def test_bf16_rounding_matches_torch():
+    import torch
+
+    rng = np.random.default_rng(0)
+    x = np.concatenate([
+        rng.standard_normal(10000).astype(np.float32),
+        np.array([0.0, -0.0, np.inf, -np.inf, 1e-40, -1e-40, 3.4e38, 1.0 + 2 ** -8,
+                  1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9], np.float32),
+        # exact ties: low 16 bits = 0x8000
+        (np.arange(1, 200, dtype=np.uint32) << 16 | 0x8000).view(np.float32),
+    ])
+    ours = oracle.f32_to_bf16(x)
+    ref = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    np.testing.assert_array_equal(ours, ref)
+    nan = oracle.f32_to_bf16(np.array([np.nan], np.float32))
+    assert np.isnan(oracle.bf16_to_f32(nan))[0]
+
+
+# ---------------------------------------------------------------- B2SR
+@pytest.mark.parametrize("T", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("shape", [(1, 1), (13, 29), (70, 70), (257, 100)])
+def test_csr2b2sr_matches_brute_force(T, shape):
+    M, K = shape
+    g = random_csr(M, K, 0.05, seed=M + K + T, long_rows=1, long_len=K)
+    brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    rb, cb, wb = brute_b2sr(dense(g, binary=True).astype(np.uint8), T)
+    np.testing.assert_array_equal(brp, rb)
+    np.testing.assert_array_equal(bci, cb)
+    np.testing.assert_array_equal(words.astype(np.uint64), wb)
+    # invariants: one bit per stored entry, blocks ascending per block row
+    assert int(np.unpackbits(words.view(np.uint8)).sum()) == g.nnz
+    for br in range(len(brp) - 1):
+        seg = bci[brp[br]:brp[br + 1]]
+        assert (np.diff(seg) > 0).all()
+
+
+def test_planted_bit_order_is_caught():
+    g = hand_csr()
+    _, _, words = oracle.csr2b2sr(g.rowptr, g.colind, 4)
+    lsb_first = np.array([int(f"{w:04b}"[::-1], 2) for w in words])
+    assert lsb_first.tolist() != HAND["T4"]["bsrVal"]
+
+
+@pytest.mark.parametrize("T", [4, 8, 16, 32, 64])
+def test_b2sr_spmm_matches_dense(T):
+    g = chung_lu(300, 3000, seed=T)
+    B = np.random.default_rng(T).uniform(-1, 1, (300, 40)).astype(np.float32)
+    brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    C = oracle.b2sr_spmm_f64(T, g.M, brp, bci, words, B)
+    np.testing.assert_allclose(C, dense(g, binary=True) @ B.astype(np.float64), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("T", [4, 32])
+@pytest.mark.parametrize("N", [1, 31, 64, 70])
+def test_b2sr_sign_matches_integer_csr(T, N):
+    g = chung_lu(200, 2000, seed=N)
+    X = np.random.default_rng(N).uniform(-1, 1, (200, N)).astype(np.float32)
+    X[0, 0] = -0.0  # counts as +1 (x >= 0)
+    Bw = oracle.pack_sign_rows(X)
+    brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    C = oracle.b2sr_spmm_sign_i32(T, g.M, N, brp, bci, words, Bw)
+    S = np.where(X >= 0, 1, -1).astype(np.int32)
+    np.testing.assert_array_equal(C, oracle.csr_spmm_i64(g.rowptr, g.colind, S))
+
+
+# ---------------------------------------------------------------- packers
+@pytest.mark.parametrize("H,W", [(1, 1), (5, 31), (33, 32), (40, 100)])
+def test_packers_match_packbits(H, W):
+    X = np.random.default_rng(H * W).uniform(-1, 1, (H, W)).astype(np.float32)
+    X[X.shape[0] // 2, 0] = 0.0
+    Wc = -(-W // 32)
+    for fn, pred in ((oracle.pack_sign_rows, X >= 0), (oracle.pack_bin_rows, X > 0)):
+        bits = np.zeros((H, Wc * 32), np.uint8)
+        bits[:, :W] = pred
+        ref = np.packbits(bits, axis=1, bitorder="big").view(">u4").astype(np.uint32)
+        np.testing.assert_array_equal(fn(X), ref)
+    cols = oracle.pack_sign_cols(X)
+    Hp = 32 * -(-H // 32)
+    assert cols.shape == (Wc, Hp)
+    np.testing.assert_array_equal(cols[:, :H].T, oracle.pack_sign_rows(X))
+    assert (cols[:, H:] == 0).all()
+    if H % 32 == 0:  # Bit32ColTo's layout assumes whole 32-row groups
+        np.testing.assert_array_equal(oracle.unpack_cols(cols, H, W), (X >= 0).astype(np.float32))
+
+
+# ---------------------------------------------------------------- BMM
+def _unpack_pm1(words):  # [Kw][R] -> [R][32*Kw] of +-1
+    bits = np.unpackbits(np.ascontiguousarray(words.T).astype(">u4").view(np.uint8), axis=1, bitorder="big")
+    return bits.astype(np.int64) * 2 - 1
+
+
+@pytest.mark.parametrize("M,N,Kw", [(1, 1, 1), (32, 64, 3), (70, 33, 5)])
+def test_bmm_matches_pm1_matmul(M, N, Kw):
+    r = np.random.default_rng(M + N)
+    A = r.integers(0, 2 ** 32, (Kw, M), dtype=np.uint64).astype(np.uint32)
+    B = r.integers(0, 2 ** 32, (Kw, N), dtype=np.uint64).astype(np.uint32)
+    ref = _unpack_pm1(A) @ _unpack_pm1(B).T
+    np.testing.assert_array_equal(oracle.bmm32_xnor_f32(A, B, M, N, Kw), ref.astype(np.float32))
+    Kbits = 32 * Kw
+    bits = (ref >= 0).astype(np.uint8)
+    Nw = -(-N // 32)
+    padded = np.zeros((M, Nw * 32), np.uint8)
+    padded[:, :N] = bits
+    ref_bin = np.packbits(padded, axis=1, bitorder="big").view(">u4").astype(np.uint32).T
+    np.testing.assert_array_equal(oracle.bmm32_xnor_bin(A, B, M, N, Kbits), ref_bin)
+
+
+# ------------------------------------------------------------- adjoint
+@pytest.mark.parametrize("M,K", [(1, 1), (40, 25), (150, 300)])
+def test_csr2csc_matches_scipy(M, K):
+    g = random_csr(M, K, 0.1, seed=M + K, long_rows=1, long_len=K)
+    cp, ri, cv, perm = oracle.csr2csc(g.rowptr, g.colind, g.val, K)
+    A = sp.csr_matrix((g.val, g.colind, g.rowptr), shape=(M, K)).tocsc()
+    A.sort_indices()
+    np.testing.assert_array_equal(cp, A.indptr)
+    np.testing.assert_array_equal(ri, A.indices)
+    np.testing.assert_array_equal(cv, A.data)
+    np.testing.assert_array_equal(g.val[perm], cv)
+
+
+def test_sddmm_matches_dense():
+    g = random_csr(60, 45, 0.2, seed=9)
+    r = np.random.default_rng(9)
+    dY = r.standard_normal((60, 17)).astype(np.float32)
+    X = r.standard_normal((45, 17)).astype(np.float32)
+    out = oracle.sddmm_f64(g.rowptr, g.colind, dY, X)
+    rows = np.repeat(np.arange(60), g.degrees())
+    full = dY.astype(np.float64) @ X.astype(np.float64).T
+    np.testing.assert_allclose(out, full[rows, g.colind], rtol=1e-12, atol=1e-12)
+
+
+def test_sddmm_is_value_gradient():
+    """d/dval <A X, G> = SDDMM(G, X): central differences on the FP64 dense
+    form (exact up to rounding since the loss is linear in val)."""
+    g = random_csr(20, 15, 0.3, seed=4, empty_rows=False)
+    r = np.random.default_rng(4)
+    X = r.standard_normal((15, 6))
+    G = r.standard_normal((20, 6))
+    grad = oracle.sddmm_f64(g.rowptr, g.colind, G.astype(np.float32), X.astype(np.float32))
+    rows = np.repeat(np.arange(20), g.degrees())
+    X32, G32 = X.astype(np.float32).astype(np.float64), G.astype(np.float32).astype(np.float64)
+    for e in range(0, g.nnz, 7):
+        h = 1e-3
+        vp, vm = g.val.astype(np.float64).copy(), g.val.astype(np.float64).copy()
+        vp[e] += h
+        vm[e] -= h
+        Ap = sp.csr_matrix((vp, g.colind, g.rowptr), shape=(20, 15))
+        Am = sp.csr_matrix((vm, g.colind, g.rowptr), shape=(20, 15))
+        fd = ((Ap @ X32) * G32).sum() - ((Am @ X32) * G32).sum()
+        assert abs(fd / (2 * h) - grad[e]) < 1e-9
