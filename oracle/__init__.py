@@ -18,7 +18,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_REF_PATH = os.path.join(_HERE, "_ref", "libref_readmtx.so")
 _lib = None
+_ref = None
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
@@ -72,6 +74,20 @@ def lib():
             f.restype = res
         _lib = L
     return _lib
+
+
+def ref():
+    """The reference's own readMtx.hpp helpers (oracle/_ref), or None when
+    that library was not built (the reference tree is absent)."""
+    global _ref
+    if _ref is None and os.path.exists(_REF_PATH):
+        R = C.CDLL(_REF_PATH)
+        R.ref_csr2csc.argtypes = [_i, _i, _i32p, _i32p, _f32p, _i32p, _i32p, _f32p]
+        R.ref_csr2csc.restype = None
+        R.ref_coo2csr.argtypes = [_i, _i, _i, _i32p, _i32p, _f32p, _i32p, _i32p, _f32p]
+        R.ref_coo2csr.restype = None
+        _ref = R
+    return _ref
 
 
 def _c(a, dt):

@@ -279,3 +279,45 @@ def test_sddmm_is_value_gradient():
         Am = sp.csr_matrix((vm, g.colind, g.rowptr), shape=(20, 15))
         fd = ((Ap @ X32) * G32).sum() - ((Am @ X32) * G32).sum()
         assert abs(fd / (2 * h) - grad[e]) < 1e-9
+
+
+# ------------------------------------------ against the reference's code
+def _ref_or_skip():
+    R = oracle.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    return R
+
+
+@pytest.mark.parametrize("M,K", [(1, 1), (70, 70), (400, 400)])
+def test_csr2csc_matches_reference_code(M, K):
+    """oracle.csr2csc against the reference's csr2csc (readMtx.hpp:319-345)
+    compiled from its own source. Square shapes only: the reference passes
+    (ncols, nrows) to coo2csc (:341), which swaps them again (:311-316), so
+    its transpose sizes the output by the row count and is only correct when
+    M == K (every workload here is square). The oracle transposes any shape;
+    test_csr2csc_matches_scipy pins the rectangular case."""
+    R = _ref_or_skip()
+    g = random_csr(M, K, 0.05, seed=M, long_rows=1, long_len=K)
+    nnz = g.nnz
+    cp, ri, cv = np.empty(K + 1, np.int32), np.empty(max(nnz, 1), np.int32), np.empty(max(nnz, 1), np.float32)
+    R.ref_csr2csc(M, K, g.rowptr, g.colind if nnz else np.zeros(1, np.int32),
+                  g.val if nnz else np.zeros(1, np.float32), cp, ri, cv)
+    ocp, ori, ocv, _ = oracle.csr2csc(g.rowptr, g.colind, g.val, K)
+    np.testing.assert_array_equal(cp, ocp)
+    np.testing.assert_array_equal(ri[:nnz], ori)
+    np.testing.assert_array_equal(cv[:nnz], ocv)
+
+
+def test_generator_csr_matches_reference_coo2csr():
+    """The synthetic graphs are in the CSR form the reference's coo2csr
+    (readMtx.hpp:246-304) builds from the same entries in shuffled order."""
+    R = _ref_or_skip()
+    g = chung_lu(500, 6000, seed=9)
+    rows = np.repeat(np.arange(g.M), g.degrees()).astype(np.int32)
+    perm = np.random.default_rng(0).permutation(g.nnz)
+    rp, ci, cv = np.empty(g.M + 1, np.int32), np.empty(g.nnz, np.int32), np.empty(g.nnz, np.float32)
+    R.ref_coo2csr(g.M, g.K, g.nnz, np.ascontiguousarray(rows[perm]), np.ascontiguousarray(g.colind[perm]),
+                  np.ascontiguousarray(g.val[perm]), rp, ci, cv)
+    np.testing.assert_array_equal(rp, g.rowptr)
+    np.testing.assert_array_equal(ci, g.colind)
