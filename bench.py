@@ -130,6 +130,13 @@ def headline(args, h, dev):
 
     ms_step, ms_launch = time_launches(step, args.steps, args.warmup)
     nbytes = b2sr_bytes(g.M, T, nb, g.nnz, N)
+    # the same product through the CSR kernel (the reference prints its
+    # cuSPARSE CSR time beside the B2SR one, test_spmm_full.cu:505)
+    v = torch.ones(g.nnz, device=dev)
+    plan = api.CsrPlan(h, rp, g.nnz)
+    Ccsr = torch.empty((g.M, N), device=dev)
+    csr_ms = time_loop(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=Ccsr, plan=plan), args.steps, args.warmup)
+    del plan
     flops = 2.0 * g.nnz * N
     # exactness spot check: +-1 B against an integer recompute of 2000 rows
     S = np.random.default_rng(3).choice([-1, 1], size=(g.K, N)).astype(np.float32)
@@ -137,8 +144,10 @@ def headline(args, h, dev):
     rows = np.random.default_rng(4).choice(g.M, 2000, replace=False)
     exact = all(np.array_equal(Cs[i], S[g.colind[g.rowptr[i]:g.rowptr[i + 1]]].sum(axis=0)) for i in rows)
     err = sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows)
+    same = float((C - Ccsr).abs().max())
     return dict(g=g, B=B, T=T, N=N, nb=nb, nbytes=nbytes, flops=flops, ms_step=ms_step,
-                ms_launch=ms_launch, conv_ms=conv_ms, exact=exact, err=err)
+                ms_launch=ms_launch, conv_ms=conv_ms, exact=exact, err=err, csr_ms=csr_ms,
+                csr_bytes=csr_bytes(g.M, g.nnz, N), csr_maxdiff=same)
 
 
 def cpu_baseline(hd):
@@ -202,9 +211,10 @@ def extra_csr(args, h, dev):
         Bh = api.to_bf16_bits(Bd)
         C = torch.empty((g.M, N), device=dev)
         for name, BB, sb in (("f32", Bd, 4), ("bf16", Bh, 2)):
+            geo = plan.autotune(rp, ci, v, BB, C)
             ms = time_loop(lambda: api.spmm_csr(h, rp, ci, v, BB, C_out=C, plan=plan), args.big_steps, 3)
             nbytes = csr_bytes(g.M, g.nnz, N, s_b=sb)
-            out[f"csr_{name}_233k_n{N}"] = {
+            out[f"csr_{name}_233k_n{N}"] = {"geometry": geo,
                 "nnz": g.nnz, "ms": round(ms, 3), "GBps": round(nbytes / ms / 1e6, 1),
                 "pct_hbm_peak": round(100 * nbytes / ms / 1e6 / HBM_PEAK_GBS, 1),
                 "GFLOPs": round(2 * g.nnz * N / ms / 1e6, 1),
@@ -239,6 +249,7 @@ def extra_fwd_bwd(args, h, dev):
 
     res = {}
     for name, XX, GG, s in (("f32", Xd, dYd, 4), ("bf16", Xh, dYh, 2)):
+        geo = (pf.autotune(rp, ci, v, XX, Y), pb.autotune(trp, tci, tval, GG, dX, K=g.M))
         ms = time_loop(lambda: step(XX, GG), 100, 10)
         nbytes = 2 * csr_bytes(g.M, g.nnz, N, s_b=s) + sddmm_bytes(g.M, g.nnz, N, s)
         rows = np.random.default_rng(7).choice(g.M, 1500, replace=False)
@@ -250,7 +261,7 @@ def extra_fwd_bwd(args, h, dev):
             ref = X[g.colind[s0:e0]].astype(np.float64) @ dY[i].astype(np.float64)
             num += float(((dvc[s0:e0] - ref) ** 2).sum())
             den += float((ref ** 2).sum())
-        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1),
+        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1), "geometry": geo,
                      "rel_err_Y_fp64": e_y, "rel_err_dval_fp64": (num / den) ** 0.5}
     res["bf16_speedup_vs_f32"] = round(res["f32"]["ms"] / res["bf16"]["ms"], 3)
     return {"gcn_fwd_bwd_89k_n128": res}
@@ -289,8 +300,9 @@ def extra_batched(args, h, dev):
             api.sddmm_csr(h, rp, ci, dY[r0:r1], X[r0:r1], out=dval[e0:e1])
 
     ms_sep = time_loop(separate, 5, 1)
+    info = plan.info()
     plan.close()
-    return {"batched_512_fwd_bwd_n64": {
+    return {"batched_512_fwd_bwd_n64": {"fused_launch": info,
         "graphs": len(gs), "rows": M, "nnz": plan.nnz,
         "bf16_one_graph_ms": round(ms_graph, 4), "f32_separate_launches_ms": round(ms_sep, 3),
         "speedup": round(ms_sep / ms_graph, 2)}}
@@ -306,6 +318,8 @@ def main():
     ap.add_argument("--big-steps", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", choices=["csr", "fwd_bwd", "batched"], default=None,
+                    help="run one extra workload alone and print its result (for profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -330,6 +344,11 @@ def main():
     from spmm_eval_amd import api
 
     h = api.Handle()
+    if args.only:
+        fn = {"csr": extra_csr, "fwd_bwd": extra_fwd_bwd, "batched": extra_batched}[args.only]
+        print(json.dumps(fn(args, h, dev)))
+        h.close()
+        return
     log(f"headline: B2SR-{args.tile} FBF SpMM, N={args.hidden}")
     hd = headline(args, h, dev)
     gbs = hd["nbytes"] / (hd["ms_step"] * 1e-3) / 1e9
@@ -357,6 +376,9 @@ def main():
                      "bytes_per_launch": hd["nbytes"], "mean_launch_ms": round(hd["ms_launch"], 5)},
         "checks": {"pm1_exact_2000_rows": bool(hd["exact"]), "rel_err_fp64_sampled": hd["err"]},
         "csr2b2sr_ms": round(hd["conv_ms"], 2),
+        "csr_same_graph": {"ms": round(hd["csr_ms"], 5),
+                           "GBps": round(hd["csr_bytes"] / hd["csr_ms"] / 1e6, 1),
+                           "max_abs_diff_vs_b2sr": hd["csr_maxdiff"]},
     }
     if not args.no_cpu:
         log("cpu baseline")

@@ -98,6 +98,17 @@ int spmm_csr_bf16_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
                           float *C, int ldc, const spmm_tune_t *tune,
                           void *stream);
 
+/* Autotuner: times every valid launch geometry (lanes per row, 16-byte
+ * vectors per lane, rows per subgroup) of the planned kernel on these
+ * operands (B is uint16 BF16 when bf16 != 0), records the fastest in the
+ * plan for this (N, dtype) -- later planned calls with tune == NULL use it
+ * -- and returns it in *best. Writes C; SYNCHRONISES. No reference
+ * counterpart. */
+int spmm_csr_autotune(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
+                      const int *rowptr, const int *colind, const float *val,
+                      const void *B, int bf16, int ldb, float *C, int ldc,
+                      spmm_tune_t *best, void *stream);
+
 /* ---------------------------------------------------------------- B2SR
  * csr2b2sr_nnzb + csr2b2sr replace cusparseXcsr2bsrNnz
  * (test_spmm_full.cu:148-151) and csr2bsr_batch_{4,8,16,32}
@@ -192,13 +203,21 @@ int sddmm_csr_bf16(spmm_handle_t h, int M, int K, int N, int nnz,
  * [row_off[g], row_off[g+1]) and entries [nnz_off[g], nnz_off[g+1]) of the
  * concatenated arrays; its column indices are local (0-based in g), and its
  * feature rows follow the same row offsets (square graphs). The plan holds
- * the analysed forward and transposed schedules and a captured hipGraph of
- * one fwd+bwd step (Y = A X, dX = A^T dY, dval = SDDMM(dY, X)). */
+ * the analysed forward and transposed schedules and runs one fwd+bwd step
+ * (Y = A X, dX = A^T dY, dval = SDDMM(dY, X)) as ONE fused kernel launch
+ * (optionally persistent over an atomic work queue), captured as a hipGraph.
+ * Creation autotunes the launch geometry and SYNCHRONISES. N must be a
+ * multiple of 16 (FP32 features) or 32 (BF16). */
 int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off,
                            const int *nnz_off, const int *rowptr,
                            const int *colind, const float *val, int N,
                            int bf16, spmm_batch_plan_t *plan, void *stream);
 int spmm_batch_plan_destroy(spmm_batch_plan_t plan);
+/* The geometry the plan's autotuner chose (lanes per row, vector width,
+ * rows per subgroup), whether the fused launch is persistent, and the tuned
+ * time of one step in ms. */
+int spmm_batch_plan_info(spmm_batch_plan_t plan, spmm_tune_t *tune,
+                         int *persistent, float *tuned_ms);
 /* One fwd+bwd step; features are uint16 BF16 when the plan is bf16, else
  * float. Y, dX are FP32 [total_rows][N]; dval FP32 [total_nnz]. */
 int spmm_batch_fwd_bwd(spmm_batch_plan_t plan, const void *X, const void *dY,

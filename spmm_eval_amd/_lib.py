@@ -50,6 +50,7 @@ _SIGS = {
     "spmm_csr_bf16": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_csr_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_csr_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
+    "spmm_csr_autotune": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, C.POINTER(Tune), _vp],
     "csr2b2sr_nnzb": [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _ip, _vp],
     "csr2b2sr": [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "spmm_b2sr_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
@@ -66,6 +67,7 @@ _SIGS = {
     "sddmm_csr_bf16": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_batch_plan_create": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, C.POINTER(_vp), _vp],
     "spmm_batch_plan_destroy": [_vp],
+    "spmm_batch_plan_info": [_vp, C.POINTER(Tune), _ip, C.POINTER(C.c_float)],
     "spmm_batch_fwd_bwd": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
 }
 

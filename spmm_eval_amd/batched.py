@@ -1,7 +1,8 @@
 """Batched uneven graphs through one plan (spmm_batch_plan_create /
 spmm_batch_fwd_bwd of the C-ABI): the graphs' CSR arrays are concatenated
 block-diagonally, analysed once, and one fwd+bwd step (Y = A X,
-dX = A^T dY, dval = SDDMM(dY, X)) is replayed as a captured hipGraph."""
+dX = A^T dY, dval = SDDMM(dY, X)) runs as one fused (autotuned, optionally
+persistent) kernel launch, replayed as a captured hipGraph."""
 from __future__ import annotations
 
 import ctypes as C
@@ -12,7 +13,7 @@ import torch
 from . import _lib
 from .api import Handle, _p, _stream
 from .graphs import Csr
-from ._lib import call
+from ._lib import Tune, call
 
 
 class BatchPlan:
@@ -41,6 +42,13 @@ class BatchPlan:
         self.Y = torch.empty((self.M, N), dtype=torch.float32, device=dev)
         self.dX = torch.empty((self.M, N), dtype=torch.float32, device=dev)
         self.dval = torch.empty(max(self.nnz, 1), dtype=torch.float32, device=dev)[: self.nnz]
+
+    def info(self) -> dict:
+        """Autotuned geometry of the fused launch."""
+        t, pers, ms = Tune(), C.c_int(), C.c_float()
+        call("spmm_batch_plan_info", self.p, C.byref(t), C.byref(pers), C.byref(ms))
+        return {"lanes_per_row": t.lanes_per_row, "vec": t.vec, "rows_scale": t.rows_scale,
+                "persistent": bool(pers.value), "tuned_ms": round(ms.value, 4)}
 
     def step(self, X: torch.Tensor, dY: torch.Tensor, use_graph: bool = True, stream=None):
         call("spmm_batch_fwd_bwd", self.p, _p(X), _p(dY), _p(self.Y), _p(self.dX), _p(self.dval),

@@ -11,11 +11,13 @@
 // identical to the sequential counting sort. Row indices and values are then
 // gathered through the sorted positions.
 //
-// SDDMM: one subgroup of G lanes per row keeps its dY row segment in
-// registers; for each stored entry it gathers the X row segment (16-byte
-// loads, several entries in flight), forms the partial dot products and
-// reduces them across the G lanes with xor-shuffles.
-#include "common.hpp"
+// SDDMM: entry-chunked (each subgroup owns 32 consecutive stored entries,
+// so hub rows are split across subgroups); per row the dY segment stays in
+// registers, X row segments are gathered eight at a time and the partial
+// dots are reduce-scattered across the G lanes (about one shuffle per
+// entry). The kernel body lives in csr_core.hpp (shared with the fused
+// batched launch).
+#include "csr_core.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -112,66 +114,12 @@ gather_kernel(int nnz, const int *__restrict__ pos, const int *__restrict__ rowi
   }
 }
 
-// ---------------------------------------------------------------- SDDMM
 template <typename TB, int G>
 __global__ void __launch_bounds__(kAdjThreads)
-sddmm_kernel(int M, const int *__restrict__ rowptr, const int *__restrict__ colind,
+sddmm_kernel(int M, int nnz, const int *__restrict__ rowptr, const int *__restrict__ colind,
              const TB *__restrict__ dY, int lddy, const TB *__restrict__ X, int ldx,
              float *__restrict__ out, int ntile) {
-  constexpr int EPL = Vec16<TB>::kElems;
-  constexpr int S = kWave / G;
-  constexpr int U = 8;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int lane_sub = lane & (G - 1);
-  const int gsub = (threadIdx.x / kWave) * S + lane / G;
-  const int row = blockIdx.x * (kAdjThreads / G) + gsub;
-  if (row >= M) return;
-  const int e0 = rowptr[row], e1 = rowptr[row + 1];
-  if (e0 == e1) return;
-  // ntile column tiles of G*EPL each; partial dots are summed tile by tile.
-  float y[4][EPL];
-  for (int tt = 0; tt < ntile; tt += 4) {
-    const int nt = min(4, ntile - tt);
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-      if (t < nt)
-        Vec16<TB>::load(dY + static_cast<size_t>(row) * lddy + (tt + t) * G * EPL + lane_sub * EPL,
-                        y[t]);
-    for (int e = e0; e < e1; e += U) {
-      float part[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) part[u] = 0.f;
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (e + u < e1) {
-          const TB *xr = X + static_cast<size_t>(colind[e + u]) * ldx + lane_sub * EPL;
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            if (t < nt) {
-              float x[EPL];
-              Vec16<TB>::load(xr + (tt + t) * G * EPL, x);
-#pragma unroll
-              for (int q = 0; q < EPL; ++q) part[u] = fmaf(y[t][q], x[q], part[u]);
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-#pragma unroll
-        for (int m = 1; m < G; m <<= 1) part[u] += __shfl_xor(part[u], m, G);
-      }
-      if (lane_sub == 0) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (e + u < e1) {
-            if (tt == 0) out[e + u] = part[u];
-            else out[e + u] += part[u];
-          }
-        }
-      }
-    }
-  }
+  sddmm_wg<TB, G>(blockIdx.x, M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile);
 }
 
 template <typename TB>
@@ -194,10 +142,11 @@ sddmm_generic_kernel(int M, int N, const int *__restrict__ rowptr, const int *__
 }
 
 template <typename TB, int G>
-static void launch_sddmm(int M, const int *rowptr, const int *colind, const TB *dY, int lddy,
-                         const TB *X, int ldx, float *out, int ntile, hipStream_t s) {
-  hipLaunchKernelGGL((sddmm_kernel<TB, G>), dim3(ceil_div(M, kAdjThreads / G)),
-                     dim3(kAdjThreads), 0, s, M, rowptr, colind, dY, lddy, X, ldx, out, ntile);
+static void launch_sddmm(int M, int nnz, const int *rowptr, const int *colind, const TB *dY,
+                         int lddy, const TB *X, int ldx, float *out, int ntile, hipStream_t s) {
+  const long long per_wg = static_cast<long long>(kAdjThreads / G) * kSddmmChunk;
+  hipLaunchKernelGGL((sddmm_kernel<TB, G>), dim3(static_cast<unsigned>(ceil_div_ll(nnz, per_wg))),
+                     dim3(kAdjThreads), 0, s, M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile);
 }
 
 template <typename TB>
@@ -218,7 +167,7 @@ static int run_sddmm(int M, int K, int N, int nnz, const int *rowptr, const int 
   int G = 0;
   if (vec) {
     const int lanes = N / e16;
-    for (int t = 64; t >= 1; t >>= 1)
+    for (int t = 32; t >= 1; t >>= 1)  // the reduce-scatter keeps G partials per lane
       if (t <= lanes && lanes % t == 0) { G = t; break; }
   }
   if (G == 0) {
@@ -227,13 +176,12 @@ static int run_sddmm(int M, int K, int N, int nnz, const int *rowptr, const int 
   } else {
     const int ntile = N / (G * e16);
     switch (G) {
-      case 1: launch_sddmm<TB, 1>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
-      case 2: launch_sddmm<TB, 2>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
-      case 4: launch_sddmm<TB, 4>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
-      case 8: launch_sddmm<TB, 8>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
-      case 16: launch_sddmm<TB, 16>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
-      case 32: launch_sddmm<TB, 32>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
-      default: launch_sddmm<TB, 64>(M, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 1: launch_sddmm<TB, 1>(M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 2: launch_sddmm<TB, 2>(M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 4: launch_sddmm<TB, 4>(M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 8: launch_sddmm<TB, 8>(M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 16: launch_sddmm<TB, 16>(M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      default: launch_sddmm<TB, 32>(M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
     }
   }
   SPMM_LAUNCHED();

@@ -15,11 +15,13 @@
 // csr2bsr_batch_bsrbmv.cu:236-379).
 //
 // FBF SpMM: one subgroup of G lanes per output row (G lanes x 16-byte
-// vectors span the B row segment). For each chunk of G tiles, lane l reads
-// tile l's word for the subgroup's row, the set bits are counted, prefix
-// summed across the subgroup and decoded into an LDS list of B row indices
-// (ascending column order), and the subgroup then streams that list with
-// several 16-byte B gathers in flight per lane, as the CSR kernel does.
+// vectors span the B row segment). For each chunk of tiles (up to 64 for
+// T=4; each lane reads the row's word of several consecutive tiles), the
+// set bits are counted, prefix summed across the subgroup and decoded into
+// an LDS list of B row indices (ascending column order), and the subgroup
+// then streams that list with several 16-byte B gathers in flight per lane,
+// as the CSR kernel does. Power-law graphs leave ~1 set bit per 4x4 tile,
+// so decoding a whole block row per round keeps the gathers deep.
 #include "common.hpp"
 
 #include <climits>
@@ -34,6 +36,7 @@ template <> struct Word<32> { using type = uint32_t; };
 
 constexpr int kB2Threads = 256;
 constexpr int kB2Waves = kB2Threads / kWave;
+constexpr int kLongChunks = 2;  // block rows above this many tile chunks go to b2sr_long_kernel
 
 static bool valid_T(int T) { return T == 4 || T == 8 || T == 16 || T == 32; }
 
@@ -93,20 +96,39 @@ csr2b2sr_kernel(int M, int K, const int *__restrict__ rowptr, const int *__restr
 }
 
 // --------------------------------------------------- decode to LDS list
-// The subgroup's tiles [t, t+n) for row r: writes the B row indices of the
-// set bits, tile by tile and ascending within a tile, to list[0..total).
+// LDS list entries per subgroup and tiles decoded per chunk: a chunk of
+// TC tiles can hold at most TC*T set bits, which must fit the list.
+template <int T, int G> struct Chunk {
+  static constexpr int kCap = (16 * G > T) ? 16 * G : T;  // 16 KB per WG (32 KB for G=1,T=32)
+  static constexpr int kTiles = kCap / T;                  // TC
+  static constexpr int kPerLane = (kTiles + G - 1) / G;    // TPL
+};
+
+// The subgroup's tiles [t, min(t + TC, t1)) for row r, TPL consecutive
+// tiles per lane: writes the B row indices of the set bits, tile by tile
+// and ascending within a tile, to list[0..total).
 template <int T, int G>
-__device__ __forceinline__ int decode_chunk(int t, int n, int r, int lane_sub,
+__device__ __forceinline__ int decode_chunk(int t, int t1, int r, int lane_sub,
                                             const int *__restrict__ bci,
                                             const typename Word<T>::type *__restrict__ words,
                                             int *list) {
-  int bc = 0;
-  uint32_t w = 0;
-  if (lane_sub < n) {
-    bc = bci[t + lane_sub];
-    w = words[static_cast<size_t>(t + lane_sub) * T + r];
+  constexpr int TC = Chunk<T, G>::kTiles;
+  constexpr int TPL = Chunk<T, G>::kPerLane;
+  int bc[TPL];
+  uint32_t w[TPL];
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int ti = lane_sub * TPL + k;
+    const int tile = t + ti;
+    bc[k] = 0;
+    w[k] = 0;
+    if (ti < TC && tile < t1) {
+      bc[k] = bci[tile];
+      w[k] = words[static_cast<size_t>(tile) * T + r];
+    }
+    cnt += __popc(w[k]);
   }
-  const int cnt = __popc(w);
   int inc = cnt;
 #pragma unroll
   for (int d = 1; d < G; d <<= 1) {
@@ -115,11 +137,15 @@ __device__ __forceinline__ int decode_chunk(int t, int n, int r, int lane_sub,
   }
   const int total = __shfl(inc, G - 1, G);
   int off = inc - cnt;
-  const int base = bc * T + T - 1;
-  while (w) {
-    const int p = 31 - __clz(w);
-    w ^= 1u << p;
-    list[off++] = base - p;
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int base = bc[k] * T + T - 1;
+    uint32_t x = w[k];
+    while (x) {
+      const int p = 31 - __clz(x);
+      x ^= 1u << p;
+      list[off++] = base - p;
+    }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -131,13 +157,12 @@ template <int T, typename TB, int G, int VEC>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_spmm_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci,
                  const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
-                 int ldb, float *__restrict__ C, int ldc) {
+                 int ldb, float *__restrict__ C, int ldc, int *__restrict__ state) {
   constexpr int EPL = VEC * Vec16<TB>::kElems;
   constexpr int S = kWave / G;
   constexpr int CW = G * EPL;
-  constexpr int U0 = EPL >= 16 ? 4 : 8;
-  constexpr int U = U0;
-  __shared__ int lds[kB2Waves * S][G * T];
+  constexpr int U = EPL >= 16 ? 4 : 8;
+  __shared__ int lds[kB2Waves * S][Chunk<T, G>::kCap];
 
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
@@ -150,28 +175,35 @@ b2sr_spmm_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
   const TB *Bc = B + col0;
   int *list = lds[gsub];
 
+  const int t0 = brp[br], t1 = brp[br + 1];
+  if (t1 - t0 > kLongChunks * Chunk<T, G>::kTiles) {  // left to b2sr_long_kernel
+    // one entry per block row: its row 0, first column tile
+    if (blockIdx.y == 0 && r == 0 && lane_sub == 0) state[2 + atomicAdd(&state[0], 1)] = br;
+    return;
+  }
   float acc[EPL];
 #pragma unroll
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-  const int t1 = brp[br + 1];
-  for (int t = brp[br]; t < t1; t += G) {
-    const int total = decode_chunk<T, G>(t, min(G, t1 - t), r, lane_sub, bci, words, list);
+  for (int t = t0; t < t1; t += Chunk<T, G>::kTiles) {
+    const int total = decode_chunk<T, G>(t, t1, r, lane_sub, bci, words, list);
     for (int k = 0; k < total; k += U) {
-      float b[U][EPL];
+      // unconditional loads (slots past total re-read entry 0, which
+      // exists since total > 0), one wait, predicated accumulation
+      typename Vec16<TB>::Raw raw[U][VEC];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (k + u < total) {
-          const TB *p = Bc + static_cast<size_t>(list[k + u]) * ldb;
+        const TB *p = Bc + static_cast<size_t>(list[k + u < total ? k + u : 0]) * ldb;
 #pragma unroll
-          for (int q = 0; q < VEC; ++q)
-            Vec16<TB>::load(p + q * Vec16<TB>::kElems, &b[u][q * Vec16<TB>::kElems]);
-        }
+        for (int q = 0; q < VEC; ++q) raw[u][q] = Vec16<TB>::raw(p + q * Vec16<TB>::kElems);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (k + u < total) {
+          float b[EPL];
 #pragma unroll
-          for (int q = 0; q < EPL; ++q) acc[q] += b[u][q];
+          for (int q = 0; q < VEC; ++q) Vec16<TB>::widen(raw[u][q], &b[q * Vec16<TB>::kElems]);
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) acc[q] += b[q];
         }
       }
     }
@@ -181,6 +213,89 @@ b2sr_spmm_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
 #pragma unroll
   for (int q = 0; q < EPL; q += 4)
     *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
+}
+
+// Block rows with more than kLongChunks chunks of tiles (the hub rows of a
+// power-law graph) are appended by b2sr_spmm_kernel to a list that this
+// persistent kernel drains, one block row per workgroup: each wave takes
+// one of the block row's rows, its S subgroups take the row's tile chunks
+// in turn (decode + gather), and a butterfly over the subgroups sums the
+// partial rows in a fixed order. The last workgroup to finish resets the
+// list counters, so consecutive (or graph-replayed) calls need no memset.
+template <int T, typename TB, int G, int VEC>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci,
+                 const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
+                 int ldb, float *__restrict__ C, int ldc, int *__restrict__ state) {
+  constexpr int EPL = VEC * Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * EPL;
+  constexpr int U = EPL >= 16 ? 4 : 8;
+  constexpr int TC = Chunk<T, G>::kTiles;
+  __shared__ int lds[kB2Waves * S][Chunk<T, G>::kCap];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1);
+  const int sub = lane / G;
+  const int col0 = blockIdx.y * CW + lane_sub * EPL;
+  const TB *Bc = B + col0;
+  int *list = lds[wave * S + sub];
+  const int n_long = state[0];
+  for (int i = blockIdx.x; i < n_long; i += gridDim.x) {
+    const int br = state[2 + i];
+    const int t0 = brp[br], t1 = brp[br + 1];
+    for (int r0 = 0; r0 < T; r0 += kB2Waves) {
+      const int r = r0 + wave;
+      const int row = br * T + r;
+      if (row >= M) continue;  // whole wave
+      float acc[EPL];
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+      for (int t = t0 + sub * TC; t < t1; t += S * TC) {
+        const int total = decode_chunk<T, G>(t, t1, r, lane_sub, bci, words, list);
+        for (int k = 0; k < total; k += U) {
+          typename Vec16<TB>::Raw raw[U][VEC];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const TB *p = Bc + static_cast<size_t>(list[k + u < total ? k + u : 0]) * ldb;
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) raw[u][q] = Vec16<TB>::raw(p + q * Vec16<TB>::kElems);
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if (k + u < total) {
+              float b[EPL];
+#pragma unroll
+              for (int q = 0; q < VEC; ++q) Vec16<TB>::widen(raw[u][q], &b[q * Vec16<TB>::kElems]);
+#pragma unroll
+              for (int q = 0; q < EPL; ++q) acc[q] += b[q];
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+#pragma unroll
+      for (int m = G; m < kWave; m <<= 1) {
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
+      }
+      if (sub == 0) {
+        float *Crow = C + static_cast<size_t>(row) * ldc + col0;
+#pragma unroll
+        for (int q = 0; q < EPL; q += 4)
+          *reinterpret_cast<float4 *>(Crow + q) =
+              make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // Every workgroup has read the count before it counts itself done.
+    if (atomicAdd(&state[1], 1) == static_cast<int>(gridDim.x * gridDim.y) - 1) {
+      atomicExch(&state[0], 0);
+      atomicExch(&state[1], 0);
+    }
+  }
 }
 
 // Any N / alignment: one wave per output row, lanes stride over columns.
@@ -217,7 +332,7 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
                  const uint32_t *__restrict__ Bw, int ldbw, int32_t *__restrict__ C, int ldc) {
   constexpr int S = kWave / G;
   constexpr int U = 8;
-  __shared__ int lds[kB2Waves * S][G * T];
+  __shared__ int lds[kB2Waves * S][Chunk<T, G>::kCap];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int lane_sub = lane & (G - 1);
@@ -233,17 +348,18 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
   int cnt[4] = {0, 0, 0, 0};
   int deg = 0;
   const int t1 = brp[br + 1];
-  for (int t = brp[br]; t < t1; t += G) {
-    const int total = decode_chunk<T, G>(t, min(G, t1 - t), r, lane_sub, bci, words, list);
+  for (int t = brp[br]; t < t1; t += Chunk<T, G>::kTiles) {
+    const int total = decode_chunk<T, G>(t, t1, r, lane_sub, bci, words, list);
     deg += total;
     for (int k = 0; k < total; k += U) {
       uint32_t wv[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        wv[u] = (k + u < total && has_cols) ? Bw[static_cast<size_t>(list[k + u]) * ldbw + wi] : 0u;
+      for (int u = 0; u < U; ++u)  // unconditional: past total re-read entry 0
+        wv[u] = Bw[static_cast<size_t>(list[k + u < total ? k + u : 0]) * ldbw + wi];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const uint32_t nib = (wv[u] >> sh) & 0xfu;  // columns col0..col0+3 at bits 3..0
+        // columns col0..col0+3 at bits 3..0; slots past total count nothing
+        const uint32_t nib = (k + u < total && has_cols) ? (wv[u] >> sh) & 0xfu : 0u;
         cnt[0] += (nib >> 3) & 1u;
         cnt[1] += (nib >> 2) & 1u;
         cnt[2] += (nib >> 1) & 1u;
@@ -338,49 +454,86 @@ static Geo geometry(int N, int elem_bytes, const void *B, int ldb, const void *C
 
 template <int T, typename TB, int G>
 static void launch_fbf(int M, const int *brp, const int *bci, const void *val, const TB *B,
-                       int ldb, float *C, int ldc, int tiles, hipStream_t s) {
+                       int ldb, float *C, int ldc, int tiles, int *state, int long_grid,
+                       hipStream_t s) {
   constexpr int S = kWave / G;
   dim3 grid(ceil_div(M, kB2Waves * S), tiles);
   hipLaunchKernelGGL((b2sr_spmm_kernel<T, TB, G, 1>), grid, dim3(kB2Threads), 0, s, M, brp, bci,
-                     static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc);
+                     static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc, state);
+  hipLaunchKernelGGL((b2sr_long_kernel<T, TB, G, 1>), dim3(long_grid, tiles), dim3(kB2Threads), 0, s,
+                     M, brp, bci, static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc,
+                     state);
 }
 
 template <int T, typename TB>
 static void dispatch_fbf(const Geo &g, int M, int N, const int *brp, const int *bci,
                          const void *val, const TB *B, int ldb, float *C, int ldc,
-                         hipStream_t s) {
+                         int *state, int long_grid, hipStream_t s) {
   switch (g.G) {
     case 0:
       hipLaunchKernelGGL((b2sr_spmm_generic_kernel<T, TB>), dim3(ceil_div(M, kB2Waves)),
                          dim3(kB2Threads), 0, s, M, N, brp, bci,
                          static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc);
       break;
-    case 1: launch_fbf<T, TB, 1>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 2: launch_fbf<T, TB, 2>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 4: launch_fbf<T, TB, 4>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 8: launch_fbf<T, TB, 8>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 16: launch_fbf<T, TB, 16>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 32: launch_fbf<T, TB, 32>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    default: launch_fbf<T, TB, 64>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 1: launch_fbf<T, TB, 1>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
+    case 2: launch_fbf<T, TB, 2>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
+    case 4: launch_fbf<T, TB, 4>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
+    case 8: launch_fbf<T, TB, 8>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
+    case 16: launch_fbf<T, TB, 16>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
+    case 32: launch_fbf<T, TB, 32>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
+    default: launch_fbf<T, TB, 64>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
   }
 }
 
+// The long-block-row list of the handle, sized for MB block rows. Growing
+// allocates zeroed counters (synchronously); afterwards the kernels keep
+// the counters at zero between calls.
+static int b2sr_state(spmm_handle_s *h, int MB, int **out) {
+  if (h->b2sr_cap < MB) {
+    if (h->b2sr_state) {
+      if (hipDeviceSynchronize() != hipSuccess) SPMM_FAIL(SPMM_ERR_HIP, "sync before regrowing");
+      (void)hipFree(h->b2sr_state);
+      h->b2sr_state = nullptr;
+      h->b2sr_cap = 0;
+    }
+    const int cap = MB + MB / 4 + 64;
+    if (hipMalloc(&h->b2sr_state, sizeof(int) * (static_cast<size_t>(cap) + 2)) != hipSuccess)
+      SPMM_FAIL(SPMM_ERR_ALLOC, "long-row list allocation failed");
+    if (hipMemset(h->b2sr_state, 0, sizeof(int) * 2) != hipSuccess)
+      SPMM_FAIL(SPMM_ERR_HIP, "long-row list init");
+    h->b2sr_cap = cap;
+  }
+  *out = h->b2sr_state;
+  return SPMM_OK;
+}
+
 template <typename TB>
-static int run_fbf(int T, int M, int K, int N, const int *brp, const int *bci, const void *val,
-                   const TB *B, int ldb, float *C, int ldc, void *stream) {
+static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp, const int *bci,
+                   const void *val, const TB *B, int ldb, float *C, int ldc, void *stream) {
   SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
   if (M == 0 || N == 0) return SPMM_OK;
   SPMM_CHECK_ARG(brp && C, "null bsrRowPtr or C");
   SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldb >= N), "ldb/ldc smaller than N");
   SPMM_CHECK_ARG(K == 0 || B, "null B");
+  SPMM_CHECK_ARG(h != nullptr, "null handle");
   hipStream_t s = as_stream(stream);
   Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
+  int *state = nullptr;
+  int long_grid = 0;
+  if (g.G) {
+    const int st = b2sr_state(h, (M + T - 1) / T, &state);
+    if (st) return st;
+    int dev = 0, cus = 0;
+    SPMM_HIP(hipGetDevice(&dev));
+    SPMM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    long_grid = 8 * cus;
+  }
   switch (T) {
-    case 4: dispatch_fbf<4, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 8: dispatch_fbf<8, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 16: dispatch_fbf<16, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, s); break;
-    default: dispatch_fbf<32, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 4: dispatch_fbf<4, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s); break;
+    case 8: dispatch_fbf<8, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s); break;
+    case 16: dispatch_fbf<16, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s); break;
+    default: dispatch_fbf<32, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s); break;
   }
   SPMM_LAUNCHED();
   return SPMM_OK;
@@ -463,15 +616,13 @@ int csr2b2sr(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const in
 int spmm_b2sr_f32(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,
                   const int *bsrColInd, const void *bsrVal, const float *B, int ldb, float *C,
                   int ldc, void *stream) {
-  (void)h;
-  return run_fbf<float>(T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc, stream);
+  return run_fbf<float>(h, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc, stream);
 }
 
 int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,
                    const int *bsrColInd, const void *bsrVal, const uint16_t *B, int ldb,
                    float *C, int ldc, void *stream) {
-  (void)h;
-  return run_fbf<uint16_t>(T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc, stream);
+  return run_fbf<uint16_t>(h, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc, stream);
 }
 
 int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,

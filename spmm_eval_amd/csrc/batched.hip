@@ -2,11 +2,18 @@
 // The plan rewrites the per-graph (local) column indices to global ones,
 // analyses the union once (rows of all graphs binned together, so short and
 // long rows from different graphs share waves), builds the transposed union
-// for dX = A^T dY and captures one fwd+bwd step -- Y = A X, dX = A^T dY,
-// dval = SDDMM(dY, X) -- as a hipGraph that is replayed per step. Each row's
-// arithmetic is the same as in a per-graph run of the same kernels, so the
-// batched results equal per-graph results bit for bit.
-#include "common.hpp"
+// for dX = A^T dY, and runs one fwd+bwd step -- Y = A X, dX = A^T dY,
+// dval = SDDMM(dY, X) -- as ONE kernel launch whose work items are the
+// forward CSR workgroups, the transposed CSR workgroups and the SDDMM row
+// blocks. In persistent mode a grid sized to the machine pulls items from
+// an atomic queue that the last finishing workgroup resets (no memset node
+// is needed when the launch is captured). The plan's autotuner times every
+// (lanes per row, vector width, rows per subgroup, persistent or not)
+// candidate at creation and keeps the fastest; the step is captured as a
+// hipGraph and replayed. Each row's arithmetic is the same as in a
+// per-graph run of the same kernels, so results equal per-graph results bit
+// for bit.
+#include "csr_core.hpp"
 
 #include <new>
 
@@ -18,7 +25,11 @@ struct spmm_batch_plan_s {
   int *gcol = nullptr;          // global column indices (owned)
   int *trowptr = nullptr, *tcol = nullptr;  // transposed union (owned)
   float *tval = nullptr;
+  int *counters = nullptr;      // [queue, done] of the persistent mode
   spmm_csr_plan_t fwd = nullptr, bwd = nullptr;
+  // chosen geometry
+  int G = 0, VEC = 1, rows_scale = 1, persistent = 0, grid = 0;
+  float tuned_ms = 0.f;
   hipStream_t cap = nullptr;  // capture stream (default priority)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -38,6 +49,148 @@ __global__ void globalise_cols_kernel(int ngraphs, const int *__restrict__ row_o
     gcol[e] = colind[e] + off;
 }
 
+struct FusedArgs {
+  LaunchTable fwd, bwd;
+  int ctiles;                 // CSR column tiles
+  int n_fwd, n_bwd, n_sddmm;  // items of each part (CSR parts include column tiles)
+  int M, N, nnz, sddmm_ntile;
+  const int *rows_fwd, *rows_bwd;
+  const int *rowptr, *colind, *trowptr, *tcol;
+  const float *val, *tval;
+  const void *X, *dY;
+  float *Y, *dX, *dval;
+  int *queue, *done;          // persistent mode only
+};
+
+template <typename TB, int G, int VEC, int GS>
+__device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
+  const TB *X = static_cast<const TB *>(a.X);
+  const TB *dY = static_cast<const TB *>(a.dY);
+  if (item < a.n_fwd) {
+    csr_wg<TB, G, VEC>(a.fwd, item / a.ctiles, item % a.ctiles, a.rows_fwd, a.rowptr, a.colind,
+                       a.val, X, a.N, a.Y, a.N);
+    return;
+  }
+  item -= a.n_fwd;
+  if (item < a.n_bwd) {
+    csr_wg<TB, G, VEC>(a.bwd, item / a.ctiles, item % a.ctiles, a.rows_bwd, a.trowptr, a.tcol,
+                       a.tval, dY, a.N, a.dX, a.N);
+    return;
+  }
+  item -= a.n_bwd;
+  sddmm_wg<TB, GS>(item, a.M, a.nnz, a.rowptr, a.colind, dY, a.N, X, a.N, a.dval, a.sddmm_ntile);
+}
+
+template <typename TB, int G, int VEC, int GS, bool kPersistent>
+__global__ void __launch_bounds__(kThreads) fused_step_kernel(FusedArgs a) {
+  const int total = a.n_fwd + a.n_bwd + a.n_sddmm;
+  if (!kPersistent) {
+    fused_item<TB, G, VEC, GS>(a, blockIdx.x);
+    return;
+  }
+  __shared__ int next;
+  while (true) {
+    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int item = next;
+    __syncthreads();
+    if (item >= total) break;
+    fused_item<TB, G, VEC, GS>(a, item);
+  }
+  if (threadIdx.x == 0) {
+    // Every workgroup stops fetching before it counts itself done, so the
+    // last one can reset both counters for the next launch.
+    if (atomicAdd(a.done, 1) == static_cast<int>(gridDim.x) - 1) {
+      atomicExch(a.queue, 0);
+      atomicExch(a.done, 0);
+    }
+  }
+}
+
+static int sddmm_G(int N, int e16) {
+  const int lanes = N / e16;
+  for (int t = 32; t >= 1; t >>= 1)
+    if (t <= lanes && lanes % t == 0) return t;
+  return 0;
+}
+
+template <typename TB, int G, int VEC, int GS>
+static int launch_fused_g(const FusedArgs &a, bool persistent, int grid, hipStream_t s) {
+  if (persistent)
+    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, true>), dim3(grid), dim3(kThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, false>),
+                       dim3(a.n_fwd + a.n_bwd + a.n_sddmm), dim3(kThreads), 0, s, a);
+  return hipGetLastError() == hipSuccess ? SPMM_OK : SPMM_ERR_HIP;
+}
+
+template <typename TB, int VEC, int GS>
+static int launch_fused_v(int G, const FusedArgs &a, bool persistent, int grid, hipStream_t s) {
+  switch (G) {
+    case 4: return launch_fused_g<TB, 4, VEC, GS>(a, persistent, grid, s);
+    case 8: return launch_fused_g<TB, 8, VEC, GS>(a, persistent, grid, s);
+    case 16: return launch_fused_g<TB, 16, VEC, GS>(a, persistent, grid, s);
+    case 32: return launch_fused_g<TB, 32, VEC, GS>(a, persistent, grid, s);
+    case 64: return launch_fused_g<TB, 64, VEC, GS>(a, persistent, grid, s);
+    default: return SPMM_ERR_UNSUPPORTED;
+  }
+}
+
+template <typename TB, int GS>
+static int launch_fused_s(int G, int VEC, const FusedArgs &a, bool persistent, int grid,
+                          hipStream_t s) {
+  return VEC == 2 ? launch_fused_v<TB, 2, GS>(G, a, persistent, grid, s)
+                  : launch_fused_v<TB, 1, GS>(G, a, persistent, grid, s);
+}
+
+template <typename TB>
+static int launch_fused(int G, int VEC, int GS, const FusedArgs &a, bool persistent, int grid,
+                        hipStream_t s) {
+  switch (GS) {
+    case 4: return launch_fused_s<TB, 4>(G, VEC, a, persistent, grid, s);
+    case 8: return launch_fused_s<TB, 8>(G, VEC, a, persistent, grid, s);
+    case 16: return launch_fused_s<TB, 16>(G, VEC, a, persistent, grid, s);
+    case 32: return launch_fused_s<TB, 32>(G, VEC, a, persistent, grid, s);
+    default: return SPMM_ERR_UNSUPPORTED;
+  }
+}
+
+static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, int persistent,
+                        const void *X, const void *dY, float *Y, float *dX, float *dval,
+                        hipStream_t s) {
+  const int elem = p->bf16 ? 2 : 4;
+  const int e16 = 16 / elem;
+  FusedArgs a;
+  build_table(p->fwd->counts, p->fwd->offs, G, rows_scale, &a.fwd);
+  build_table(p->bwd->counts, p->bwd->offs, G, rows_scale, &a.bwd);
+  a.ctiles = p->N / (G * e16 * VEC);
+  a.n_fwd = a.fwd.total_wg * a.ctiles;
+  a.n_bwd = a.bwd.total_wg * a.ctiles;
+  const int GS = sddmm_G(p->N, e16);
+  a.n_sddmm = static_cast<int>(ceil_div_ll(p->nnz, static_cast<long long>(kThreads / GS) * kSddmmChunk));
+  a.sddmm_ntile = p->N / (GS * e16);
+  a.M = p->M;
+  a.N = p->N;
+  a.nnz = p->nnz;
+  a.rows_fwd = p->fwd->d_rows;
+  a.rows_bwd = p->bwd->d_rows;
+  a.rowptr = p->rowptr;
+  a.colind = p->gcol;
+  a.trowptr = p->trowptr;
+  a.tcol = p->tcol;
+  a.val = p->val;
+  a.tval = p->tval;
+  a.X = X;
+  a.dY = dY;
+  a.Y = Y;
+  a.dX = dX;
+  a.dval = dval;
+  a.queue = p->counters;
+  a.done = p->counters + 1;
+  return p->bf16 ? launch_fused<uint16_t>(G, VEC, GS, a, persistent, p->grid, s)
+                 : launch_fused<float>(G, VEC, GS, a, persistent, p->grid, s);
+}
+
 static void destroy_graph(spmm_batch_plan_s *p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -45,31 +198,69 @@ static void destroy_graph(spmm_batch_plan_s *p) {
   p->graph = nullptr;
 }
 
-static int enqueue_step(spmm_batch_plan_s *p, const void *X, const void *dY, float *Y, float *dX,
-                        float *dval, hipStream_t s) {
-  const int M = p->M, N = p->N;
-  int st;
-  if (p->bf16) {
-    st = spmm_csr_bf16_planned(p->fwd, M, M, N, p->nnz, p->rowptr, p->gcol, p->val,
-                               static_cast<const uint16_t *>(X), N, Y, N, nullptr, s);
-    if (st) return st;
-    st = spmm_csr_bf16_planned(p->bwd, M, M, N, p->nnz, p->trowptr, p->tcol, p->tval,
-                               static_cast<const uint16_t *>(dY), N, dX, N, nullptr, s);
-    if (st) return st;
-    st = sddmm_csr_bf16(p->h, M, M, N, p->nnz, p->rowptr, p->gcol,
-                        static_cast<const uint16_t *>(dY), N, static_cast<const uint16_t *>(X), N,
-                        dval, s);
-  } else {
-    st = spmm_csr_f32_planned(p->fwd, M, M, N, p->nnz, p->rowptr, p->gcol, p->val,
-                              static_cast<const float *>(X), N, Y, N, nullptr, s);
-    if (st) return st;
-    st = spmm_csr_f32_planned(p->bwd, M, M, N, p->nnz, p->trowptr, p->tcol, p->tval,
-                              static_cast<const float *>(dY), N, dX, N, nullptr, s);
-    if (st) return st;
-    st = sddmm_csr_f32(p->h, M, M, N, p->nnz, p->rowptr, p->gcol, static_cast<const float *>(dY),
-                       N, static_cast<const float *>(X), N, dval, s);
+// Times each geometry on scratch operands (zeros: the gather pattern, not
+// the values, sets the time) and keeps the fastest.
+static int tune_plan(spmm_batch_plan_s *p) {
+  const int elem = p->bf16 ? 2 : 4;
+  const int e16 = 16 / elem;
+  const size_t feat = static_cast<size_t>(p->M) * p->N;
+  void *X = nullptr, *dY = nullptr;
+  float *Y = nullptr, *dX = nullptr, *dval = nullptr;
+  int st = SPMM_OK;
+  if (hipMalloc(&X, feat * elem) != hipSuccess || hipMalloc(&dY, feat * elem) != hipSuccess ||
+      hipMalloc(&Y, feat * 4) != hipSuccess || hipMalloc(&dX, feat * 4) != hipSuccess ||
+      hipMalloc(&dval, sizeof(float) * static_cast<size_t>(p->nnz > 0 ? p->nnz : 1)) != hipSuccess)
+    st = SPMM_ERR_ALLOC;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!st && (hipMemsetAsync(X, 0, feat * elem, p->cap) != hipSuccess ||
+              hipMemsetAsync(dY, 0, feat * elem, p->cap) != hipSuccess ||
+              hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = SPMM_ERR_HIP;
+  float best = 1e30f;
+  const int lanes_max = p->N / e16;
+  for (int vec = 1; vec <= 2 && !st; ++vec) {
+    if (p->N % (e16 * vec)) continue;
+    const int lanes = p->N / (e16 * vec);
+    for (int G = 64; G >= 4 && !st; G >>= 1) {
+      if (lanes % G || G > lanes || G * 4 < std::min(lanes, 64)) continue;
+      for (int rs = 1; rs <= 8 && !st; rs <<= 1) {
+        for (int pers = 0; pers <= 1 && !st; ++pers) {
+          st = enqueue_step(p, G, vec, rs, pers, X, dY, Y, dX, dval, p->cap);  // warm-up
+          if (st) break;
+          if (hipEventRecord(e0, p->cap) != hipSuccess) { st = SPMM_ERR_HIP; break; }
+          for (int r = 0; r < 3 && !st; ++r)
+            st = enqueue_step(p, G, vec, rs, pers, X, dY, Y, dX, dval, p->cap);
+          if (st) break;
+          if (hipEventRecord(e1, p->cap) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            st = SPMM_ERR_HIP;
+            break;
+          }
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (ms < best) {
+            best = ms;
+            p->G = G;
+            p->VEC = vec;
+            p->rows_scale = rs;
+            p->persistent = pers;
+          }
+        }
+      }
+    }
   }
-  return st;
+  (void)lanes_max;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(p->cap);
+  if (X) (void)hipFree(X);
+  if (dY) (void)hipFree(dY);
+  if (Y) (void)hipFree(Y);
+  if (dX) (void)hipFree(dX);
+  if (dval) (void)hipFree(dval);
+  if (st) return st;
+  if (p->G == 0) return SPMM_ERR_UNSUPPORTED;
+  p->tuned_ms = best / 3.0f;
+  return SPMM_OK;
 }
 
 }  // namespace spmm
@@ -88,6 +279,7 @@ int spmm_batch_plan_destroy(spmm_batch_plan_t p) {
   if (p->trowptr) (void)hipFree(p->trowptr);
   if (p->tcol) (void)hipFree(p->tcol);
   if (p->tval) (void)hipFree(p->tval);
+  if (p->counters) (void)hipFree(p->counters);
   delete p;
   return SPMM_OK;
 }
@@ -104,6 +296,9 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off, con
   for (int g = 0; g < ngraphs; ++g)
     SPMM_CHECK_ARG(row_off[g + 1] >= row_off[g] && nnz_off[g + 1] >= nnz_off[g],
                    "offsets must be non-decreasing");
+  const int e16 = bf16 ? 8 : 4;
+  if (N % e16 || sddmm_G(N, e16) < 4)
+    SPMM_FAIL(SPMM_ERR_UNSUPPORTED, "batched plans need N a multiple of 16 (FP32) or 32 (BF16)");
   hipStream_t s = as_stream(stream);
   auto *p = new (std::nothrow) spmm_batch_plan_s();
   if (!p) SPMM_FAIL(SPMM_ERR_ALLOC, "host allocation failed");
@@ -124,9 +319,11 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off, con
   if (hipMalloc(&p->gcol, sizeof(int) * nn) != hipSuccess ||
       hipMalloc(&p->trowptr, sizeof(int) * (static_cast<size_t>(M) + 1)) != hipSuccess ||
       hipMalloc(&p->tcol, sizeof(int) * nn) != hipSuccess ||
-      hipMalloc(&p->tval, sizeof(float) * nn) != hipSuccess)
+      hipMalloc(&p->tval, sizeof(float) * nn) != hipSuccess ||
+      hipMalloc(&p->counters, sizeof(int) * 2) != hipSuccess)
     return fail(SPMM_ERR_ALLOC, "device allocation failed");
-  // Offsets to the device (the handle workspace is free until the transpose).
+  if (hipMemsetAsync(p->counters, 0, sizeof(int) * 2, s) != hipSuccess)
+    return fail(SPMM_ERR_HIP, "counter init");
   int *d_off = static_cast<int *>(workspace(h, sizeof(int) * 2 * (static_cast<size_t>(ngraphs) + 1)));
   if (!d_off) return fail(SPMM_ERR_ALLOC, "workspace");
   if (hipMemcpyAsync(d_off, row_off, sizeof(int) * (ngraphs + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -148,7 +345,23 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off, con
   if (st) { spmm_batch_plan_destroy(p); return st; }
   if (hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
     return fail(SPMM_ERR_HIP, "capture stream");
+  // Persistent grid: resident workgroups of the widest instantiation.
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return fail(SPMM_ERR_HIP, "device query");
+  p->grid = cus * 8;  // 8 workgroups of 4 waves per CU = 32 waves per CU
+  st = tune_plan(p);
+  if (st) { spmm_batch_plan_destroy(p); return st; }
   *out = p;
+  return SPMM_OK;
+}
+
+int spmm_batch_plan_info(spmm_batch_plan_t p, spmm_tune_t *tune, int *persistent, float *tuned_ms) {
+  SPMM_CHECK_ARG(p, "null plan");
+  if (tune) *tune = {p->G, p->VEC, p->rows_scale};
+  if (persistent) *persistent = p->persistent;
+  if (tuned_ms) *tuned_ms = p->tuned_ms;
   return SPMM_OK;
 }
 
@@ -157,16 +370,22 @@ int spmm_batch_fwd_bwd(spmm_batch_plan_t p, const void *X, const void *dY, float
   SPMM_CHECK_ARG(p, "null plan");
   SPMM_CHECK_ARG(X && dY && Y && dX && (dval || p->nnz == 0), "null operand");
   hipStream_t s = as_stream(stream);
-  if (!use_graph) return enqueue_step(p, X, dY, Y, dX, dval, s);
+  if (!use_graph) {
+    const int st = enqueue_step(p, p->G, p->VEC, p->rows_scale, p->persistent, X, dY, Y, dX, dval, s);
+    if (st) SPMM_FAIL(st, "fused launch failed");
+    return SPMM_OK;
+  }
   if (!p->exec || p->gX != X || p->gdY != dY || p->gY != Y || p->gdX != dX || p->gdval != dval) {
     destroy_graph(p);
+    // Capture only records the launch; the graph runs on s below.
     SPMM_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-    int st = enqueue_step(p, X, dY, Y, dX, dval, p->cap);
+    const int st = enqueue_step(p, p->G, p->VEC, p->rows_scale, p->persistent, X, dY, Y, dX, dval,
+                                p->cap);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(p->cap, &g);
+    const hipError_t e = hipStreamEndCapture(p->cap, &g);
     if (st) {
       if (g) (void)hipGraphDestroy(g);
-      return st;
+      SPMM_FAIL(st, "capture of the fused launch failed");
     }
     SPMM_HIP(e);
     p->graph = g;

@@ -59,6 +59,10 @@ struct spmm_handle_s {
   size_t ws_bytes = 0;
   void *pinned = nullptr;  // small pinned host buffer for count read-backs
   size_t pinned_bytes = 0;
+  // B2SR long-block-row list: [count, done, list...]; counters start at 0
+  // and are reset by the kernel that drains the list.
+  int *b2sr_state = nullptr;
+  int b2sr_cap = 0;
 };
 
 namespace spmm {
@@ -79,24 +83,35 @@ __device__ __forceinline__ float bf16_hi(uint32_t u) {
   return __uint_as_float(u & 0xffff0000u);
 }
 
-// 16-byte operand fetch: four floats or eight bf16 widened to float.
+// 16-byte operand fetch: four floats or eight bf16 widened to float. The
+// raw load and the widening are separate so a batch of loads can be issued
+// back to back and waited for once (widening right after each load would
+// force a wait per load).
 template <typename TB> struct Vec16;
 template <> struct Vec16<float> {
   static constexpr int kElems = 4;
-  __device__ __forceinline__ static void load(const float *p, float *out) {
-    float4 v = *reinterpret_cast<const float4 *>(p);
+  using Raw = float4;
+  __device__ __forceinline__ static Raw raw(const float *p) {
+    return *reinterpret_cast<const float4 *>(p);
+  }
+  __device__ __forceinline__ static void widen(const Raw &v, float *out) {
     out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
   }
+  __device__ __forceinline__ static void load(const float *p, float *out) { widen(raw(p), out); }
 };
 template <> struct Vec16<uint16_t> {
   static constexpr int kElems = 8;
-  __device__ __forceinline__ static void load(const uint16_t *p, float *out) {
-    uint4 v = *reinterpret_cast<const uint4 *>(p);
+  using Raw = uint4;
+  __device__ __forceinline__ static Raw raw(const uint16_t *p) {
+    return *reinterpret_cast<const uint4 *>(p);
+  }
+  __device__ __forceinline__ static void widen(const Raw &v, float *out) {
     out[0] = bf16_lo(v.x); out[1] = bf16_hi(v.x);
     out[2] = bf16_lo(v.y); out[3] = bf16_hi(v.y);
     out[4] = bf16_lo(v.z); out[5] = bf16_hi(v.z);
     out[6] = bf16_lo(v.w); out[7] = bf16_hi(v.w);
   }
+  __device__ __forceinline__ static void load(const uint16_t *p, float *out) { widen(raw(p), out); }
 };
 
 template <typename TB>

@@ -1,0 +1,361 @@
+// Device building blocks shared by the plain launches (csr_spmm.hip,
+// adjoint.hip) and the fused batched launch (batched.hip): the launch table
+// of row bins, the per-subgroup gather loop, the work of one 256-thread
+// CSR workgroup and of one SDDMM workgroup.
+#pragma once
+
+#include "common.hpp"
+
+namespace spmm {
+
+constexpr int kBins = 32;
+constexpr int kThreads = 256;  // 4 waves
+constexpr int kWavesPerWG = kThreads / kWave;
+
+// ------------------------------------------------------------ launch table
+enum SegKind : int { kSegA = 0, kSegB = 1, kSegC = 2 };
+
+struct Seg {
+  int list_off;   // first position in the permuted row list
+  int count;      // rows in this segment
+  int kind;       // SegKind
+  int rows_per_sub;  // R, class A only
+  int wg_begin;   // first workgroup (blockIdx.x) of this segment
+};
+
+struct LaunchTable {
+  int nseg;
+  int total_wg;
+  Seg seg[kBins];
+};
+
+// One subgroup (G lanes) accumulates entries [e_begin, e_end) taking chunks
+// of G entries at stride `chunk_stride`, starting at chunk `first`.
+template <typename TB, int G, int EPL>
+__device__ __forceinline__ void accumulate_range(
+    int e_begin, int e_end, int first_chunk_off, int chunk_stride, int lane_sub,
+    const int *__restrict__ colind, const float *__restrict__ val,
+    const TB *__restrict__ Bc, int ldb, float (&acc)[EPL]) {
+  constexpr int U0 = EPL >= 16 ? 4 : 8;  // loads in flight per lane
+  constexpr int U = G < U0 ? G : U0;
+  constexpr int V = EPL / Vec16<TB>::kElems;
+  int e = e_begin + first_chunk_off;
+  int cidx = 0;
+  float v = 0.f;
+  if (e + lane_sub < e_end) {
+    cidx = colind[e + lane_sub];
+    v = val[e + lane_sub];
+  }
+  for (; e < e_end; e += chunk_stride) {
+    const int n = min(G, e_end - e);
+    // the next chunk's indices are requested before this chunk's gathers
+    const int my_n = e + chunk_stride + lane_sub;
+    int cidx_n = 0;
+    float v_n = 0.f;
+    if (my_n < e_end) {
+      cidx_n = colind[my_n];
+      v_n = val[my_n];
+    }
+    for (int k = 0; k < n; k += U) {
+      int cc[U];
+      float vv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        cc[u] = __shfl(cidx, k + u, G);
+        vv[u] = __shfl(v, k + u, G);
+      }
+      // All U loads are issued unconditionally (slots past n read row
+      // cc = 0 of the padding lanes, always in range) and waited for once;
+      // widening and accumulation are predicated.
+      typename Vec16<TB>::Raw raw[U][V];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const TB *p = Bc + static_cast<size_t>(cc[u]) * ldb;
+#pragma unroll
+        for (int q = 0; q < V; ++q) raw[u][q] = Vec16<TB>::raw(p + q * G * Vec16<TB>::kElems);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (k + u < n) {
+          float b[EPL];
+#pragma unroll
+          for (int q = 0; q < V; ++q) Vec16<TB>::widen(raw[u][q], &b[q * Vec16<TB>::kElems]);
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) acc[q] = fmaf(vv[u], b[q], acc[q]);
+        }
+      }
+    }
+    cidx = cidx_n;
+    v = v_n;
+  }
+}
+
+// Lane layout of a row segment: vector q of lane l covers columns
+// q*G*E + l*E .. +E (E = elements per 16-byte vector), so each load or
+// store instruction of a subgroup touches one contiguous run of G*16 bytes.
+template <int G, int E, int EPL>
+__device__ __forceinline__ void store_row(float *__restrict__ Crow, const float (&acc)[EPL]) {
+#pragma unroll
+  for (int q = 0; q < EPL / E; ++q) {
+#pragma unroll
+    for (int k = 0; k < E; k += 4)
+      *reinterpret_cast<float4 *>(Crow + q * G * E + k) =
+          make_float4(acc[q * E + k], acc[q * E + k + 1], acc[q * E + k + 2], acc[q * E + k + 3]);
+  }
+}
+
+// Butterfly over the S subgroups of a wave: every subgroup ends with the
+// same sum (IEEE addition is commutative, so partners agree bitwise).
+template <int G, int EPL>
+__device__ __forceinline__ void wave_reduce_subgroups(float (&acc)[EPL]) {
+#pragma unroll
+  for (int m = G; m < kWave; m <<= 1) {
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
+  }
+}
+
+// Segment of workgroup `wg`. The unrolled select keeps the table in SGPRs
+// (a dynamic index into a by-value kernel argument would spill to scratch).
+__device__ __forceinline__ Seg find_seg(const LaunchTable &tab, int wg) {
+  Seg sg = tab.seg[0];
+#pragma unroll
+  for (int i = 1; i < kBins; ++i) {
+    if (i < tab.nseg && wg >= tab.seg[i].wg_begin) sg = tab.seg[i];
+  }
+  return sg;
+}
+
+// The work of one 256-thread workgroup of the CSR launch table: virtual
+// workgroup `wg`, column tile `ctile`. Used by csr_spmm_kernel and by the
+// fused batched kernel.
+template <typename TB, int G, int VEC>
+__device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile,
+                                       const int *__restrict__ rows,
+                                       const int *__restrict__ rowptr,
+                                       const int *__restrict__ colind,
+                                       const float *__restrict__ val,
+                                       const TB *__restrict__ B, int ldb,
+                                       float *__restrict__ C, int ldc) {
+  constexpr int EPL = VEC * Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * EPL;  // columns covered per column tile
+  __shared__ float red[kWavesPerWG - 1][CW];
+
+  const Seg sg = find_seg(tab, wg);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1);
+  const int sub = lane / G;  // subgroup within the wave
+  constexpr int E = Vec16<TB>::kElems;
+  const int col0 = ctile * CW + lane_sub * E;
+  const TB *Bc = B + col0;
+  const int local_wg = wg - sg.wg_begin;
+
+  if (sg.kind == kSegA) {
+    const int R = sg.rows_per_sub;
+    const int subs_per_wg = kWavesPerWG * S;
+    const int base = sg.list_off + local_wg * subs_per_wg * R;
+    const int end = sg.list_off + sg.count;
+    const int my_sub = wave * S + sub;
+    // Two-stage software pipeline over the subgroup's R rows: the next row's
+    // row pointers and the id of the row after it are requested before the
+    // current row's gathers, so only colind -> gather stays on the critical
+    // path of each row.
+    int pos = base + my_sub;
+    int row = pos < end ? rows[pos] : -1;
+    int e0 = 0, e1 = 0;
+    if (row >= 0) {
+      e0 = rowptr[row];
+      e1 = rowptr[row + 1];
+    }
+    int row_n = (R > 1 && pos + subs_per_wg < end) ? rows[pos + subs_per_wg] : -1;
+    for (int k = 0; k < R && row >= 0; ++k) {
+      int e0n = 0, e1n = 0;
+      if (row_n >= 0) {
+        e0n = rowptr[row_n];
+        e1n = rowptr[row_n + 1];
+      }
+      const int pos_nn = pos + 2 * subs_per_wg;
+      const int row_nn = (k + 2 < R && pos_nn < end) ? rows[pos_nn] : -1;
+      float acc[EPL];
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+      accumulate_range<TB, G, EPL>(e0, e1, 0, G, lane_sub, colind, val, Bc, ldb, acc);
+      store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+      row = row_n;
+      e0 = e0n;
+      e1 = e1n;
+      row_n = row_nn;
+      pos += subs_per_wg;
+    }
+  } else if (sg.kind == kSegB) {
+    const int pos = sg.list_off + local_wg * kWavesPerWG + wave;
+    if (pos >= sg.list_off + sg.count) return;  // whole wave
+    const int row = rows[pos];
+    float acc[EPL];
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+    accumulate_range<TB, G, EPL>(rowptr[row], rowptr[row + 1], sub * G, S * G,
+                                 lane_sub, colind, val, Bc, ldb, acc);
+    wave_reduce_subgroups<G, EPL>(acc);
+    if (sub == 0) store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+  } else {
+    const int row = rows[sg.list_off + local_wg];
+    float acc[EPL];
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+    const int gsub = wave * S + sub;
+    accumulate_range<TB, G, EPL>(rowptr[row], rowptr[row + 1], gsub * G,
+                                 kWavesPerWG * S * G, lane_sub, colind, val, Bc,
+                                 ldb, acc);
+    wave_reduce_subgroups<G, EPL>(acc);
+    if (wave > 0 && sub == 0) {
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) red[wave - 1][lane_sub * EPL + q] = acc[q];
+    }
+    __syncthreads();
+    if (wave == 0 && sub == 0) {
+#pragma unroll
+      for (int w = 0; w < kWavesPerWG - 1; ++w) {
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
+      }
+      store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+    }
+    __syncthreads();  // red[] may be reused by the next work item
+  }
+}
+
+
+// ---------------------------------------------------------------- SDDMM
+// Entry-chunked (merge-path) SDDMM: each subgroup of G lanes (G <= 32) owns
+// kSddmmChunk consecutive stored entries, whatever rows they fall in, so a
+// 20k-entry hub row is spread over many subgroups and no subgroup waits on
+// one. The subgroup finds the row of its first entry with a G-ary search of
+// rowptr (G probes per round, ballot), then walks the rows its chunk covers.
+// Per row it keeps the dY segment in registers; entries are taken L =
+// min(8, G) at a time: lane l forms the partial dots of the L entries over
+// its EPL columns, a reduce-scatter over the low log2(L) lane bits (L-1
+// shuffles) leaves entry (l mod L)'s partial in lane l, and a butterfly over
+// the remaining lane bits completes it. Lanes 0..L-1 store (coalesced).
+// Column tiles beyond the first add into out. Outputs are per entry, so no
+// cross-chunk fix-up exists and the result is bitwise reproducible.
+constexpr int kSddmmChunk = 32;
+
+template <typename TB, int G>
+__device__ __forceinline__ void sddmm_wg(int wg, int M, int nnz, const int *__restrict__ rowptr,
+                                         const int *__restrict__ colind,
+                                         const TB *__restrict__ dY, int lddy,
+                                         const TB *__restrict__ X, int ldx,
+                                         float *__restrict__ out, int ntile) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * EPL;
+  constexpr int L = G < 8 ? G : 8;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int lane_sub = lane & (G - 1);
+  const int sub = lane / G;
+  const int gsub = (threadIdx.x / kWave) * S + sub;
+  const long long e_beg_ll = (static_cast<long long>(wg) * (kThreads / G) + gsub) * kSddmmChunk;
+  if (e_beg_ll >= nnz) return;
+  const int e_beg = static_cast<int>(e_beg_ll);
+  const int e_end = min(nnz, e_beg + kSddmmChunk);
+  // Largest r with rowptr[r] <= e_beg (rowptr[0] = 0 <= e_beg < rowptr[M]).
+  int lo = 0, hi = M;
+  const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  while (hi - lo > 1) {
+    const int step = (hi - lo + G) / (G + 1);
+    const int p = lo + (lane_sub + 1) * step;
+    const bool le = p < hi && rowptr[p] <= e_beg;
+    const unsigned long long m = (__ballot(le) >> (sub * G)) & gmask;
+    lo += __popcll(m) * step;
+    hi = min(hi, lo + step);
+  }
+  for (int tt = 0; tt < ntile; ++tt) {
+    const TB *Xc = X + tt * CW + lane_sub * EPL;
+    int row = lo;
+    int e = e_beg;
+    while (e < e_end) {
+      int rend = rowptr[row + 1];
+      while (rend <= e) rend = rowptr[++row + 1];  // skip rows with no entry left
+      const int stop = min(rend, e_end);
+      float y[EPL];
+      Vec16<TB>::load(dY + static_cast<size_t>(row) * lddy + tt * CW + lane_sub * EPL, y);
+      for (; e < stop; e += L) {
+        const int n = min(L, stop - e);
+        const int c = lane_sub < n ? colind[e + lane_sub] : 0;
+        // unconditional loads (slots past n read row 0), one wait
+        typename Vec16<TB>::Raw xr[L];
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+          const int j = __shfl(c, u, G);
+          xr[u] = Vec16<TB>::raw(Xc + static_cast<size_t>(j) * ldx);
+        }
+        float p[L];
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+          float d = 0.f;
+          if (u < n) {
+            float x[EPL];
+            Vec16<TB>::widen(xr[u], x);
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) d = fmaf(y[q], x[q], d);
+          }
+          p[u] = d;
+        }
+#pragma unroll
+        for (int m = L / 2; m >= 1; m >>= 1) {
+          const bool hi_half = (lane_sub & m) != 0;
+#pragma unroll
+          for (int i = 0; i < m; ++i) {
+            const float send = hi_half ? p[i] : p[i + m];
+            const float keep = hi_half ? p[i + m] : p[i];
+            p[i] = keep + __shfl_xor(send, m, G);
+          }
+        }
+#pragma unroll
+        for (int m = L; m < G; m <<= 1) p[0] += __shfl_xor(p[0], m, G);
+        if (lane_sub < n) {
+          if (tt == 0) out[e + lane_sub] = p[0];
+          else out[e + lane_sub] += p[0];
+        }
+      }
+      e = stop;  // the last group may have been partial
+      ++row;
+    }
+  }
+}
+
+// Host side.
+struct Geometry {
+  int G = 0;    // lanes per row (0 = scalar kernel)
+  int VEC = 1;  // 16-byte vectors per lane
+  int col_tiles = 1;
+  int rows_scale = 1;
+};
+
+constexpr int kTuneSlots = 8;
+struct TuneEntry {
+  int N;
+  int elem_bytes;
+  spmm_tune_t tune;
+};
+
+// Launch table for a plan's bin counts and lane geometry G.
+void build_table(const int *counts, const int *offs, int G, int rows_scale, LaunchTable *t);
+
+}  // namespace spmm
+
+// Row schedule of one CSR structure: rows permuted into degree bins
+// (highest bin first) plus the autotuned geometries.
+struct spmm_csr_plan_s {
+  int M = 0;
+  int nnz = 0;
+  int device = 0;
+  int *d_rows = nullptr;
+  int counts[32] = {0};
+  int offs[32] = {0};
+  int ntuned = 0;
+  spmm::TuneEntry tuned[spmm::kTuneSlots];
+};

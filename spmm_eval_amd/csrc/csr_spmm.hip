@@ -14,26 +14,13 @@
 //                    partial rows reduced through LDS in a fixed order.
 // Every row's summation order depends only on its degree, so results are
 // bitwise reproducible from run to run. No atomics touch C.
-#include "common.hpp"
+#include "csr_core.hpp"
 
 #include <algorithm>
 #include <new>
 #include <vector>
 
-struct spmm_csr_plan_s {
-  int M = 0;
-  int nnz = 0;
-  int device = 0;
-  int *d_rows = nullptr;  // rows permuted into bin order (highest bin first)
-  int counts[32] = {0};
-  int offs[32] = {0};
-};
-
 namespace spmm {
-
-constexpr int kBins = 32;
-constexpr int kThreads = 256;  // 4 waves
-constexpr int kWavesPerWG = kThreads / kWave;
 
 __device__ __forceinline__ int degree_bin(int deg) {
   return deg <= 0 ? 0 : 32 - __clz(deg);
@@ -79,169 +66,13 @@ bin_scatter_kernel(const int *__restrict__ rowptr, int M, int *__restrict__ curs
   }
 }
 
-// ------------------------------------------------------------ launch table
-enum SegKind : int { kSegA = 0, kSegB = 1, kSegC = 2 };
-
-struct Seg {
-  int list_off;   // first position in the permuted row list
-  int count;      // rows in this segment
-  int kind;       // SegKind
-  int rows_per_sub;  // R, class A only
-  int wg_begin;   // first workgroup (blockIdx.x) of this segment
-};
-
-struct LaunchTable {
-  int nseg;
-  int total_wg;
-  Seg seg[kBins];
-};
-
-// One subgroup (G lanes) accumulates entries [e_begin, e_end) taking chunks
-// of G entries at stride `chunk_stride`, starting at chunk `first`.
-template <typename TB, int G, int EPL>
-__device__ __forceinline__ void accumulate_range(
-    int e_begin, int e_end, int first_chunk_off, int chunk_stride, int lane_sub,
-    const int *__restrict__ colind, const float *__restrict__ val,
-    const TB *__restrict__ Bc, int ldb, float (&acc)[EPL]) {
-  constexpr int U0 = EPL >= 16 ? 4 : 8;  // loads in flight per lane
-  constexpr int U = G < U0 ? G : U0;
-  constexpr int V = EPL / Vec16<TB>::kElems;
-  for (int e = e_begin + first_chunk_off; e < e_end; e += chunk_stride) {
-    const int n = min(G, e_end - e);
-    const int my = e + lane_sub;
-    int cidx = 0;
-    float v = 0.f;
-    if (my < e_end) {
-      cidx = colind[my];
-      v = val[my];
-    }
-    for (int k = 0; k < n; k += U) {
-      int cc[U];
-      float vv[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        cc[u] = __shfl(cidx, k + u, G);
-        vv[u] = __shfl(v, k + u, G);
-      }
-      float b[U][EPL];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (k + u < n) {
-          const TB *p = Bc + static_cast<size_t>(cc[u]) * ldb;
-#pragma unroll
-          for (int q = 0; q < V; ++q)
-            Vec16<TB>::load(p + q * Vec16<TB>::kElems, &b[u][q * Vec16<TB>::kElems]);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (k + u < n) {
-#pragma unroll
-          for (int q = 0; q < EPL; ++q) acc[q] = fmaf(vv[u], b[u][q], acc[q]);
-        }
-      }
-    }
-  }
-}
-
-template <int EPL>
-__device__ __forceinline__ void store_row(float *__restrict__ Crow, const float (&acc)[EPL]) {
-#pragma unroll
-  for (int q = 0; q < EPL; q += 4)
-    *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
-}
-
-// Butterfly over the S subgroups of a wave: every subgroup ends with the
-// same sum (IEEE addition is commutative, so partners agree bitwise).
-template <int G, int EPL>
-__device__ __forceinline__ void wave_reduce_subgroups(float (&acc)[EPL]) {
-#pragma unroll
-  for (int m = G; m < kWave; m <<= 1) {
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
-  }
-}
-
 template <typename TB, int G, int VEC>
 __global__ void __launch_bounds__(kThreads)
 csr_spmm_kernel(LaunchTable tab, const int *__restrict__ rows,
                 const int *__restrict__ rowptr, const int *__restrict__ colind,
                 const float *__restrict__ val, const TB *__restrict__ B, int ldb,
                 float *__restrict__ C, int ldc) {
-  constexpr int EPL = VEC * Vec16<TB>::kElems;
-  constexpr int S = kWave / G;
-  constexpr int CW = G * EPL;  // columns covered per column tile
-  __shared__ float red[kWavesPerWG - 1][CW];
-
-  const int wg = blockIdx.x;
-  // Unrolled select keeps the table in SGPRs (a dynamic index into the
-  // by-value argument would spill it to scratch).
-  Seg sg = tab.seg[0];
-#pragma unroll
-  for (int i = 1; i < kBins; ++i) {
-    if (i < tab.nseg && wg >= tab.seg[i].wg_begin) sg = tab.seg[i];
-  }
-
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int lane_sub = lane & (G - 1);
-  const int sub = lane / G;  // subgroup within the wave
-  const int col0 = blockIdx.y * CW + lane_sub * EPL;
-  const TB *Bc = B + col0;
-  const int local_wg = wg - sg.wg_begin;
-
-  if (sg.kind == kSegA) {
-    const int R = sg.rows_per_sub;
-    const int subs_per_wg = kWavesPerWG * S;
-    const int base = sg.list_off + local_wg * subs_per_wg * R;
-    const int end = sg.list_off + sg.count;
-    const int my_sub = wave * S + sub;
-    for (int k = 0; k < R; ++k) {
-      const int pos = base + k * subs_per_wg + my_sub;
-      if (pos >= end) break;
-      const int row = rows[pos];
-      float acc[EPL];
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-      accumulate_range<TB, G, EPL>(rowptr[row], rowptr[row + 1], 0, G, lane_sub,
-                                   colind, val, Bc, ldb, acc);
-      store_row<EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
-    }
-  } else if (sg.kind == kSegB) {
-    const int pos = sg.list_off + local_wg * kWavesPerWG + wave;
-    if (pos >= sg.list_off + sg.count) return;
-    const int row = rows[pos];
-    float acc[EPL];
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    accumulate_range<TB, G, EPL>(rowptr[row], rowptr[row + 1], sub * G, S * G,
-                                 lane_sub, colind, val, Bc, ldb, acc);
-    wave_reduce_subgroups<G, EPL>(acc);
-    if (sub == 0) store_row<EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
-  } else {
-    const int row = rows[sg.list_off + local_wg];
-    float acc[EPL];
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    const int gsub = wave * S + sub;
-    accumulate_range<TB, G, EPL>(rowptr[row], rowptr[row + 1], gsub * G,
-                                 kWavesPerWG * S * G, lane_sub, colind, val, Bc,
-                                 ldb, acc);
-    wave_reduce_subgroups<G, EPL>(acc);
-    if (wave > 0 && sub == 0) {
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) red[wave - 1][lane_sub * EPL + q] = acc[q];
-    }
-    __syncthreads();
-    if (wave == 0 && sub == 0) {
-#pragma unroll
-      for (int w = 0; w < kWavesPerWG - 1; ++w) {
-#pragma unroll
-        for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
-      }
-      store_row<EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
-    }
-  }
+  csr_wg<TB, G, VEC>(tab, blockIdx.x, blockIdx.y, rows, rowptr, colind, val, B, ldb, C, ldc);
 }
 
 // Any N, any alignment: one wave per row, lanes stride over the columns.
@@ -264,59 +95,63 @@ csr_spmm_generic_kernel(int M, int N, const int *__restrict__ rowptr,
 }
 
 // ------------------------------------------------------------------- host
-struct Geometry {
-  int G = 0;    // lanes per row (0 = generic kernel)
-  int VEC = 1;  // 16-byte vectors per lane
-  int col_tiles = 1;
-  int rows_scale = 1;
-};
+static bool tune_valid(int N, int e16, int G, int VEC) {
+  if (G < 1 || G > 64 || (G & (G - 1)) || (VEC != 1 && VEC != 2)) return false;
+  const int epl = e16 * VEC;
+  return N % (G * epl) == 0;
+}
 
-static Geometry pick_geometry(int N, int elem_bytes, const void *B, int ldb,
-                              const void *C, int ldc, const spmm_tune_t *tune) {
+// Lane geometry for one call: an explicit tune, else the plan's autotuned
+// entry for (N, dtype), else the widest power-of-two subgroup that tiles N.
+// G == 0 selects the scalar kernel (unaligned operands or N % e16 != 0).
+static Geometry pick_geometry(const spmm_csr_plan_s *p, int N, int elem_bytes, const void *B,
+                              int ldb, const void *C, int ldc, const spmm_tune_t *tune) {
   Geometry g;
   const int e16 = 16 / elem_bytes;
   const bool aligned = aligned16(B) && aligned16(C) && (ldb % e16 == 0) && (ldc % 4 == 0);
-  if (tune && tune->rows_scale > 0) g.rows_scale = tune->rows_scale;
   if (!aligned || N % e16 != 0) return g;
-  int vec = (tune && tune->vec == 2) ? 2 : 1;
+  spmm_tune_t t = {0, 0, 0};
+  if (tune) {
+    t = *tune;
+  } else if (p) {
+    for (int i = 0; i < p->ntuned; ++i)
+      if (p->tuned[i].N == N && p->tuned[i].elem_bytes == elem_bytes) t = p->tuned[i].tune;
+  }
+  g.rows_scale = t.rows_scale > 0 ? t.rows_scale : 1;
+  int vec = t.vec == 2 ? 2 : 1;
   if (N % (e16 * vec) != 0) vec = 1;
-  const int epl = e16 * vec;
-  const int lanes_needed = N / epl;
-  int G = 0;
-  if (tune && tune->lanes_per_row > 0) {
-    int t = tune->lanes_per_row;
-    if ((t & (t - 1)) == 0 && t <= 64 && N % (t * epl) == 0) G = t;
-  }
+  int G = tune_valid(N, e16, t.lanes_per_row, vec) ? t.lanes_per_row : 0;
   if (G == 0) {
-    for (int t = 64; t >= 1; t >>= 1) {
-      if (t <= lanes_needed && lanes_needed % t == 0) { G = t; break; }
-    }
+    const int lanes = N / (e16 * vec);
+    for (int c = 64; c >= 1; c >>= 1)
+      if (c <= lanes && lanes % c == 0) { G = c; break; }
   }
-  if (G == 0) return g;
   g.G = G;
   g.VEC = vec;
-  g.col_tiles = N / (G * epl);
+  g.col_tiles = N / (G * e16 * vec);
   return g;
 }
 
-static void build_table(const spmm_csr_plan_s *p, int G, int rows_scale, LaunchTable *t) {
+void build_table(const int *counts, const int *offs, int G, int rows_scale, LaunchTable *t) {
   const int S = kWave / G;
   const int a_limit = std::max(2 * G, 8);   // deg < a_limit -> one subgroup per row
-  const int c_limit = 256 * S;              // deg >= c_limit -> one workgroup per row
+  const int c_limit = 64 * S;               // deg >= c_limit -> one workgroup per row
   t->nseg = 0;
   int wg = 0;
   for (int b = kBins - 1; b >= 0; --b) {
-    const int cnt = p->counts[b];
+    const int cnt = counts[b];
     if (cnt == 0) continue;
     const long long lo = b == 0 ? 0 : (1ll << (b - 1));
     const long long hi = b == 0 ? 1 : (1ll << b);  // exclusive upper bound
     Seg s;
-    s.list_off = p->offs[b];
+    s.list_off = offs[b];
     s.count = cnt;
     s.wg_begin = wg;
     if (hi <= a_limit) {
       s.kind = kSegA;
-      int R = b == 0 ? 64 : std::max(1, static_cast<int>(64 / hi));
+      // Each row costs a dependent index -> gather round trip, so short rows
+      // get few rows per subgroup and many workgroups.
+      int R = b == 0 ? 16 : std::max(1, static_cast<int>(8 / hi));
       R = std::max(1, R * rows_scale);
       s.rows_per_sub = R;
       wg += ceil_div(cnt, kWavesPerWG * S * R);
@@ -374,7 +209,7 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
   SPMM_CHECK_ARG(nnz == 0 || (colind && val && B), "null colind, val or B");
   SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldb >= N), "ldb/ldc smaller than N");
   hipStream_t s = as_stream(stream);
-  Geometry g = pick_geometry(N, sizeof(TB), B, ldb, C, ldc, tune);
+  Geometry g = pick_geometry(p, N, sizeof(TB), B, ldb, C, ldc, tune);
   if (g.G == 0) {
     hipLaunchKernelGGL((csr_spmm_generic_kernel<TB>), dim3(ceil_div(M, kWavesPerWG)),
                        dim3(kThreads), 0, s, M, N, rowptr, colind, val, B, ldb, C, ldc);
@@ -382,13 +217,64 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
     return SPMM_OK;
   }
   LaunchTable t;
-  build_table(p, g.G, g.rows_scale, &t);
+  build_table(p->counts, p->offs, g.G, g.rows_scale, &t);
   if (t.total_wg == 0) return SPMM_OK;
   if (g.VEC == 2)
     dispatch_G<TB, 2>(g.G, t, g.col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s);
   else
     dispatch_G<TB, 1>(g.G, t, g.col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s);
   SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
+// Times every valid (lanes per row, vector width, rows per subgroup) on the
+// caller's operands and records the fastest in the plan for (N, dtype).
+template <typename TB>
+static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int *rowptr,
+                    const int *colind, const float *val, const TB *B, int ldb, float *C,
+                    int ldc, spmm_tune_t *best_out, void *stream) {
+  SPMM_CHECK_ARG(p != nullptr, "plan is null");
+  hipStream_t s = as_stream(stream);
+  const int e16 = 16 / static_cast<int>(sizeof(TB));
+  spmm_tune_t best = {0, 0, 0};
+  float best_ms = 1e30f;
+  hipEvent_t e0, e1;
+  SPMM_HIP(hipEventCreate(&e0));
+  SPMM_HIP(hipEventCreate(&e1));
+  int st = SPMM_OK;
+  for (int vec = 1; vec <= 2 && st == SPMM_OK; ++vec) {
+    if (N % (e16 * vec)) continue;
+    const int lanes = N / (e16 * vec);
+    for (int G = 64; G >= 1 && st == SPMM_OK; G >>= 1) {
+      if (!tune_valid(N, e16, G, vec) || G * 4 < std::min(lanes, 64)) continue;  // <= 4 column tiles
+      for (int rs = 1; rs <= 8 && st == SPMM_OK; rs <<= 1) {
+        spmm_tune_t cand = {G, vec, rs};
+        st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, &cand, stream);
+        if (st) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { st = SPMM_ERR_HIP; break; }
+        for (int r = 0; r < 3 && st == SPMM_OK; ++r)
+          st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, &cand, stream);
+        if (st) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          st = SPMM_ERR_HIP;
+          break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) { best_ms = ms; best = cand; }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  if (best.lanes_per_row == 0) SPMM_FAIL(SPMM_ERR_UNSUPPORTED, "no vector geometry fits this N / alignment");
+  int slot = -1;
+  for (int i = 0; i < p->ntuned; ++i)
+    if (p->tuned[i].N == N && p->tuned[i].elem_bytes == static_cast<int>(sizeof(TB))) slot = i;
+  if (slot < 0) slot = p->ntuned < kTuneSlots ? p->ntuned++ : kTuneSlots - 1;
+  p->tuned[slot] = {N, static_cast<int>(sizeof(TB)), best};
+  if (best_out) *best_out = best;
   return SPMM_OK;
 }
 
@@ -474,6 +360,16 @@ int spmm_csr_plan_destroy(spmm_csr_plan_t plan) {
   if (plan->d_rows) (void)hipFree(plan->d_rows);
   delete plan;
   return SPMM_OK;
+}
+
+int spmm_csr_autotune(spmm_csr_plan_t plan, int M, int K, int N, int nnz, const int *rowptr,
+                      const int *colind, const float *val, const void *B, int bf16, int ldb,
+                      float *C, int ldc, spmm_tune_t *best, void *stream) {
+  if (bf16)
+    return spmm::autotune<uint16_t>(plan, M, K, N, nnz, rowptr, colind, val,
+                                    static_cast<const uint16_t *>(B), ldb, C, ldc, best, stream);
+  return spmm::autotune<float>(plan, M, K, N, nnz, rowptr, colind, val,
+                               static_cast<const float *>(B), ldb, C, ldc, best, stream);
 }
 
 int spmm_csr_plan_bins(spmm_csr_plan_t plan, int *counts32) {

@@ -68,6 +68,7 @@ int spmm_destroy(spmm_handle_t h) {
   if (!h) return SPMM_OK;
   if (h->ws) (void)hipFree(h->ws);
   if (h->pinned) (void)hipHostFree(h->pinned);
+  if (h->b2sr_state) (void)hipFree(h->b2sr_state);
   delete h;
   return SPMM_OK;
 }

@@ -157,3 +157,26 @@ def test_dense_packers(dev, H, W):
     Xh = torch.tensor(sign["X"], dtype=torch.float32, device=dev)
     assert np_u32(api.pack_sign_rows(Xh)).reshape(-1).tolist() == sign["sign"]
     assert np_u32(api.pack_bin_rows(Xh)).reshape(-1).tolist() == sign["bin"]
+
+
+@pytest.mark.parametrize("T", [4, 32])
+@pytest.mark.parametrize("N", [64, 512])
+def test_fbf_long_block_rows_repeated(handle, dev, T, N):
+    """Block rows long enough to go through the long-row list (N=512 adds a
+    second column tile), three calls in a row with another matrix between
+    them: the list counters must come back to zero after every call."""
+    g = graphs.random_csr(64, 8192, 0.002, seed=T + N, long_rows=3, long_len=3000)
+    other = graphs.random_csr(300, 8192, 0.01, seed=5, long_rows=2, long_len=4000)
+    B = graphs.uniform_pm1((8192, N), seed=N)
+    Bd = torch.from_numpy(B).to(dev)
+    outs = []
+    for rep in range(3):
+        for mat in ((g, other) if rep == 1 else (g,)):
+            rp, ci = dev_csr(mat, dev)
+            brp, bci, words = api.csr2b2sr(handle, rp, ci, mat.M, mat.K, T)
+            C = api.spmm_b2sr(handle, T, mat.M, brp, bci, words, Bd).cpu().numpy()
+            ref = oracle.csr_spmm_f64(mat.rowptr, mat.colind, np.ones(mat.nnz, np.float32), B)
+            assert np.linalg.norm(C - ref) / np.linalg.norm(ref) <= 1e-5, (rep, mat.M)
+            if mat is g:
+                outs.append(C)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])

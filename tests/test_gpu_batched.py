@@ -1,7 +1,8 @@
-"""Batched uneven graphs: the one-plan block-diagonal fwd+bwd equals
-per-graph runs of the same kernels bit for bit, the hipGraph replay equals
-the eager step on every replay (with eager work and synchronisations
-between replays), and both sit within tolerance of the FP64 oracle."""
+"""Batched uneven graphs: the one-plan block-diagonal fwd+bwd (one fused,
+autotuned launch) equals per-graph runs of the same kernels at the same
+geometry bit for bit, the hipGraph replay equals the eager step on every
+replay (with eager work and synchronisations between replays), and both sit
+within tolerance of the FP64 oracle."""
 import numpy as np
 import pytest
 import torch
@@ -40,7 +41,9 @@ def test_batched_matches_per_graph_and_oracle(handle, dev, batch, bf16):
         torch.cuda.synchronize()
         assert torch.equal(Y, Yr) and torch.equal(dX, dXr) and torch.equal(dval, dvalr), rep
         del junk
-    # per-graph runs of the same kernels
+    # per-graph runs of the same kernels at the geometry the plan chose
+    info = plan.info()
+    tune = (info["lanes_per_row"], info["vec"], info["rows_scale"])
     for gi in (0, 7, len(gs) - 1):
         g = gs[gi]
         r0, r1 = int(row_off[gi]), int(row_off[gi + 1])
@@ -48,7 +51,7 @@ def test_batched_matches_per_graph_and_oracle(handle, dev, batch, bf16):
         rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
         v = torch.from_numpy(g.val).to(dev)
         p = api.CsrPlan(handle, rp, g.nnz)
-        Yi = api.spmm_csr(handle, rp, ci, v, Xd[r0:r1], plan=p)
+        Yi = api.spmm_csr(handle, rp, ci, v, Xd[r0:r1], plan=p, tune=tune)
         assert torch.equal(Yi, Y[r0:r1])
         dvi = api.sddmm_csr(handle, rp, ci, dYd[r0:r1], Xd[r0:r1])
         assert torch.equal(dvi, dval[e0:e1])

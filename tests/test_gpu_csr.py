@@ -161,3 +161,25 @@ def test_csr_large_workload_properties(handle, dev):
     lhs = api.spmm_csr(handle, rp, ci, v, 0.5 * Bd + B2, plan=plan)
     rhs = 0.5 * api.spmm_csr(handle, rp, ci, v, Bd, plan=plan) + api.spmm_csr(handle, rp, ci, v, B2, plan=plan)
     assert torch.linalg.norm(lhs - rhs) / torch.linalg.norm(rhs) < 1e-5
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_autotune_picks_a_valid_geometry(handle, dev, bf16):
+    g = graphs.chung_lu(20000, 400000, seed=8)
+    g.val = graphs.row_normalised_uniform(g, 8)
+    N = 64
+    B = graphs.uniform_pm1((g.K, N), seed=9)
+    rp, ci, v = to_dev(g, dev)
+    Bd = torch.from_numpy(B).to(dev)
+    if bf16:
+        Bd = api.to_bf16_bits(Bd)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    C = torch.empty((g.M, N), device=dev)
+    G, vec, rs = plan.autotune(rp, ci, v, Bd, C)
+    assert G in (4, 8, 16, 32, 64) and vec in (1, 2) and rs in (1, 2, 4, 8)
+    C1 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan)            # uses the tuned entry
+    C2 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=(G, vec, rs))
+    assert torch.equal(C1, C2)
+    ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B) if not bf16 else \
+        oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, Bd.view(torch.int16).cpu().numpy().view(np.uint16))
+    assert rel_fro(C1.cpu().numpy(), ref) <= 1e-5
