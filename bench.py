@@ -52,6 +52,34 @@ def sddmm_bytes(M, nnz, N, s=4):
     return 4 * (M + 1) + 8 * nnz + nnz * N * s + M * N * s
 
 
+def b2sr_lanes(N, elem_bytes=4):
+    """Lanes per row the FBF kernels use for an aligned N (widest power of
+    two dividing N / (16 / elem_bytes), at most 64)."""
+    lanes = N // (16 // elem_bytes)
+    t = 64
+    while t > 1 and (t > lanes or lanes % t):
+        t //= 2
+    return t
+
+
+def profiled_traffic(profile: str, kernel: str):
+    """HBM bytes per dispatch of `kernel` from a committed counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by what it reads for a
+    device copy of known size). None when the summary has no row for it."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    key = f"`void spmm::{kernel}`"
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == key:
+            try:
+                return int(cells[4])
+            except ValueError:
+                return None
+    return None
+
+
 # ----------------------------------------------------------------- timing
 def time_launches(fn, steps, warmup):
     """Runs fn() warmup times, then `steps` times between two events on the
@@ -125,10 +153,20 @@ def headline(args, h, dev):
     Bd = torch.from_numpy(B).to(dev)
     C = torch.empty((g.M, N), device=dev)
 
+    # row schedule built once per structure, outside the timed region (the
+    # reference converts and uploads its B2SR arrays before timing too)
+    bplan = api.B2srPlan(h, T, g.M, brp)
+    torch.cuda.synchronize()
+
     def step():
-        api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=C)
+        api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=C, plan=bplan)
 
     ms_step, ms_launch = time_launches(step, args.steps, args.warmup)
+    # the plan-free drop-in form (bulk kernel + hub-row list kernel)
+    Cd = torch.empty((g.M, N), device=dev)
+    dropin_ms = time_loop(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=Cd),
+                          args.steps, args.warmup)
+    dropin_diff = float((C - Cd).abs().max())
     nbytes = b2sr_bytes(g.M, T, nb, g.nnz, N)
     # the same product through the CSR kernel (the reference prints its
     # cuSPARSE CSR time beside the B2SR one, test_spmm_full.cu:505)
@@ -140,14 +178,15 @@ def headline(args, h, dev):
     flops = 2.0 * g.nnz * N
     # exactness spot check: +-1 B against an integer recompute of 2000 rows
     S = np.random.default_rng(3).choice([-1, 1], size=(g.K, N)).astype(np.float32)
-    Cs = api.spmm_b2sr(h, T, g.M, brp, bci, words, torch.from_numpy(S).to(dev)).cpu().numpy()
+    Cs = api.spmm_b2sr(h, T, g.M, brp, bci, words, torch.from_numpy(S).to(dev), plan=bplan).cpu().numpy()
     rows = np.random.default_rng(4).choice(g.M, 2000, replace=False)
     exact = all(np.array_equal(Cs[i], S[g.colind[g.rowptr[i]:g.rowptr[i + 1]]].sum(axis=0)) for i in rows)
     err = sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows)
     same = float((C - Ccsr).abs().max())
     return dict(g=g, B=B, T=T, N=N, nb=nb, nbytes=nbytes, flops=flops, ms_step=ms_step,
                 ms_launch=ms_launch, conv_ms=conv_ms, exact=exact, err=err, csr_ms=csr_ms,
-                csr_bytes=csr_bytes(g.M, g.nnz, N), csr_maxdiff=same)
+                csr_bytes=csr_bytes(g.M, g.nnz, N), csr_maxdiff=same, dropin_ms=dropin_ms,
+                dropin_diff=dropin_diff)
 
 
 def cpu_baseline(hd):
@@ -353,6 +392,8 @@ def main():
     hd = headline(args, h, dev)
     gbs = hd["nbytes"] / (hd["ms_step"] * 1e-3) / 1e9
     ach = hd["nbytes"] / (hd["ms_launch"] * 1e-3) / 1e9
+    kname = f"b2sr_planned_kernel<{hd['T']}, float, {b2sr_lanes(hd['N'])}>"
+    traffic = profiled_traffic(f"b2sr_fbf_t{hd['T']}_n{hd['N']}_89k.md", kname)
     result = {
         "metric": "SpMM HBM GB/s (algorithmic) -- B2SR-4 FBF SpMM, FP32 B",
         "value": round(gbs, 2),
@@ -371,11 +412,16 @@ def main():
         "gflops": round(hd["flops"] / (hd["ms_step"] * 1e-3) / 1e9, 1),
         "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "b2sr_spmm_kernel<4,float,16,1>",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/ counter summary (FETCH_SIZE x2 + WRITE_SIZE x1, "
+                                       "factors from a 256 MiB copy)" if traffic else None,
+                     "kernel": kname,
                      "bytes_per_launch": hd["nbytes"], "mean_launch_ms": round(hd["ms_launch"], 5)},
         "checks": {"pm1_exact_2000_rows": bool(hd["exact"]), "rel_err_fp64_sampled": hd["err"]},
         "csr2b2sr_ms": round(hd["conv_ms"], 2),
+        "dropin_unplanned": {"ms": round(hd["dropin_ms"], 5),
+                             "GBps": round(hd["nbytes"] / hd["dropin_ms"] / 1e6, 1),
+                             "max_abs_diff_vs_planned": hd["dropin_diff"]},
         "csr_same_graph": {"ms": round(hd["csr_ms"], 5),
                            "GBps": round(hd["csr_bytes"] / hd["csr_ms"] / 1e6, 1),
                            "max_abs_diff_vs_b2sr": hd["csr_maxdiff"]},

@@ -127,7 +127,9 @@ int csr2b2sr(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
 
 /* FBF SpMM: C[M][N] = A_bin * B, replacing spmm4_full_full_new2 and its
  * siblings (backend/spmm.cu:877-956, launched at test_spmm_full.cu:351-361).
- * Any N >= 1 (the reference requires 32 | N and N | 1024). */
+ * Any N >= 1 (the reference requires 32 | N and N | 1024). The handle keeps
+ * a device list of hub block rows that a second kernel drains; calls on one
+ * handle must therefore be ordered on one stream. */
 int spmm_b2sr_f32(spmm_handle_t h, int T, int M, int K, int N,
                   const int *bsrRowPtr, const int *bsrColInd,
                   const void *bsrVal, const float *B, int ldb, float *C,
@@ -136,6 +138,22 @@ int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N,
                    const int *bsrRowPtr, const int *bsrColInd,
                    const void *bsrVal, const uint16_t *B, int ldb, float *C,
                    int ldc, void *stream);
+
+/* Planned FBF SpMM. spmm_b2sr_analyze bins the M rows by the tile count of
+ * their block row (SYNCHRONISES once); the planned call then runs short
+ * rows one subgroup each, medium rows one wave each and hub rows one
+ * workgroup each in a single launch (no list state, asynchronous). The
+ * plan is a spmm_csr_plan_t; it keeps no pointer to bsrRowPtr. */
+int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr,
+                      spmm_csr_plan_t *plan, void *stream);
+int spmm_b2sr_f32_planned(spmm_csr_plan_t plan, int T, int M, int K, int N,
+                          const int *bsrRowPtr, const int *bsrColInd,
+                          const void *bsrVal, const float *B, int ldb,
+                          float *C, int ldc, void *stream);
+int spmm_b2sr_bf16_planned(spmm_csr_plan_t plan, int T, int M, int K, int N,
+                           const int *bsrRowPtr, const int *bsrColInd,
+                           const void *bsrVal, const uint16_t *B, int ldb,
+                           float *C, int ldc, void *stream);
 
 /* Binary A x sign-packed B -> int32: C[i][k] = sum over stored (i,j) of
  * (+1 if B[j][k] >= 0 else -1), with B packed by pack_sign_rows_u32 into

@@ -138,13 +138,30 @@ def csr2b2sr(h: Handle, rowptr, colind, M: int, K: int, T: int, stream=None):
     return brp, bci[: nb.value], words[: nb.value * T]
 
 
-def spmm_b2sr(h: Handle, T: int, M: int, brp, bci, words, B, C_out=None, stream=None):
+class B2srPlan(CsrPlan):
+    """Rows binned by their block row's tile count (spmm_b2sr_analyze)."""
+
+    def __init__(self, h: Handle, T: int, M: int, brp: torch.Tensor, stream=None):
+        self.M = M
+        self.nnz = 0
+        self.T = T
+        p = C.c_void_p()
+        call("spmm_b2sr_analyze", h.h, T, M, _p(brp), C.byref(p), _stream(stream))
+        self.p = p
+
+
+def spmm_b2sr(h: Handle, T: int, M: int, brp, bci, words, B, C_out=None,
+              plan: B2srPlan | None = None, stream=None):
     K, N = B.shape
     if C_out is None:
         C_out = torch.empty((M, N), dtype=torch.float32, device=B.device)
-    fn = "spmm_b2sr_bf16" if _is_bf16(B) else "spmm_b2sr_f32"
-    call(fn, h.h, T, M, K, N, _p(brp), _p(bci), _p(words), _p(B), B.stride(0), _p(C_out),
-         C_out.stride(0), _stream(stream))
+    bf16 = _is_bf16(B)
+    args = (T, M, K, N, _p(brp), _p(bci), _p(words), _p(B), B.stride(0), _p(C_out),
+            C_out.stride(0), _stream(stream))
+    if plan is None:
+        call("spmm_b2sr_bf16" if bf16 else "spmm_b2sr_f32", h.h, *args)
+    else:
+        call("spmm_b2sr_bf16_planned" if bf16 else "spmm_b2sr_f32_planned", plan.p, *args)
     return C_out
 
 

@@ -22,7 +22,7 @@
 // then streams that list with several 16-byte B gathers in flight per lane,
 // as the CSR kernel does. Power-law graphs leave ~1 set bit per 4x4 tile,
 // so decoding a whole block row per round keeps the gathers deep.
-#include "common.hpp"
+#include "csr_core.hpp"
 
 #include <climits>
 
@@ -298,6 +298,140 @@ b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
   }
 }
 
+// Planned FBF SpMM: rows binned by their block row's tile count (the plan
+// of spmm_b2sr_analyze) run in one launch with the CSR classes: short rows
+// one subgroup per row (several rows per subgroup, the next row's tile range
+// requested before this row's gathers), medium rows one wave per row (its
+// subgroups take tile chunks in turn, butterfly reduction), hub rows one
+// workgroup per row (LDS reduction in a fixed order).
+template <int T, typename TB, int G>
+__device__ __forceinline__ void b2sr_range(int t, int t1, int stride, int r, int lane_sub,
+                                           const int *__restrict__ bci,
+                                           const typename Word<T>::type *__restrict__ words,
+                                           int *list, const TB *__restrict__ Bc, int ldb,
+                                           float (&acc)[Vec16<TB>::kElems]) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int U = 8;
+  for (; t < t1; t += stride) {
+    const int total = decode_chunk<T, G>(t, t1, r, lane_sub, bci, words, list);
+    for (int k = 0; k < total; k += U) {
+      typename Vec16<TB>::Raw raw[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        raw[u] = Vec16<TB>::raw(Bc + static_cast<size_t>(list[k + u < total ? k + u : 0]) * ldb);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (k + u < total) {
+          float b[EPL];
+          Vec16<TB>::widen(raw[u], b);
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) acc[q] += b[q];
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // list is rewritten by the next chunk
+  }
+}
+
+template <int T, typename TB, int G>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
+                    const int *__restrict__ brp, const int *__restrict__ bci,
+                    const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
+                    int ldb, float *__restrict__ C, int ldc) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * EPL;
+  constexpr int TC = Chunk<T, G>::kTiles;
+  __shared__ int lds[kB2Waves * S][Chunk<T, G>::kCap];
+  __shared__ float red[kB2Waves - 1][CW];
+  const Seg sg = find_seg(tab, blockIdx.x);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1);
+  const int sub = lane / G;
+  const int gsub = wave * S + sub;
+  const int col0 = blockIdx.y * CW + lane_sub * EPL;
+  const TB *Bc = B + col0;
+  int *list = lds[gsub];
+  const int local_wg = blockIdx.x - sg.wg_begin;
+  float acc[EPL];
+
+  if (sg.kind == kSegA) {
+    const int R = sg.rows_per_sub;
+    const int subs_per_wg = kB2Waves * S;
+    const int end = sg.list_off + sg.count;
+    int pos = sg.list_off + local_wg * subs_per_wg * R + gsub;
+    int row = pos < end ? rows[pos] : -1;
+    int t0 = 0, t1 = 0;
+    if (row >= 0) {
+      t0 = brp[row / T];
+      t1 = brp[row / T + 1];
+    }
+    for (int k = 0; k < R && row >= 0; ++k) {
+      const int pos_n = pos + subs_per_wg;
+      const int row_n = (k + 1 < R && pos_n < end) ? rows[pos_n] : -1;
+      int t0n = 0, t1n = 0;
+      if (row_n >= 0) {
+        t0n = brp[row_n / T];
+        t1n = brp[row_n / T + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+      b2sr_range<T, TB, G>(t0, t1, TC, row % T, lane_sub, bci, words, list, Bc, ldb, acc);
+      float *Crow = C + static_cast<size_t>(row) * ldc + col0;
+#pragma unroll
+      for (int q = 0; q < EPL; q += 4)
+        *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
+      row = row_n;
+      t0 = t0n;
+      t1 = t1n;
+      pos = pos_n;
+    }
+    return;
+  }
+  const bool wave_mode = sg.kind == kSegB;
+  const int pos = wave_mode ? sg.list_off + local_wg * kB2Waves + wave : sg.list_off + local_wg;
+  if (wave_mode && pos >= sg.list_off + sg.count) return;  // whole wave (no barrier in this mode)
+  const int row = rows[pos];
+  const int t0 = brp[row / T], t1 = brp[row / T + 1];
+  const int part = wave_mode ? sub : gsub;
+  const int parts = wave_mode ? S : kB2Waves * S;
+#pragma unroll
+  for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+  b2sr_range<T, TB, G>(t0 + part * TC, t1, parts * TC, row % T, lane_sub, bci, words, list, Bc,
+                       ldb, acc);
+#pragma unroll
+  for (int m = G; m < kWave; m <<= 1) {
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
+  }
+  float *Crow = C + static_cast<size_t>(row) * ldc + col0;
+  if (wave_mode) {
+    if (sub == 0) {
+#pragma unroll
+      for (int q = 0; q < EPL; q += 4)
+        *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
+    }
+    return;
+  }
+  if (wave > 0 && sub == 0) {
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) red[wave - 1][lane_sub * EPL + q] = acc[q];
+  }
+  __syncthreads();
+  if (wave == 0 && sub == 0) {
+#pragma unroll
+    for (int w = 0; w < kB2Waves - 1; ++w) {
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
+    }
+#pragma unroll
+    for (int q = 0; q < EPL; q += 4)
+      *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
+  }
+}
+
 // Any N / alignment: one wave per output row, lanes stride over columns.
 template <int T, typename TB>
 __global__ void __launch_bounds__(kB2Threads)
@@ -539,6 +673,73 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
   return SPMM_OK;
 }
 
+// Launch table of a B2SR plan for lane geometry G: rows of block rows with
+// fewer than 2 chunks of tiles -> one subgroup per row, at least 2*S chunks
+// -> one workgroup per row, else one wave per row.
+template <int T, typename TB, int G>
+static void launch_planned(const spmm_csr_plan_s *p, int M, const int *brp, const int *bci,
+                           const void *val, const TB *B, int ldb, float *C, int ldc, int tiles,
+                           hipStream_t s) {
+  constexpr int S = kWave / G;
+  constexpr int TC = Chunk<T, G>::kTiles;
+  LaunchTable t;
+  build_table_ex(p->counts, p->offs, S, 2ll * TC, 2ll * S * TC, TC, 16, 1, &t);
+  if (t.total_wg == 0) return;
+  hipLaunchKernelGGL((b2sr_planned_kernel<T, TB, G>), dim3(t.total_wg, tiles), dim3(kB2Threads), 0,
+                     s, t, p->d_rows, M, brp, bci,
+                     static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc);
+}
+
+template <int T, typename TB>
+static void dispatch_planned(const Geo &g, const spmm_csr_plan_s *p, int M, const int *brp,
+                             const int *bci, const void *val, const TB *B, int ldb, float *C,
+                             int ldc, hipStream_t s) {
+  switch (g.G) {
+    case 1: launch_planned<T, TB, 1>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 2: launch_planned<T, TB, 2>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 4: launch_planned<T, TB, 4>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 8: launch_planned<T, TB, 8>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 16: launch_planned<T, TB, 16>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 32: launch_planned<T, TB, 32>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    default: launch_planned<T, TB, 64>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+  }
+}
+
+template <typename TB>
+static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const int *brp,
+                           const int *bci, const void *val, const TB *B, int ldb, float *C,
+                           int ldc, void *stream) {
+  SPMM_CHECK_ARG(p != nullptr, "plan is null");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
+  if (p->tile != T || p->M != M)
+    SPMM_FAIL(SPMM_ERR_MISMATCH, "plan was not built by spmm_b2sr_analyze for this T and M");
+  if (M == 0 || N == 0) return SPMM_OK;
+  SPMM_CHECK_ARG(brp && C, "null bsrRowPtr or C");
+  SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldb >= N), "ldb/ldc smaller than N");
+  SPMM_CHECK_ARG(K == 0 || B, "null B");
+  hipStream_t s = as_stream(stream);
+  const Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
+  if (g.G == 0) {  // scalar path needs no schedule
+    switch (T) {
+      case 4: dispatch_fbf<4, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s); break;
+      case 8: dispatch_fbf<8, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s); break;
+      case 16: dispatch_fbf<16, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s); break;
+      default: dispatch_fbf<32, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s); break;
+    }
+    SPMM_LAUNCHED();
+    return SPMM_OK;
+  }
+  switch (T) {
+    case 4: dispatch_planned<4, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 8: dispatch_planned<8, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 16: dispatch_planned<16, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    default: dispatch_planned<32, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+  }
+  SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
 template <int T, int G>
 static void launch_sign(int M, int N, const int *brp, const int *bci, const void *val,
                         const uint32_t *Bw, int ldbw, int32_t *C, int ldc, hipStream_t s) {
@@ -623,6 +824,32 @@ int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRo
                    const int *bsrColInd, const void *bsrVal, const uint16_t *B, int ldb,
                    float *C, int ldc, void *stream) {
   return run_fbf<uint16_t>(h, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc, stream);
+}
+
+int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr, spmm_csr_plan_t *plan,
+                      void *stream) {
+  SPMM_CHECK_ARG(h && plan && bsrRowPtr, "null handle, plan or bsrRowPtr");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(M > 0, "M must be positive");
+  int shift = 0;
+  while ((1 << shift) < T) ++shift;
+  const int st = analyze(h, M, 0, bsrRowPtr, shift, plan, stream);
+  if (st == SPMM_OK) (*plan)->tile = T;
+  return st;
+}
+
+int spmm_b2sr_f32_planned(spmm_csr_plan_t plan, int T, int M, int K, int N, const int *bsrRowPtr,
+                          const int *bsrColInd, const void *bsrVal, const float *B, int ldb,
+                          float *C, int ldc, void *stream) {
+  return run_planned_fbf<float>(plan, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc,
+                                stream);
+}
+
+int spmm_b2sr_bf16_planned(spmm_csr_plan_t plan, int T, int M, int K, int N, const int *bsrRowPtr,
+                           const int *bsrColInd, const void *bsrVal, const uint16_t *B, int ldb,
+                           float *C, int ldc, void *stream) {
+  return run_planned_fbf<uint16_t>(plan, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc,
+                                   stream);
 }
 
 int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,

@@ -342,9 +342,22 @@ struct TuneEntry {
   spmm_tune_t tune;
 };
 
-// Launch table for a plan's bin counts and lane geometry G.
+// Launch table from bin counts: bins whose degrees all lie below a_limit run
+// one subgroup per row (max(1, r_num / bound) rows per subgroup, r_empty for
+// empty rows, times rows_scale), bins at or above c_limit one workgroup per
+// row, the rest one wave per row.
+void build_table_ex(const int *counts, const int *offs, int S, long long a_limit,
+                    long long c_limit, int r_num, int r_empty, int rows_scale, LaunchTable *t);
+// Launch table for a CSR plan's bin counts and lane geometry G.
 void build_table(const int *counts, const int *offs, int G, int rows_scale, LaunchTable *t);
 
+}  // namespace spmm
+
+struct spmm_csr_plan_s;
+namespace spmm {
+// Bins rows by degree rowptr[(r >> shift) + 1] - rowptr[r >> shift].
+int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
+            spmm_csr_plan_s **out, void *stream);
 }  // namespace spmm
 
 // Row schedule of one CSR structure: rows permuted into degree bins
@@ -358,4 +371,5 @@ struct spmm_csr_plan_s {
   int offs[32] = {0};
   int ntuned = 0;
   spmm::TuneEntry tuned[spmm::kTuneSlots];
+  int tile = 0;  // 0: CSR plan; T: B2SR plan (rows binned by block-row tile count)
 };

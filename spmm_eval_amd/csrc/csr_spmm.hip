@@ -27,13 +27,14 @@ __device__ __forceinline__ int degree_bin(int deg) {
 }
 
 __global__ void __launch_bounds__(256)
-bin_count_kernel(const int *__restrict__ rowptr, int M, int *__restrict__ counts) {
+bin_count_kernel(const int *__restrict__ rowptr, int M, int shift, int *__restrict__ counts) {
   __shared__ int h[kBins];
   if (threadIdx.x < kBins) h[threadIdx.x] = 0;
   __syncthreads();
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < M;
        r += gridDim.x * blockDim.x) {
-    int b = degree_bin(rowptr[r + 1] - rowptr[r]);
+    const int q = r >> shift;
+    int b = degree_bin(rowptr[q + 1] - rowptr[q]);
     atomicAdd(&h[b], 1);
   }
   __syncthreads();
@@ -44,7 +45,7 @@ bin_count_kernel(const int *__restrict__ rowptr, int M, int *__restrict__ counts
 // cursor[b] starts at the bin's offset; each workgroup reserves its rows of
 // each bin with one atomic and ranks them through LDS.
 __global__ void __launch_bounds__(256)
-bin_scatter_kernel(const int *__restrict__ rowptr, int M, int *__restrict__ cursor,
+bin_scatter_kernel(const int *__restrict__ rowptr, int M, int shift, int *__restrict__ cursor,
                    int *__restrict__ rows) {
   __shared__ int h[kBins];
   __shared__ int base[kBins];
@@ -54,7 +55,8 @@ bin_scatter_kernel(const int *__restrict__ rowptr, int M, int *__restrict__ curs
     int r = r0 + threadIdx.x;
     int b = -1, rank = 0;
     if (r < M) {
-      b = degree_bin(rowptr[r + 1] - rowptr[r]);
+      const int q = r >> shift;
+      b = degree_bin(rowptr[q + 1] - rowptr[q]);
       rank = atomicAdd(&h[b], 1);
     }
     __syncthreads();
@@ -132,10 +134,8 @@ static Geometry pick_geometry(const spmm_csr_plan_s *p, int N, int elem_bytes, c
   return g;
 }
 
-void build_table(const int *counts, const int *offs, int G, int rows_scale, LaunchTable *t) {
-  const int S = kWave / G;
-  const int a_limit = std::max(2 * G, 8);   // deg < a_limit -> one subgroup per row
-  const int c_limit = 64 * S;               // deg >= c_limit -> one workgroup per row
+void build_table_ex(const int *counts, const int *offs, int S, long long a_limit,
+                    long long c_limit, int r_num, int r_empty, int rows_scale, LaunchTable *t) {
   t->nseg = 0;
   int wg = 0;
   for (int b = kBins - 1; b >= 0; --b) {
@@ -149,9 +149,7 @@ void build_table(const int *counts, const int *offs, int G, int rows_scale, Laun
     s.wg_begin = wg;
     if (hi <= a_limit) {
       s.kind = kSegA;
-      // Each row costs a dependent index -> gather round trip, so short rows
-      // get few rows per subgroup and many workgroups.
-      int R = b == 0 ? 16 : std::max(1, static_cast<int>(8 / hi));
+      int R = b == 0 ? r_empty : std::max(1, static_cast<int>(r_num / hi));
       R = std::max(1, R * rows_scale);
       s.rows_per_sub = R;
       wg += ceil_div(cnt, kWavesPerWG * S * R);
@@ -167,6 +165,14 @@ void build_table(const int *counts, const int *offs, int G, int rows_scale, Laun
     t->seg[t->nseg++] = s;
   }
   t->total_wg = wg;
+}
+
+// CSR classes: degree < max(2G, 8) -> one subgroup per row (each row costs
+// a dependent index -> gather round trip, so few rows per subgroup and many
+// workgroups); degree >= 64*S -> one workgroup per row; else one wave.
+void build_table(const int *counts, const int *offs, int G, int rows_scale, LaunchTable *t) {
+  const int S = kWave / G;
+  build_table_ex(counts, offs, S, std::max(2 * G, 8), 64ll * S, 8, 16, rows_scale, t);
 }
 
 template <typename TB, int G, int VEC>
@@ -201,6 +207,7 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
                        const spmm_tune_t *tune, void *stream) {
   SPMM_CHECK_ARG(p != nullptr, "plan is null");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0 && nnz >= 0, "negative size");
+  if (p->tile != 0) SPMM_FAIL(SPMM_ERR_MISMATCH, "a B2SR plan was given to a CSR call");
   if (p->M != M || p->nnz != nnz)
     SPMM_FAIL(SPMM_ERR_MISMATCH, "plan was built for M=" + std::to_string(p->M) +
                                      " nnz=" + std::to_string(p->nnz));
@@ -278,7 +285,10 @@ static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int 
   return SPMM_OK;
 }
 
-int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr,
+// Bins the M rows by degree, where row r's degree is
+// rowptr[(r >> shift) + 1] - rowptr[r >> shift] (shift 0: a CSR; shift
+// log2(T): the tile count of a B2SR block row).
+int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
             spmm_csr_plan_s **out, void *stream) {
   SPMM_CHECK_ARG(h && out, "null handle or out");
   *out = nullptr;
@@ -305,7 +315,7 @@ int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr,
   };
   const int grid = std::min(ceil_div(M, kThreads), 2048);
   if (hipMemsetAsync(ws, 0, sizeof(int) * kBins, s) != hipSuccess) return fail("memset");
-  hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(kThreads), 0, s, rowptr, M, ws);
+  hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(kThreads), 0, s, rowptr, M, shift, ws);
   if (hipGetLastError() != hipSuccess) return fail("bin_count_kernel launch");
   if (hipMemcpyAsync(host, ws, sizeof(int) * kBins, hipMemcpyDeviceToHost, s) != hipSuccess)
     return fail("count read-back");
@@ -323,7 +333,7 @@ int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr,
   for (int b = 0; b < kBins; ++b) host[b] = p->offs[b];
   if (hipMemcpyAsync(ws, host, sizeof(int) * kBins, hipMemcpyHostToDevice, s) != hipSuccess)
     return fail("cursor upload");
-  hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(kThreads), 0, s, rowptr, M, ws,
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(kThreads), 0, s, rowptr, M, shift, ws,
                      p->d_rows);
   if (hipGetLastError() != hipSuccess) return fail("bin_scatter_kernel launch");
   // The pinned buffer and workspace are reused by the next call.
@@ -337,7 +347,7 @@ static int run_dropin(spmm_handle_s *h, int M, int K, int N, int nnz,
                       const int *rowptr, const int *colind, const float *val,
                       const TB *B, int ldb, float *C, int ldc, void *stream) {
   spmm_csr_plan_s *p = nullptr;
-  int st = analyze(h, M, nnz, rowptr, &p, stream);
+  int st = analyze(h, M, nnz, rowptr, 0, &p, stream);
   if (st != SPMM_OK) return st;
   st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, nullptr, stream);
   // The plan's row list is read by the kernel just queued.
@@ -352,7 +362,7 @@ extern "C" {
 
 int spmm_csr_analyze(spmm_handle_t h, int M, int nnz, const int *rowptr,
                      spmm_csr_plan_t *plan, void *stream) {
-  return spmm::analyze(h, M, nnz, rowptr, plan, stream);
+  return spmm::analyze(h, M, nnz, rowptr, 0, plan, stream);
 }
 
 int spmm_csr_plan_destroy(spmm_csr_plan_t plan) {

@@ -101,11 +101,17 @@ def test_fbf_workload(handle, dev, b2sr_graph, T, N):
     S = np.random.default_rng(N + T).choice([-1, 1], size=(g.K, N)).astype(np.int32)
     C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(S.astype(np.float32)).to(dev))
     np.testing.assert_array_equal(C.cpu().numpy(), oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.float32))
-    # float B vs FP64
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    Cp = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(S.astype(np.float32)).to(dev),
+                       plan=plan)
+    assert torch.equal(Cp, C)  # planned schedule: same known answer
+    # float B vs FP64, plain and planned (same per-row summation order)
     B = graphs.uniform_pm1((g.K, N), seed=N * T)
     Cf = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(B).to(dev)).cpu().numpy()
     ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
     assert np.linalg.norm(Cf - ref) / np.linalg.norm(ref) <= 1e-5
+    Cfp = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(B).to(dev), plan=plan).cpu().numpy()
+    assert np.linalg.norm(Cfp - ref) / np.linalg.norm(ref) <= 1e-5
     # BF16 B
     Bh = torch.from_numpy(B).to(dev).to(torch.bfloat16).view(torch.uint16)
     Ch = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bh).cpu().numpy()
@@ -123,6 +129,9 @@ def test_fbf_odd_widths(handle, dev, T, N):
     C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(B).to(dev)).cpu().numpy()
     ref = oracle.b2sr_spmm_f64(T, g.M, *oracle.csr2b2sr(g.rowptr, g.colind, T), B)
     assert np.linalg.norm(C - ref) / max(np.linalg.norm(ref), 1e-30) <= 1e-5
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    Cp = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(B).to(dev), plan=plan).cpu().numpy()
+    assert np.linalg.norm(Cp - ref) / max(np.linalg.norm(ref), 1e-30) <= 1e-5
 
 
 @pytest.mark.parametrize("T", [4, 8, 16, 32])
@@ -180,3 +189,27 @@ def test_fbf_long_block_rows_repeated(handle, dev, T, N):
             if mat is g:
                 outs.append(C)
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("T", [4, 16])
+def test_fbf_planned_hub_rows_and_mismatch(handle, dev, T):
+    """Rows of 3000-4000 entries reach the one-workgroup-per-row class; a
+    plan for another T or M, or a CSR plan, is refused."""
+    g = graphs.random_csr(300, 8192, 0.01, seed=T, long_rows=4, long_len=4000)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    B = graphs.uniform_pm1((g.K, 64), seed=3)
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(B).to(dev), plan=plan).cpu().numpy()
+    ref = oracle.csr_spmm_f64(g.rowptr, g.colind, np.ones(g.nnz, np.float32), B)
+    assert np.linalg.norm(C - ref) / np.linalg.norm(ref) <= 1e-5
+    Bd = torch.from_numpy(B).to(dev)
+    other_T = 8 if T == 4 else 4
+    with pytest.raises(SpmmError) as ei:
+        api.spmm_b2sr(handle, other_T, g.M, brp, bci, words, Bd, plan=plan)
+    assert ei.value.status == 5
+    csr_plan = api.CsrPlan(handle, rp, g.nnz)
+    with pytest.raises(SpmmError):
+        api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=csr_plan)
+    with pytest.raises(SpmmError):
+        api.spmm_csr(handle, rp, ci, torch.ones(g.nnz, device=dev), Bd, plan=plan)
