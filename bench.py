@@ -392,7 +392,8 @@ def main():
     hd = headline(args, h, dev)
     gbs = hd["nbytes"] / (hd["ms_step"] * 1e-3) / 1e9
     ach = hd["nbytes"] / (hd["ms_launch"] * 1e-3) / 1e9
-    kname = f"b2sr_planned_kernel<{hd['T']}, float, {b2sr_lanes(hd['N'])}>"
+    kfam = "b2sr_brow_kernel" if hd["T"] <= 8 else "b2sr_planned_kernel"
+    kname = f"{kfam}<{hd['T']}, float, {b2sr_lanes(hd['N'])}>"
     traffic = profiled_traffic(f"b2sr_fbf_t{hd['T']}_n{hd['N']}_89k.md", kname)
     result = {
         "metric": "SpMM HBM GB/s (algorithmic) -- B2SR-4 FBF SpMM, FP32 B",

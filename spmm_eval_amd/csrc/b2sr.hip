@@ -298,8 +298,8 @@ b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
   }
 }
 
-// Planned FBF SpMM: rows binned by their block row's tile count (the plan
-// of spmm_b2sr_analyze) run in one launch with the CSR classes: short rows
+// Planned FBF SpMM for T = 16 and 32: rows binned by their block row's tile
+// count (the plan of spmm_b2sr_analyze) run in one launch with the CSR classes: short rows
 // one subgroup per row (several rows per subgroup, the next row's tile range
 // requested before this row's gathers), medium rows one wave per row (its
 // subgroups take tile chunks in turn, butterfly reduction), hub rows one
@@ -430,6 +430,347 @@ b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
     for (int q = 0; q < EPL; q += 4)
       *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
   }
+}
+
+// ------------------------------------------- block-row FBF (T = 4 and 8)
+// For the small tiles the planned launch schedules BLOCK rows (binned by
+// tile count), so each tile is decoded once for all T rows. A tile is one
+// 32-bit (T=4) or 64-bit (T=8) word, byte r holding row r. A group of L
+// lanes (a subgroup, or a whole wave) loads 4 consecutive tiles per lane;
+// the T per-row bit counts go through one group scan as packed 16-bit
+// pairs, and the LDS list is written row-segmented (row 0's B rows, then
+// row 1's, ...), ascending in column within each row, so every row is
+// summed in ascending column order exactly as the CSR kernel sums it. If
+// the bits exceed the list (dense tiles) it is filled and drained in
+// windows; each window reloads its tiles, so nothing of the decode stays
+// live while the gathers run.
+//
+// Classes by tile count: up to one subgroup chunk (4G tiles, most block
+// rows of a power-law graph) -> one subgroup per block row, streamed; up to
+// one wave chunk (256 tiles) -> one wave per block row, rows in turn, each
+// row's entries dealt round-robin to the subgroups; wider -> one workgroup
+// per block row, one wave per row (its subgroups take the tile chunks in
+// turn, decoding only their row's bits).
+template <int T> struct TileWord;
+template <> struct TileWord<4> { using type = uint32_t; };
+template <> struct TileWord<8> { using type = uint64_t; };
+
+template <int G> struct BrowChunk {
+  static constexpr int kCap = 16 * G;  // list entries per subgroup (16 KB per WG)
+  static constexpr int kPerLane = 4;   // tiles per lane per chunk
+  static constexpr int kTiles = kPerLane * G;
+  static constexpr int kWaveTiles = kPerLane * kWave;  // 256
+  static constexpr int kWaveCap = 16 * kWave;          // the wave's subgroup lists, contiguous
+};
+
+// Decode tiles [t, min(t + 4L, t1)) by a group of L lanes (lane_l its lane)
+// into list window [base, base + CAPL): entry = B row, or with kFlags
+// (B row << 4) | (last of its row << 3) | row. Returns the chunk's bit
+// count; seg[r] = first position of row r (seg[T] = total).
+template <int T, int L, int CAPL, bool kFlags>
+__device__ __forceinline__ int decode_rows(int t, int t1, int lane_l, int base,
+                                           const int *__restrict__ bci,
+                                           const typename TileWord<T>::type *__restrict__ tw,
+                                           int *list, int (&seg)[T + 1]) {
+  using TWd = typename TileWord<T>::type;
+  constexpr int TPL = 4;
+  constexpr int NP = T / 2;
+  int bc[TPL];
+  TWd w[TPL];
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int tile = t + lane_l * TPL + k;
+    bc[k] = 0;
+    w[k] = 0;
+    if (tile < t1) {
+      bc[k] = bci[tile];
+      w[k] = tw[tile];
+    }
+  }
+  uint32_t pk[NP], inc[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    pk[j] = 0;
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+      const uint32_t lo = static_cast<uint32_t>(w[k] >> (16 * j)) & 0xffu;
+      const uint32_t hi = static_cast<uint32_t>(w[k] >> (16 * j + 8)) & 0xffu;
+      pk[j] += __popc(lo) | (__popc(hi) << 16);
+    }
+    inc[j] = pk[j];
+  }
+#pragma unroll
+  for (int d = 1; d < L; d <<= 1) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const uint32_t v = __shfl_up(inc[j], d, L);
+      if (lane_l >= d) inc[j] += v;
+    }
+  }
+  seg[0] = 0;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint32_t tot = __shfl(inc[j], L - 1, L);
+    seg[2 * j + 1] = seg[2 * j] + static_cast<int>(tot & 0xffffu);
+    seg[2 * j + 2] = seg[2 * j + 1] + static_cast<int>(tot >> 16);
+  }
+  // this lane's next position per row, two rows per word like the counts
+  // (positions stay below 2^16: at most 4L tiles x T*T bits)
+  uint32_t Q[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+    Q[j] = (inc[j] - pk[j]) + static_cast<uint32_t>(seg[2 * j]) +
+           (static_cast<uint32_t>(seg[2 * j + 1]) << 16);
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    TWd x = w[k];
+    const int b0 = bc[k] * T + T - 1;
+    while (x) {
+      // highest set bit first: last row first, lowest column first
+      const int q = (sizeof(TWd) == 8 ? 63 - __clzll(static_cast<unsigned long long>(x))
+                                      : 31 - __clz(static_cast<uint32_t>(x)));
+      x ^= static_cast<TWd>(1) << q;
+      const int r = q >> 3;
+      const int sh = 16 * (r & 1);
+      uint32_t cq = Q[0];
+#pragma unroll
+      for (int j = 1; j < NP; ++j) cq = (r >> 1) == j ? Q[j] : cq;
+      const int pos = static_cast<int>((cq >> sh) & 0xffffu);
+      const int e = pos - base;
+      if (e >= 0 && e < CAPL) {
+        int v = b0 - (q & 7);
+        if (kFlags) {
+          int end = seg[1];
+#pragma unroll
+          for (int i = 1; i < T; ++i) end = r == i ? seg[i + 1] : end;
+          v = (v << 4) | ((pos + 1 == end) << 3) | r;
+        }
+        list[e] = v;
+      }
+#pragma unroll
+      for (int j = 0; j < NP; ++j)
+        if ((r >> 1) == j) Q[j] += 1u << sh;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return seg[T];
+}
+
+template <int EPL>
+__device__ __forceinline__ void store_row(float *__restrict__ C, int ldc, int row, int col0,
+                                          const float (&v)[EPL]) {
+  float *Crow = C + static_cast<size_t>(row) * ldc + col0;
+#pragma unroll
+  for (int q = 0; q < EPL; q += 4)
+    *reinterpret_cast<float4 *>(Crow + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+}
+
+// One subgroup, one block row of at most one subgroup chunk: the list is
+// streamed flat in groups of 8 gathers into ONE running sum, stored to C at
+// each row's last entry (a block row of ~40 bits costs ~5 gather round
+// trips). Rows without entries are zeroed.
+template <int T, typename TB, int G>
+__device__ __forceinline__ void brow_stream(int t, int t1, int lane_sub,
+                                            const int *__restrict__ bci,
+                                            const typename TileWord<T>::type *__restrict__ tw,
+                                            int *list, const TB *__restrict__ Bc, int ldb,
+                                            float *__restrict__ C, int ldc, int row0, int M,
+                                            int col0) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int CAP = BrowChunk<G>::kCap;
+  float cur[EPL];
+#pragma unroll
+  for (int q = 0; q < EPL; ++q) cur[q] = 0.f;
+  int total = 1;
+  for (int base = 0; base < total; base += CAP) {
+    int seg[T + 1];
+    total = decode_rows<T, G, CAP, true>(t, t1, lane_sub, base, bci, tw, list, seg);
+    if (base == 0) {
+#pragma unroll
+      for (int r = 0; r < T; ++r)
+        if (seg[r] == seg[r + 1] && row0 + r < M) store_row<EPL>(C, ldc, row0 + r, col0, cur);
+    }
+    const int n = min(CAP, total - base);
+    for (int k = 0; k < n; k += 8) {
+      typename Vec16<TB>::Raw raw[8];
+      uint32_t meta = 0;  // low 4 bits (end flag, row) of the 8 entries
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t ent = static_cast<uint32_t>(list[k + u < n ? k + u : k]);
+        meta |= (ent & 15u) << (4 * u);
+        raw[u] = Vec16<TB>::raw(Bc + static_cast<size_t>(ent >> 4) * ldb);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (k + u < n) {
+          float b[EPL];
+          Vec16<TB>::widen(raw[u], b);
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) cur[q] += b[q];
+          const uint32_t m = meta >> (4 * u);
+          if (m & 8u) {
+            const int row = row0 + static_cast<int>(m & 7u);
+            if (row < M) store_row<EPL>(C, ldc, row, col0, cur);
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) cur[q] = 0.f;
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // list is rewritten next
+  }
+}
+
+// One wave, one block row of at most one wave chunk (256 tiles): decoded
+// once by all 64 lanes into the wave's lists; then row by row, the row's
+// entries dealt round-robin to the S subgroups (8 gathers in flight each),
+// a butterfly over the subgroups in a fixed order, subgroup 0 stores.
+template <int T, typename TB, int G>
+__device__ __forceinline__ void brow_wave(int t, int t1, int lane, const int *__restrict__ bci,
+                                          const typename TileWord<T>::type *__restrict__ tw,
+                                          int *wlist, const TB *__restrict__ Bc, int ldb,
+                                          float *__restrict__ C, int ldc, int row0, int M,
+                                          int col0) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CAPW = BrowChunk<G>::kWaveCap;
+  const int sub = lane / G;
+  float cur[EPL];
+#pragma unroll
+  for (int q = 0; q < EPL; ++q) cur[q] = 0.f;
+  int total = 1;
+  for (int base = 0; base < total; base += CAPW) {
+    int seg[T + 1];
+    total = decode_rows<T, kWave, CAPW, false>(t, t1, lane, base, bci, tw, wlist, seg);
+#pragma unroll
+    for (int r = 0; r < T; ++r) {
+      if (seg[r + 1] <= base && !(base == 0 && seg[r] == seg[r + 1])) continue;  // done earlier
+      if (seg[r] >= base + CAPW) continue;                                     // a later window
+      const int hi = min(seg[r + 1], base + CAPW) - base;
+      for (int k = max(seg[r], base) - base + sub; k < hi; k += 8 * S) {
+        typename Vec16<TB>::Raw raw[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          raw[u] = Vec16<TB>::raw(
+              Bc + static_cast<size_t>(wlist[k + u * S < hi ? k + u * S : k]) * ldb);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (k + u * S < hi) {
+            float b[EPL];
+            Vec16<TB>::widen(raw[u], b);
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) cur[q] += b[q];
+          }
+        }
+      }
+      if (seg[r + 1] <= base + CAPW) {  // row complete
+#pragma unroll
+        for (int m = G; m < kWave; m <<= 1) {
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) cur[q] += __shfl_xor(cur[q], m);
+        }
+        if (sub == 0 && row0 + r < M) store_row<EPL>(C, ldc, row0 + r, col0, cur);
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) cur[q] = 0.f;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // lists are rewritten next
+  }
+}
+
+// The work of virtual workgroup `vwg` of the block-row launch table.
+template <int T, typename TB, int G>
+__device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *lds,
+                                        const int *__restrict__ brows, int M,
+                                        const int *__restrict__ brp, const int *__restrict__ bci,
+                                        const typename TileWord<T>::type *__restrict__ tw,
+                                        const TB *__restrict__ B, int ldb, float *__restrict__ C,
+                                        int ldc) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * EPL;
+  constexpr int CAP = BrowChunk<G>::kCap;
+  const Seg sg = find_seg(tab, vwg);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1);
+  const int sub = lane / G;
+  const int gsub = wave * S + sub;
+  const int col0 = blockIdx.y * CW + lane_sub * EPL;
+  const TB *Bc = B + col0;
+  const int local_wg = vwg - sg.wg_begin;
+
+  if (sg.kind == kSegA) {
+    int *list = lds + gsub * CAP;
+    const int R = sg.rows_per_sub;
+    const int subs_per_wg = kB2Waves * S;
+    const int end = sg.list_off + sg.count;
+    int pos = sg.list_off + local_wg * subs_per_wg * R + gsub;
+    int br = pos < end ? brows[pos] : -1;
+    int t0 = 0, t1 = 0;
+    if (br >= 0) {
+      t0 = brp[br];
+      t1 = brp[br + 1];
+    }
+    for (int k = 0; k < R && br >= 0; ++k) {
+      const int pos_n = pos + subs_per_wg;
+      const int br_n = (k + 1 < R && pos_n < end) ? brows[pos_n] : -1;
+      int t0n = 0, t1n = 0;
+      if (br_n >= 0) {
+        t0n = brp[br_n];
+        t1n = brp[br_n + 1];
+      }
+      brow_stream<T, TB, G>(t0, t1, lane_sub, bci, tw, list, Bc, ldb, C, ldc, br * T, M, col0);
+      br = br_n;
+      t0 = t0n;
+      t1 = t1n;
+      pos = pos_n;
+    }
+    return;
+  }
+  if (sg.kind == kSegB) {
+    const int pos = sg.list_off + local_wg * kB2Waves + wave;
+    if (pos >= sg.list_off + sg.count) return;  // whole wave
+    const int br = brows[pos];
+    brow_wave<T, TB, G>(brp[br], brp[br + 1], lane, bci, tw, lds + wave * S * CAP, Bc, ldb, C,
+                        ldc, br * T, M, col0);
+    return;
+  }
+  // class C: one workgroup per block row, wave w takes rows w, w + 4, ...
+  int *list = lds + gsub * CAP;
+  const int br = brows[sg.list_off + local_wg];
+  const int t0 = brp[br], t1 = brp[br + 1];
+  const auto *words = reinterpret_cast<const typename Word<T>::type *>(tw);
+  constexpr int RC = Chunk<T, G>::kTiles;
+  for (int r = wave; r < T; r += kB2Waves) {
+    const int row = br * T + r;
+    if (row >= M) break;  // whole wave
+    float acc[EPL];
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+    b2sr_range<T, TB, G>(t0 + sub * RC, t1, S * RC, r, lane_sub, bci, words, list, Bc, ldb, acc);
+#pragma unroll
+    for (int m = G; m < kWave; m <<= 1) {
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
+    }
+    if (sub == 0) store_row<EPL>(C, ldc, row, col0, acc);
+  }
+}
+
+template <int T, typename TB, int G>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_brow_kernel(LaunchTable tab, const int *__restrict__ brows, int M,
+                 const int *__restrict__ brp, const int *__restrict__ bci,
+                 const typename TileWord<T>::type *__restrict__ tw, const TB *__restrict__ B,
+                 int ldb, float *__restrict__ C, int ldc) {
+  constexpr int S = kWave / G;
+  constexpr int CAP = BrowChunk<G>::kCap;
+  static_assert(Chunk<T, G>::kCap == CAP, "all paths share the list area");
+  __shared__ int lds[kB2Waves * S][CAP];
+  brow_wg<T, TB, G>(tab, blockIdx.x, &lds[0][0], brows, M, brp, bci, tw, B, ldb, C, ldc);
 }
 
 // Any N / alignment: one wave per output row, lanes stride over columns.
@@ -705,6 +1046,38 @@ static void dispatch_planned(const Geo &g, const spmm_csr_plan_s *p, int M, cons
   }
 }
 
+template <int T, typename TB, int G>
+static void launch_brow(const spmm_csr_plan_s *p, int M, const int *brp, const int *bci,
+                        const void *val, const TB *B, int ldb, float *C, int ldc, int tiles,
+                        hipStream_t s) {
+  constexpr int S = kWave / G;
+  constexpr int TC = BrowChunk<G>::kTiles;
+  LaunchTable t;
+  // bins wholly below one subgroup chunk of tiles: a subgroup per block row
+  // (several only below 8 tiles); wholly below one wave chunk: a wave (the
+  // wave path decodes exactly one chunk); the rest: a workgroup
+  build_table_ex(p->counts, p->offs, S, TC, BrowChunk<G>::kWaveTiles, 8, 16, 1, &t);
+  if (t.total_wg == 0) return;
+  hipLaunchKernelGGL((b2sr_brow_kernel<T, TB, G>), dim3(t.total_wg, tiles), dim3(kB2Threads), 0, s,
+                     t, p->d_rows, M, brp, bci,
+                     static_cast<const typename TileWord<T>::type *>(val), B, ldb, C, ldc);
+}
+
+template <int T, typename TB>
+static void dispatch_brow(const Geo &g, const spmm_csr_plan_s *p, int M, const int *brp,
+                          const int *bci, const void *val, const TB *B, int ldb, float *C,
+                          int ldc, hipStream_t s) {
+  switch (g.G) {
+    case 1: launch_brow<T, TB, 1>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 2: launch_brow<T, TB, 2>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 4: launch_brow<T, TB, 4>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 8: launch_brow<T, TB, 8>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 16: launch_brow<T, TB, 16>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 32: launch_brow<T, TB, 32>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    default: launch_brow<T, TB, 64>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+  }
+}
+
 template <typename TB>
 static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const int *brp,
                            const int *bci, const void *val, const TB *B, int ldb, float *C,
@@ -718,6 +1091,11 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
   SPMM_CHECK_ARG(brp && C, "null bsrRowPtr or C");
   SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldb >= N), "ldb/ldc smaller than N");
   SPMM_CHECK_ARG(K == 0 || B, "null B");
+  // T = 4 / 8 tiles are read as one 32 / 64-bit word; list entries keep
+  // the B row in 27 bits
+  SPMM_CHECK_ARG(T > 8 || reinterpret_cast<uintptr_t>(val) % T == 0,
+                 "bsrVal must be aligned to T bytes for T = 4 and 8");
+  SPMM_CHECK_ARG(T > 8 || K < (1 << 27), "planned T = 4 / 8 needs K < 2^27");
   hipStream_t s = as_stream(stream);
   const Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
   if (g.G == 0) {  // scalar path needs no schedule
@@ -731,8 +1109,8 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     return SPMM_OK;
   }
   switch (T) {
-    case 4: dispatch_planned<4, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 8: dispatch_planned<8, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 4: dispatch_brow<4, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 8: dispatch_brow<8, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
     case 16: dispatch_planned<16, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
     default: dispatch_planned<32, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
   }
@@ -831,9 +1209,17 @@ int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr, spmm_
   SPMM_CHECK_ARG(h && plan && bsrRowPtr, "null handle, plan or bsrRowPtr");
   SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
   SPMM_CHECK_ARG(M > 0, "M must be positive");
-  int shift = 0;
-  while ((1 << shift) < T) ++shift;
-  const int st = analyze(h, M, 0, bsrRowPtr, shift, plan, stream);
+  int st;
+  if (T <= 8) {
+    // block rows binned by tile count (b2sr_brow_kernel's units)
+    st = analyze(h, (M + T - 1) / T, 0, bsrRowPtr, 0, plan, stream);
+    if (st == SPMM_OK) (*plan)->M = M;
+  } else {
+    // rows binned by their block row's tile count (b2sr_planned_kernel's units)
+    int shift = 0;
+    while ((1 << shift) < T) ++shift;
+    st = analyze(h, M, 0, bsrRowPtr, shift, plan, stream);
+  }
   if (st == SPMM_OK) (*plan)->tile = T;
   return st;
 }

@@ -115,15 +115,19 @@ __device__ __forceinline__ void wave_reduce_subgroups(float (&acc)[EPL]) {
   }
 }
 
-// Segment of workgroup `wg`. The unrolled select keeps the table in SGPRs
-// (a dynamic index into a by-value kernel argument would spill to scratch).
+// Segment of workgroup `wg`: binary search on wg_begin (ascending). `wg` is
+// uniform, so each probe is one scalar load from the kernel-argument
+// segment (~80 scalar instructions a wave, against ~300 for an unrolled
+// select over all 32 entries, which made the scalar unit a bottleneck of
+// launches with thousands of short workgroups).
 __device__ __forceinline__ Seg find_seg(const LaunchTable &tab, int wg) {
-  Seg sg = tab.seg[0];
-#pragma unroll
-  for (int i = 1; i < kBins; ++i) {
-    if (i < tab.nseg && wg >= tab.seg[i].wg_begin) sg = tab.seg[i];
+  int lo = 0, hi = tab.nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab.seg[mid].wg_begin <= wg) lo = mid;
+    else hi = mid - 1;
   }
-  return sg;
+  return tab.seg[lo];
 }
 
 // The work of one 256-thread workgroup of the CSR launch table: virtual
@@ -345,9 +349,11 @@ struct TuneEntry {
 // Launch table from bin counts: bins whose degrees all lie below a_limit run
 // one subgroup per row (max(1, r_num / bound) rows per subgroup, r_empty for
 // empty rows, times rows_scale), bins at or above c_limit one workgroup per
-// row, the rest one wave per row.
+// row, the rest one wave per row (b_items waves per row when a unit holds
+// several rows, as a B2SR block row does).
 void build_table_ex(const int *counts, const int *offs, int S, long long a_limit,
-                    long long c_limit, int r_num, int r_empty, int rows_scale, LaunchTable *t);
+                    long long c_limit, int r_num, int r_empty, int rows_scale, LaunchTable *t,
+                    int b_items = 1);
 // Launch table for a CSR plan's bin counts and lane geometry G.
 void build_table(const int *counts, const int *offs, int G, int rows_scale, LaunchTable *t);
 

@@ -117,9 +117,11 @@ def test_fbf_workload(handle, dev, b2sr_graph, T, N):
     Ch = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bh).cpu().numpy()
     refq = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, np_words(Bh))
     assert np.linalg.norm(Ch - refq) / np.linalg.norm(refq) <= 1e-5
+    Chp = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bh, plan=plan).cpu().numpy()
+    assert np.linalg.norm(Chp - refq) / np.linalg.norm(refq) <= 1e-5
 
 
-@pytest.mark.parametrize("T", [4, 16, 32])
+@pytest.mark.parametrize("T", [4, 8, 16, 32])
 @pytest.mark.parametrize("N", [1, 7, 100])
 def test_fbf_odd_widths(handle, dev, T, N):
     g = graphs.random_csr(333, 444, 0.02, seed=N, long_rows=1, long_len=444)
@@ -213,3 +215,22 @@ def test_fbf_planned_hub_rows_and_mismatch(handle, dev, T):
         api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=csr_plan)
     with pytest.raises(SpmmError):
         api.spmm_csr(handle, rp, ci, torch.ones(g.nnz, device=dev), Bd, plan=plan)
+
+
+@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("N,bf16", [(64, False), (32, False), (256, True)])
+def test_fbf_planned_dense_tiles(handle, dev, T, N, bf16):
+    """Nearly full tiles: a chunk's set bits overflow one list window (the
+    list is filled and drained several times per chunk); M not a multiple
+    of T leaves a partial last block row. +-1 B: exact integer sums."""
+    g = graphs.random_csr(130, 3000, 0.85, seed=T + N)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    S = np.random.default_rng(T).choice([-1, 1], size=(g.K, N)).astype(np.float32)
+    Sd = torch.from_numpy(S).to(dev)
+    if bf16:
+        Sd = api.to_bf16_bits(Sd)
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Sd, plan=plan).cpu().numpy()
+    exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S.astype(np.int32)).astype(np.float32)
+    np.testing.assert_array_equal(C, exact)
