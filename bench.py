@@ -281,15 +281,27 @@ def extra_fwd_bwd(args, h, dev):
     dX = torch.empty((g.K, N), device=dev)
     dval = torch.empty(g.nnz, device=dev)
 
-    def step(XX, GG):
+    def fwd(XX):
         api.spmm_csr(h, rp, ci, v, XX, C_out=Y, plan=pf)
+
+    def bwd_x(GG):
         api.spmm_csr(h, trp, tci, tval, GG, K=g.M, C_out=dX, plan=pb)
-        api.sddmm_csr(h, rp, ci, GG, XX, out=dval)
+
+    def bwd_v(XX, GG):
+        api.sddmm_csr(h, rp, ci, GG, XX, out=dval, plan=pf)
+
+    def step(XX, GG):
+        fwd(XX)
+        bwd_x(GG)
+        bwd_v(XX, GG)
 
     res = {}
     for name, XX, GG, s in (("f32", Xd, dYd, 4), ("bf16", Xh, dYh, 2)):
         geo = (pf.autotune(rp, ci, v, XX, Y), pb.autotune(trp, tci, tval, GG, dX, K=g.M))
         ms = time_loop(lambda: step(XX, GG), 100, 10)
+        parts = {"fwd_ms": round(time_loop(lambda: fwd(XX), 100, 10), 4),
+                 "dX_ms": round(time_loop(lambda: bwd_x(GG), 100, 10), 4),
+                 "sddmm_ms": round(time_loop(lambda: bwd_v(XX, GG), 100, 10), 4)}
         nbytes = 2 * csr_bytes(g.M, g.nnz, N, s_b=s) + sddmm_bytes(g.M, g.nnz, N, s)
         rows = np.random.default_rng(7).choice(g.M, 1500, replace=False)
         Yc, dvc = Y.cpu().numpy(), dval.cpu().numpy()
@@ -300,7 +312,7 @@ def extra_fwd_bwd(args, h, dev):
             ref = X[g.colind[s0:e0]].astype(np.float64) @ dY[i].astype(np.float64)
             num += float(((dvc[s0:e0] - ref) ** 2).sum())
             den += float((ref ** 2).sum())
-        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1), "geometry": geo,
+        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1), "geometry": geo, **parts,
                      "rel_err_Y_fp64": e_y, "rel_err_dval_fp64": (num / den) ** 0.5}
     res["bf16_speedup_vs_f32"] = round(res["f32"]["ms"] / res["bf16"]["ms"], 3)
     return {"gcn_fwd_bwd_89k_n128": res}
@@ -392,8 +404,11 @@ def main():
     hd = headline(args, h, dev)
     gbs = hd["nbytes"] / (hd["ms_step"] * 1e-3) / 1e9
     ach = hd["nbytes"] / (hd["ms_launch"] * 1e-3) / 1e9
-    kfam = "b2sr_brow_kernel" if hd["T"] <= 8 else "b2sr_planned_kernel"
-    kname = f"{kfam}<{hd['T']}, float, {b2sr_lanes(hd['N'])}>"
+    if hd["T"] <= 8:  # block-row kernel; B read through a buffer resource below 2 GiB
+        kbuf = "true" if hd["g"].K * hd["N"] * 4 < 2 ** 31 else "false"
+        kname = f"b2sr_brow_kernel<{hd['T']}, float, {b2sr_lanes(hd['N'])}, {kbuf}>"
+    else:
+        kname = f"b2sr_planned_kernel<{hd['T']}, float, {b2sr_lanes(hd['N'])}>"
     traffic = profiled_traffic(f"b2sr_fbf_t{hd['T']}_n{hd['N']}_89k.md", kname)
     result = {
         "metric": "SpMM HBM GB/s (algorithmic) -- B2SR-4 FBF SpMM, FP32 B",

@@ -215,6 +215,17 @@ int sddmm_csr_bf16(spmm_handle_t h, int M, int K, int N, int nnz,
                    const int *rowptr, const int *colind, const uint16_t *dY,
                    int lddy, const uint16_t *X, int ldx, float *dval,
                    void *stream);
+/* The same on the row schedule of a CSR plan of (rowptr, colind) (the
+ * forward's plan in a training step): rows by degree class, each entry's
+ * dot product in one subgroup. Bitwise equal to the plan-free form. */
+int sddmm_csr_f32_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
+                          const int *rowptr, const int *colind,
+                          const float *dY, int lddy, const float *X, int ldx,
+                          float *dval, void *stream);
+int sddmm_csr_bf16_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
+                           const int *rowptr, const int *colind,
+                           const uint16_t *dY, int lddy, const uint16_t *X,
+                           int ldx, float *dval, void *stream);
 
 /* ------------------------------------------------------------- batched
  * Many CSR graphs run as one block-diagonal problem. Graph g has rows
@@ -232,10 +243,12 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off,
                            int bf16, spmm_batch_plan_t *plan, void *stream);
 int spmm_batch_plan_destroy(spmm_batch_plan_t plan);
 /* The geometry the plan's autotuner chose (lanes per row, vector width,
- * rows per subgroup), whether the fused launch is persistent, and the tuned
- * time of one step in ms. */
+ * rows per subgroup), whether the fused launch is persistent, whether its
+ * SDDMM runs on the rows of the forward plan (1) or in fixed entry chunks
+ * (0; both give the same bits), and the tuned time of one step in ms. Any
+ * output pointer may be NULL. */
 int spmm_batch_plan_info(spmm_batch_plan_t plan, spmm_tune_t *tune,
-                         int *persistent, float *tuned_ms);
+                         int *persistent, int *sddmm_rows, float *tuned_ms);
 /* One fwd+bwd step; features are uint16 BF16 when the plan is bf16, else
  * float. Y, dX are FP32 [total_rows][N]; dval FP32 [total_nnz]. */
 int spmm_batch_fwd_bwd(spmm_batch_plan_t plan, const void *X, const void *dY,

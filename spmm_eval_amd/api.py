@@ -227,14 +227,21 @@ def csr_transpose(h: Handle, rowptr, colind, val, K: int, stream=None):
     return trp, tci[:nnz], (tv[:nnz] if tv is not None else None), perm[:nnz]
 
 
-def sddmm_csr(h: Handle, rowptr, colind, dY, X, out=None, stream=None):
+def sddmm_csr(h: Handle, rowptr, colind, dY, X, out=None, plan: CsrPlan | None = None,
+              stream=None):
+    """dval[e] = dY[row(e)] . X[col(e)] at the stored positions (FP32 out).
+    With `plan` (a CsrPlan of rowptr) the rows run on its schedule."""
     M = rowptr.numel() - 1
     K, N = X.shape
     nnz = colind.numel()
     if out is None:
         out = torch.empty(max(nnz, 1), dtype=torch.float32, device=X.device)[:nnz]
     fn = "sddmm_csr_bf16" if _is_bf16(X) else "sddmm_csr_f32"
-    call(fn, h.h, M, K, N, nnz, _p(rowptr), _p(colind), _p(dY), dY.stride(0), _p(X),
+    first = h.h
+    if plan is not None:
+        fn += "_planned"
+        first = plan.p
+    call(fn, first, M, K, N, nnz, _p(rowptr), _p(colind), _p(dY), dY.stride(0), _p(X),
          X.stride(0), _p(out), _stream(stream))
     return out
 

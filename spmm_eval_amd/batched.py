@@ -45,10 +45,11 @@ class BatchPlan:
 
     def info(self) -> dict:
         """Autotuned geometry of the fused launch."""
-        t, pers, ms = Tune(), C.c_int(), C.c_float()
-        call("spmm_batch_plan_info", self.p, C.byref(t), C.byref(pers), C.byref(ms))
+        t, pers, srows, ms = Tune(), C.c_int(), C.c_int(), C.c_float()
+        call("spmm_batch_plan_info", self.p, C.byref(t), C.byref(pers), C.byref(srows), C.byref(ms))
         return {"lanes_per_row": t.lanes_per_row, "vec": t.vec, "rows_scale": t.rows_scale,
-                "persistent": bool(pers.value), "tuned_ms": round(ms.value, 4)}
+                "persistent": bool(pers.value), "sddmm_rows": bool(srows.value),
+                "tuned_ms": round(ms.value, 4)}
 
     def step(self, X: torch.Tensor, dY: torch.Tensor, use_graph: bool = True, stream=None):
         call("spmm_batch_fwd_bwd", self.p, _p(X), _p(dY), _p(self.Y), _p(self.dX), _p(self.dval),

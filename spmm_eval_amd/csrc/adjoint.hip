@@ -19,6 +19,8 @@
 // batched launch).
 #include "csr_core.hpp"
 
+#include <string>
+
 #include <algorithm>
 #include <climits>
 
@@ -149,10 +151,37 @@ static void launch_sddmm(int M, int nnz, const int *rowptr, const int *colind, c
                      dim3(kAdjThreads), 0, s, M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile);
 }
 
+template <typename TB, int G>
+__global__ void __launch_bounds__(kAdjThreads)
+sddmm_row_kernel(LaunchTable tab, const int *__restrict__ rows, const int *__restrict__ rowptr,
+                 const int *__restrict__ colind, const TB *__restrict__ dY, int lddy,
+                 const TB *__restrict__ X, int ldx, float *__restrict__ out, int ntile) {
+  sddmm_row_wg<TB, G>(tab, blockIdx.x, rows, rowptr, colind, dY, lddy, X, ldx, out, ntile);
+}
+
+template <typename TB, int G>
+static void launch_sddmm_rows(const spmm_csr_plan_s *p, const int *rowptr, const int *colind,
+                              const TB *dY, int lddy, const TB *X, int ldx, float *out,
+                              int ntile, hipStream_t s) {
+  LaunchTable t;
+  build_table(p->counts, p->offs, G, 1, &t);
+  if (t.total_wg == 0) return;
+  hipLaunchKernelGGL((sddmm_row_kernel<TB, G>), dim3(t.total_wg), dim3(kAdjThreads), 0, s, t,
+                     p->d_rows, rowptr, colind, dY, lddy, X, ldx, out, ntile);
+}
+
+// plan == nullptr: entry-chunked (no schedule needed); else rows of the plan.
 template <typename TB>
-static int run_sddmm(int M, int K, int N, int nnz, const int *rowptr, const int *colind,
-                     const TB *dY, int lddy, const TB *X, int ldx, float *out, void *stream) {
+static int run_sddmm(const spmm_csr_plan_s *plan, int M, int K, int N, int nnz,
+                     const int *rowptr, const int *colind, const TB *dY, int lddy, const TB *X,
+                     int ldx, float *out, void *stream) {
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0 && nnz >= 0, "negative size");
+  if (plan) {
+    if (plan->tile != 0) SPMM_FAIL(SPMM_ERR_MISMATCH, "a B2SR plan was given to a CSR call");
+    if (plan->M != M || plan->nnz != nnz)
+      SPMM_FAIL(SPMM_ERR_MISMATCH, "plan was built for M=" + std::to_string(plan->M) +
+                                       " nnz=" + std::to_string(plan->nnz));
+  }
   if (M == 0 || nnz == 0) return SPMM_OK;
   SPMM_CHECK_ARG(rowptr && colind && out, "null rowptr, colind or out");
   SPMM_CHECK_ARG(N == 0 || (dY && X && lddy >= N && ldx >= N), "null operand or ld < N");
@@ -173,6 +202,16 @@ static int run_sddmm(int M, int K, int N, int nnz, const int *rowptr, const int 
   if (G == 0) {
     hipLaunchKernelGGL((sddmm_generic_kernel<TB>), dim3(ceil_div(M, 4)), dim3(kAdjThreads), 0, s,
                        M, N, rowptr, colind, dY, lddy, X, ldx, out);
+  } else if (plan) {
+    const int ntile = N / (G * e16);
+    switch (G) {
+      case 1: launch_sddmm_rows<TB, 1>(plan, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 2: launch_sddmm_rows<TB, 2>(plan, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 4: launch_sddmm_rows<TB, 4>(plan, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 8: launch_sddmm_rows<TB, 8>(plan, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      case 16: launch_sddmm_rows<TB, 16>(plan, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+      default: launch_sddmm_rows<TB, 32>(plan, rowptr, colind, dY, lddy, X, ldx, out, ntile, s); break;
+    }
   } else {
     const int ntile = N / (G * e16);
     switch (G) {
@@ -248,14 +287,29 @@ int sddmm_csr_f32(spmm_handle_t h, int M, int K, int N, int nnz, const int *rowp
                   const int *colind, const float *dY, int lddy, const float *X, int ldx,
                   float *dval, void *stream) {
   (void)h;
-  return run_sddmm<float>(M, K, N, nnz, rowptr, colind, dY, lddy, X, ldx, dval, stream);
+  return run_sddmm<float>(nullptr, M, K, N, nnz, rowptr, colind, dY, lddy, X, ldx, dval, stream);
 }
 
 int sddmm_csr_bf16(spmm_handle_t h, int M, int K, int N, int nnz, const int *rowptr,
                    const int *colind, const uint16_t *dY, int lddy, const uint16_t *X, int ldx,
                    float *dval, void *stream) {
   (void)h;
-  return run_sddmm<uint16_t>(M, K, N, nnz, rowptr, colind, dY, lddy, X, ldx, dval, stream);
+  return run_sddmm<uint16_t>(nullptr, M, K, N, nnz, rowptr, colind, dY, lddy, X, ldx, dval,
+                             stream);
+}
+
+int sddmm_csr_f32_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz, const int *rowptr,
+                          const int *colind, const float *dY, int lddy, const float *X, int ldx,
+                          float *dval, void *stream) {
+  SPMM_CHECK_ARG(plan != nullptr, "plan is null");
+  return run_sddmm<float>(plan, M, K, N, nnz, rowptr, colind, dY, lddy, X, ldx, dval, stream);
+}
+
+int sddmm_csr_bf16_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
+                           const int *rowptr, const int *colind, const uint16_t *dY, int lddy,
+                           const uint16_t *X, int ldx, float *dval, void *stream) {
+  SPMM_CHECK_ARG(plan != nullptr, "plan is null");
+  return run_sddmm<uint16_t>(plan, M, K, N, nnz, rowptr, colind, dY, lddy, X, ldx, dval, stream);
 }
 
 }  // extern "C"

@@ -304,11 +304,11 @@ b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
 // requested before this row's gathers), medium rows one wave per row (its
 // subgroups take tile chunks in turn, butterfly reduction), hub rows one
 // workgroup per row (LDS reduction in a fixed order).
-template <int T, typename TB, int G>
+template <int T, typename TB, int G, class LD>
 __device__ __forceinline__ void b2sr_range(int t, int t1, int stride, int r, int lane_sub,
                                            const int *__restrict__ bci,
                                            const typename Word<T>::type *__restrict__ words,
-                                           int *list, const TB *__restrict__ Bc, int ldb,
+                                           int *list, const LD &ld,
                                            float (&acc)[Vec16<TB>::kElems]) {
   constexpr int EPL = Vec16<TB>::kElems;
   constexpr int U = 8;
@@ -318,7 +318,7 @@ __device__ __forceinline__ void b2sr_range(int t, int t1, int stride, int r, int
       typename Vec16<TB>::Raw raw[U];
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        raw[u] = Vec16<TB>::raw(Bc + static_cast<size_t>(list[k + u < total ? k + u : 0]) * ldb);
+        raw[u] = ld(list[k + u < total ? k + u : 0]);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (k + u < total) {
@@ -352,7 +352,7 @@ b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
   const int sub = lane / G;
   const int gsub = wave * S + sub;
   const int col0 = blockIdx.y * CW + lane_sub * EPL;
-  const TB *Bc = B + col0;
+  const RowLoader<TB, false> ld(B, ldb, col0, 0);
   int *list = lds[gsub];
   const int local_wg = blockIdx.x - sg.wg_begin;
   float acc[EPL];
@@ -378,7 +378,7 @@ b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
       }
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-      b2sr_range<T, TB, G>(t0, t1, TC, row % T, lane_sub, bci, words, list, Bc, ldb, acc);
+      b2sr_range<T, TB, G>(t0, t1, TC, row % T, lane_sub, bci, words, list, ld, acc);
       float *Crow = C + static_cast<size_t>(row) * ldc + col0;
 #pragma unroll
       for (int q = 0; q < EPL; q += 4)
@@ -399,8 +399,8 @@ b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
   const int parts = wave_mode ? S : kB2Waves * S;
 #pragma unroll
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-  b2sr_range<T, TB, G>(t0 + part * TC, t1, parts * TC, row % T, lane_sub, bci, words, list, Bc,
-                       ldb, acc);
+  b2sr_range<T, TB, G>(t0 + part * TC, t1, parts * TC, row % T, lane_sub, bci, words, list, ld,
+                       acc);
 #pragma unroll
   for (int m = G; m < kWave; m <<= 1) {
 #pragma unroll
@@ -571,13 +571,12 @@ __device__ __forceinline__ void store_row(float *__restrict__ C, int ldc, int ro
 // streamed flat in groups of 8 gathers into ONE running sum, stored to C at
 // each row's last entry (a block row of ~40 bits costs ~5 gather round
 // trips). Rows without entries are zeroed.
-template <int T, typename TB, int G>
+template <int T, typename TB, int G, class LD>
 __device__ __forceinline__ void brow_stream(int t, int t1, int lane_sub,
                                             const int *__restrict__ bci,
                                             const typename TileWord<T>::type *__restrict__ tw,
-                                            int *list, const TB *__restrict__ Bc, int ldb,
-                                            float *__restrict__ C, int ldc, int row0, int M,
-                                            int col0) {
+                                            int *list, const LD &ld, float *__restrict__ C,
+                                            int ldc, int row0, int M, int col0) {
   constexpr int EPL = Vec16<TB>::kElems;
   constexpr int CAP = BrowChunk<G>::kCap;
   float cur[EPL];
@@ -600,7 +599,7 @@ __device__ __forceinline__ void brow_stream(int t, int t1, int lane_sub,
       for (int u = 0; u < 8; ++u) {
         const uint32_t ent = static_cast<uint32_t>(list[k + u < n ? k + u : k]);
         meta |= (ent & 15u) << (4 * u);
-        raw[u] = Vec16<TB>::raw(Bc + static_cast<size_t>(ent >> 4) * ldb);
+        raw[u] = ld(static_cast<int>(ent >> 4));
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -627,12 +626,11 @@ __device__ __forceinline__ void brow_stream(int t, int t1, int lane_sub,
 // once by all 64 lanes into the wave's lists; then row by row, the row's
 // entries dealt round-robin to the S subgroups (8 gathers in flight each),
 // a butterfly over the subgroups in a fixed order, subgroup 0 stores.
-template <int T, typename TB, int G>
+template <int T, typename TB, int G, class LD>
 __device__ __forceinline__ void brow_wave(int t, int t1, int lane, const int *__restrict__ bci,
                                           const typename TileWord<T>::type *__restrict__ tw,
-                                          int *wlist, const TB *__restrict__ Bc, int ldb,
-                                          float *__restrict__ C, int ldc, int row0, int M,
-                                          int col0) {
+                                          int *wlist, const LD &ld, float *__restrict__ C,
+                                          int ldc, int row0, int M, int col0) {
   constexpr int EPL = Vec16<TB>::kElems;
   constexpr int S = kWave / G;
   constexpr int CAPW = BrowChunk<G>::kWaveCap;
@@ -653,8 +651,7 @@ __device__ __forceinline__ void brow_wave(int t, int t1, int lane, const int *__
         typename Vec16<TB>::Raw raw[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          raw[u] = Vec16<TB>::raw(
-              Bc + static_cast<size_t>(wlist[k + u * S < hi ? k + u * S : k]) * ldb);
+          raw[u] = ld(wlist[k + u * S < hi ? k + u * S : k]);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           if (k + u * S < hi) {
@@ -681,13 +678,13 @@ __device__ __forceinline__ void brow_wave(int t, int t1, int lane, const int *__
 }
 
 // The work of virtual workgroup `vwg` of the block-row launch table.
-template <int T, typename TB, int G>
+template <int T, typename TB, int G, bool kBuf>
 __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *lds,
                                         const int *__restrict__ brows, int M,
                                         const int *__restrict__ brp, const int *__restrict__ bci,
                                         const typename TileWord<T>::type *__restrict__ tw,
-                                        const TB *__restrict__ B, int ldb, float *__restrict__ C,
-                                        int ldc) {
+                                        const TB *__restrict__ B, int ldb, int b_bytes,
+                                        float *__restrict__ C, int ldc) {
   constexpr int EPL = Vec16<TB>::kElems;
   constexpr int S = kWave / G;
   constexpr int CW = G * EPL;
@@ -699,7 +696,7 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
   const int sub = lane / G;
   const int gsub = wave * S + sub;
   const int col0 = blockIdx.y * CW + lane_sub * EPL;
-  const TB *Bc = B + col0;
+  const RowLoader<TB, kBuf> ld(B, ldb, col0, b_bytes);
   const int local_wg = vwg - sg.wg_begin;
 
   if (sg.kind == kSegA) {
@@ -722,7 +719,7 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
         t0n = brp[br_n];
         t1n = brp[br_n + 1];
       }
-      brow_stream<T, TB, G>(t0, t1, lane_sub, bci, tw, list, Bc, ldb, C, ldc, br * T, M, col0);
+      brow_stream<T, TB, G>(t0, t1, lane_sub, bci, tw, list, ld, C, ldc, br * T, M, col0);
       br = br_n;
       t0 = t0n;
       t1 = t1n;
@@ -734,8 +731,8 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
     const int pos = sg.list_off + local_wg * kB2Waves + wave;
     if (pos >= sg.list_off + sg.count) return;  // whole wave
     const int br = brows[pos];
-    brow_wave<T, TB, G>(brp[br], brp[br + 1], lane, bci, tw, lds + wave * S * CAP, Bc, ldb, C,
-                        ldc, br * T, M, col0);
+    brow_wave<T, TB, G>(brp[br], brp[br + 1], lane, bci, tw, lds + wave * S * CAP, ld, C, ldc,
+                        br * T, M, col0);
     return;
   }
   // class C: one workgroup per block row, wave w takes rows w, w + 4, ...
@@ -750,7 +747,7 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
     float acc[EPL];
 #pragma unroll
     for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    b2sr_range<T, TB, G>(t0 + sub * RC, t1, S * RC, r, lane_sub, bci, words, list, Bc, ldb, acc);
+    b2sr_range<T, TB, G>(t0 + sub * RC, t1, S * RC, r, lane_sub, bci, words, list, ld, acc);
 #pragma unroll
     for (int m = G; m < kWave; m <<= 1) {
 #pragma unroll
@@ -760,17 +757,18 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
   }
 }
 
-template <int T, typename TB, int G>
+template <int T, typename TB, int G, bool kBuf>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_brow_kernel(LaunchTable tab, const int *__restrict__ brows, int M,
                  const int *__restrict__ brp, const int *__restrict__ bci,
                  const typename TileWord<T>::type *__restrict__ tw, const TB *__restrict__ B,
-                 int ldb, float *__restrict__ C, int ldc) {
+                 int ldb, int b_bytes, float *__restrict__ C, int ldc) {
   constexpr int S = kWave / G;
   constexpr int CAP = BrowChunk<G>::kCap;
   static_assert(Chunk<T, G>::kCap == CAP, "all paths share the list area");
   __shared__ int lds[kB2Waves * S][CAP];
-  brow_wg<T, TB, G>(tab, blockIdx.x, &lds[0][0], brows, M, brp, bci, tw, B, ldb, C, ldc);
+  brow_wg<T, TB, G, kBuf>(tab, blockIdx.x, &lds[0][0], brows, M, brp, bci, tw, B, ldb, b_bytes,
+                          C, ldc);
 }
 
 // Any N / alignment: one wave per output row, lanes stride over columns.
@@ -1047,7 +1045,7 @@ static void dispatch_planned(const Geo &g, const spmm_csr_plan_s *p, int M, cons
 }
 
 template <int T, typename TB, int G>
-static void launch_brow(const spmm_csr_plan_s *p, int M, const int *brp, const int *bci,
+static void launch_brow(const spmm_csr_plan_s *p, int M, int K, const int *brp, const int *bci,
                         const void *val, const TB *B, int ldb, float *C, int ldc, int tiles,
                         hipStream_t s) {
   constexpr int S = kWave / G;
@@ -1058,23 +1056,30 @@ static void launch_brow(const spmm_csr_plan_s *p, int M, const int *brp, const i
   // wave path decodes exactly one chunk); the rest: a workgroup
   build_table_ex(p->counts, p->offs, S, TC, BrowChunk<G>::kWaveTiles, 8, 16, 1, &t);
   if (t.total_wg == 0) return;
-  hipLaunchKernelGGL((b2sr_brow_kernel<T, TB, G>), dim3(t.total_wg, tiles), dim3(kB2Threads), 0, s,
-                     t, p->d_rows, M, brp, bci,
-                     static_cast<const typename TileWord<T>::type *>(val), B, ldb, C, ldc);
+  const auto *tw = static_cast<const typename TileWord<T>::type *>(val);
+  // B rows are read through a buffer resource when B spans < 2 GiB
+  const int b_bytes = buffer_span(K, ldb, sizeof(TB));
+  if (b_bytes)
+    hipLaunchKernelGGL((b2sr_brow_kernel<T, TB, G, true>), dim3(t.total_wg, tiles),
+                       dim3(kB2Threads), 0, s, t, p->d_rows, M, brp, bci, tw, B, ldb, b_bytes, C,
+                       ldc);
+  else
+    hipLaunchKernelGGL((b2sr_brow_kernel<T, TB, G, false>), dim3(t.total_wg, tiles),
+                       dim3(kB2Threads), 0, s, t, p->d_rows, M, brp, bci, tw, B, ldb, 0, C, ldc);
 }
 
 template <int T, typename TB>
-static void dispatch_brow(const Geo &g, const spmm_csr_plan_s *p, int M, const int *brp,
+static void dispatch_brow(const Geo &g, const spmm_csr_plan_s *p, int M, int K, const int *brp,
                           const int *bci, const void *val, const TB *B, int ldb, float *C,
                           int ldc, hipStream_t s) {
   switch (g.G) {
-    case 1: launch_brow<T, TB, 1>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 2: launch_brow<T, TB, 2>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 4: launch_brow<T, TB, 4>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 8: launch_brow<T, TB, 8>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 16: launch_brow<T, TB, 16>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 32: launch_brow<T, TB, 32>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    default: launch_brow<T, TB, 64>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 1: launch_brow<T, TB, 1>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 2: launch_brow<T, TB, 2>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 4: launch_brow<T, TB, 4>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 8: launch_brow<T, TB, 8>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 16: launch_brow<T, TB, 16>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 32: launch_brow<T, TB, 32>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    default: launch_brow<T, TB, 64>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
   }
 }
 
@@ -1109,8 +1114,8 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     return SPMM_OK;
   }
   switch (T) {
-    case 4: dispatch_brow<4, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 8: dispatch_brow<8, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 16: dispatch_planned<16, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
     default: dispatch_planned<32, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
   }

@@ -4,8 +4,8 @@
 // long rows from different graphs share waves), builds the transposed union
 // for dX = A^T dY, and runs one fwd+bwd step -- Y = A X, dX = A^T dY,
 // dval = SDDMM(dY, X) -- as ONE kernel launch whose work items are the
-// forward CSR workgroups, the transposed CSR workgroups and the SDDMM row
-// blocks. In persistent mode a grid sized to the machine pulls items from
+// forward CSR workgroups, the transposed CSR workgroups and the SDDMM
+// workgroups (on the forward plan's rows). In persistent mode a grid sized to the machine pulls items from
 // an atomic queue that the last finishing workgroup resets (no memset node
 // is needed when the launch is captured). The plan's autotuner times every
 // (lanes per row, vector width, rows per subgroup, persistent or not)
@@ -28,7 +28,7 @@ struct spmm_batch_plan_s {
   int *counters = nullptr;      // [queue, done] of the persistent mode
   spmm_csr_plan_t fwd = nullptr, bwd = nullptr;
   // chosen geometry
-  int G = 0, VEC = 1, rows_scale = 1, persistent = 0, grid = 0;
+  int G = 0, VEC = 1, rows_scale = 1, persistent = 0, sddmm_rows = 1, grid = 0;
   float tuned_ms = 0.f;
   hipStream_t cap = nullptr;  // capture stream (default priority)
   hipGraph_t graph = nullptr;
@@ -50,9 +50,10 @@ __global__ void globalise_cols_kernel(int ngraphs, const int *__restrict__ row_o
 }
 
 struct FusedArgs {
-  LaunchTable fwd, bwd;
+  LaunchTable fwd, bwd, sdd;  // sdd: the forward plan's rows at the SDDMM lane count
   int ctiles;                 // CSR column tiles
   int n_fwd, n_bwd, n_sddmm;  // items of each part (CSR parts include column tiles)
+  int sddmm_rows;             // SDDMM on the forward plan's rows (1) or entry-chunked (0)
   int M, N, nnz, sddmm_ntile;
   const int *rows_fwd, *rows_bwd;
   const int *rowptr, *colind, *trowptr, *tcol;
@@ -62,7 +63,9 @@ struct FusedArgs {
   int *queue, *done;          // persistent mode only
 };
 
-template <typename TB, int G, int VEC, int GS>
+// kSddRows: SDDMM on the forward plan's rows (one form per kernel, so the
+// other form's registers do not lower the occupancy)
+template <typename TB, int G, int VEC, int GS, bool kSddRows>
 __device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
   const TB *X = static_cast<const TB *>(a.X);
   const TB *dY = static_cast<const TB *>(a.dY);
@@ -78,14 +81,19 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
     return;
   }
   item -= a.n_bwd;
-  sddmm_wg<TB, GS>(item, a.M, a.nnz, a.rowptr, a.colind, dY, a.N, X, a.N, a.dval, a.sddmm_ntile);
+  if (kSddRows)
+    sddmm_row_wg<TB, GS>(a.sdd, item, a.rows_fwd, a.rowptr, a.colind, dY, a.N, X, a.N, a.dval,
+                         a.sddmm_ntile);
+  else
+    sddmm_wg<TB, GS>(item, a.M, a.nnz, a.rowptr, a.colind, dY, a.N, X, a.N, a.dval,
+                     a.sddmm_ntile);
 }
 
-template <typename TB, int G, int VEC, int GS, bool kPersistent>
+template <typename TB, int G, int VEC, int GS, bool kPersistent, bool kSddRows>
 __global__ void __launch_bounds__(kThreads) fused_step_kernel(FusedArgs a) {
   const int total = a.n_fwd + a.n_bwd + a.n_sddmm;
   if (!kPersistent) {
-    fused_item<TB, G, VEC, GS>(a, blockIdx.x);
+    fused_item<TB, G, VEC, GS, kSddRows>(a, blockIdx.x);
     return;
   }
   __shared__ int next;
@@ -95,7 +103,7 @@ __global__ void __launch_bounds__(kThreads) fused_step_kernel(FusedArgs a) {
     const int item = next;
     __syncthreads();
     if (item >= total) break;
-    fused_item<TB, G, VEC, GS>(a, item);
+    fused_item<TB, G, VEC, GS, kSddRows>(a, item);
   }
   if (threadIdx.x == 0) {
     // Every workgroup stops fetching before it counts itself done, so the
@@ -114,13 +122,20 @@ static int sddmm_G(int N, int e16) {
   return 0;
 }
 
+template <typename TB, int G, int VEC, int GS, bool kSddRows>
+static void launch_fused_k(const FusedArgs &a, bool persistent, int grid, hipStream_t s) {
+  if (persistent)
+    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, true, kSddRows>), dim3(grid),
+                       dim3(kThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, false, kSddRows>),
+                       dim3(a.n_fwd + a.n_bwd + a.n_sddmm), dim3(kThreads), 0, s, a);
+}
+
 template <typename TB, int G, int VEC, int GS>
 static int launch_fused_g(const FusedArgs &a, bool persistent, int grid, hipStream_t s) {
-  if (persistent)
-    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, true>), dim3(grid), dim3(kThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, false>),
-                       dim3(a.n_fwd + a.n_bwd + a.n_sddmm), dim3(kThreads), 0, s, a);
+  if (a.sddmm_rows) launch_fused_k<TB, G, VEC, GS, true>(a, persistent, grid, s);
+  else launch_fused_k<TB, G, VEC, GS, false>(a, persistent, grid, s);
   return hipGetLastError() == hipSuccess ? SPMM_OK : SPMM_ERR_HIP;
 }
 
@@ -156,8 +171,8 @@ static int launch_fused(int G, int VEC, int GS, const FusedArgs &a, bool persist
 }
 
 static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, int persistent,
-                        const void *X, const void *dY, float *Y, float *dX, float *dval,
-                        hipStream_t s) {
+                        int sddmm_rows, const void *X, const void *dY, float *Y, float *dX,
+                        float *dval, hipStream_t s) {
   const int elem = p->bf16 ? 2 : 4;
   const int e16 = 16 / elem;
   FusedArgs a;
@@ -167,7 +182,11 @@ static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, in
   a.n_fwd = a.fwd.total_wg * a.ctiles;
   a.n_bwd = a.bwd.total_wg * a.ctiles;
   const int GS = sddmm_G(p->N, e16);
-  a.n_sddmm = static_cast<int>(ceil_div_ll(p->nnz, static_cast<long long>(kThreads / GS) * kSddmmChunk));
+  build_table(p->fwd->counts, p->fwd->offs, GS, 1, &a.sdd);
+  a.sddmm_rows = sddmm_rows;
+  a.n_sddmm = sddmm_rows ? a.sdd.total_wg
+                         : static_cast<int>(ceil_div_ll(
+                               p->nnz, static_cast<long long>(kThreads / GS) * kSddmmChunk));
   a.sddmm_ntile = p->N / (GS * e16);
   a.M = p->M;
   a.N = p->N;
@@ -217,38 +236,43 @@ static int tune_plan(spmm_batch_plan_s *p) {
               hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     st = SPMM_ERR_HIP;
   float best = 1e30f;
-  const int lanes_max = p->N / e16;
-  for (int vec = 1; vec <= 2 && !st; ++vec) {
-    if (p->N % (e16 * vec)) continue;
-    const int lanes = p->N / (e16 * vec);
-    for (int G = 64; G >= 4 && !st; G >>= 1) {
-      if (lanes % G || G > lanes || G * 4 < std::min(lanes, 64)) continue;
-      for (int rs = 1; rs <= 8 && !st; rs <<= 1) {
-        for (int pers = 0; pers <= 1 && !st; ++pers) {
-          st = enqueue_step(p, G, vec, rs, pers, X, dY, Y, dX, dval, p->cap);  // warm-up
-          if (st) break;
-          if (hipEventRecord(e0, p->cap) != hipSuccess) { st = SPMM_ERR_HIP; break; }
-          for (int r = 0; r < 3 && !st; ++r)
-            st = enqueue_step(p, G, vec, rs, pers, X, dY, Y, dX, dval, p->cap);
-          if (st) break;
-          if (hipEventRecord(e1, p->cap) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-            st = SPMM_ERR_HIP;
-            break;
-          }
-          float ms = 0.f;
-          (void)hipEventElapsedTime(&ms, e0, e1);
-          if (ms < best) {
-            best = ms;
-            p->G = G;
-            p->VEC = vec;
-            p->rows_scale = rs;
-            p->persistent = pers;
+  // one warm-up and three timed steps of a candidate (ms for the three)
+  auto time3 = [&](int G, int vec, int rs, int pers, int srows, float *ms) {
+    int rc = enqueue_step(p, G, vec, rs, pers, srows, X, dY, Y, dX, dval, p->cap);
+    if (rc) return rc;
+    if (hipEventRecord(e0, p->cap) != hipSuccess) return static_cast<int>(SPMM_ERR_HIP);
+    for (int r = 0; r < 3 && !rc; ++r)
+      rc = enqueue_step(p, G, vec, rs, pers, srows, X, dY, Y, dX, dval, p->cap);
+    if (rc) return rc;
+    if (hipEventRecord(e1, p->cap) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+      return static_cast<int>(SPMM_ERR_HIP);
+    (void)hipEventElapsedTime(ms, e0, e1);
+    return static_cast<int>(SPMM_OK);
+  };
+  // SDDMM form: row-scheduled (1) or entry-chunked (0); bitwise equal
+  for (int srows = 1; srows >= 0 && !st; --srows) {
+    for (int vec = 1; vec <= 2 && !st; ++vec) {
+      if (p->N % (e16 * vec)) continue;
+      const int lanes = p->N / (e16 * vec);
+      for (int G = 64; G >= 4 && !st; G >>= 1) {
+        if (lanes % G || G > lanes || G * 4 < std::min(lanes, 64)) continue;
+        for (int rs = 1; rs <= 8 && !st; rs <<= 1) {
+          for (int pers = 0; pers <= 1 && !st; ++pers) {
+            float ms = 0.f;
+            st = time3(G, vec, rs, pers, srows, &ms);
+            if (!st && ms < best) {
+              best = ms;
+              p->G = G;
+              p->VEC = vec;
+              p->rows_scale = rs;
+              p->persistent = pers;
+              p->sddmm_rows = srows;
+            }
           }
         }
       }
     }
   }
-  (void)lanes_max;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(p->cap);
@@ -357,10 +381,12 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off, con
   return SPMM_OK;
 }
 
-int spmm_batch_plan_info(spmm_batch_plan_t p, spmm_tune_t *tune, int *persistent, float *tuned_ms) {
+int spmm_batch_plan_info(spmm_batch_plan_t p, spmm_tune_t *tune, int *persistent,
+                         int *sddmm_rows, float *tuned_ms) {
   SPMM_CHECK_ARG(p, "null plan");
   if (tune) *tune = {p->G, p->VEC, p->rows_scale};
   if (persistent) *persistent = p->persistent;
+  if (sddmm_rows) *sddmm_rows = p->sddmm_rows;
   if (tuned_ms) *tuned_ms = p->tuned_ms;
   return SPMM_OK;
 }
@@ -371,7 +397,8 @@ int spmm_batch_fwd_bwd(spmm_batch_plan_t p, const void *X, const void *dY, float
   SPMM_CHECK_ARG(X && dY && Y && dX && (dval || p->nnz == 0), "null operand");
   hipStream_t s = as_stream(stream);
   if (!use_graph) {
-    const int st = enqueue_step(p, p->G, p->VEC, p->rows_scale, p->persistent, X, dY, Y, dX, dval, s);
+    const int st = enqueue_step(p, p->G, p->VEC, p->rows_scale, p->persistent, p->sddmm_rows, X,
+                                dY, Y, dX, dval, s);
     if (st) SPMM_FAIL(st, "fused launch failed");
     return SPMM_OK;
   }
@@ -379,8 +406,8 @@ int spmm_batch_fwd_bwd(spmm_batch_plan_t p, const void *X, const void *dY, float
     destroy_graph(p);
     // Capture only records the launch; the graph runs on s below.
     SPMM_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-    const int st = enqueue_step(p, p->G, p->VEC, p->rows_scale, p->persistent, X, dY, Y, dX, dval,
-                                p->cap);
+    const int st = enqueue_step(p, p->G, p->VEC, p->rows_scale, p->persistent, p->sddmm_rows, X,
+                                dY, Y, dX, dval, p->cap);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(p->cap, &g);
     if (st) {

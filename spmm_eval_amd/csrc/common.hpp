@@ -114,6 +114,44 @@ template <> struct Vec16<uint16_t> {
   __device__ __forceinline__ static void load(const uint16_t *p, float *out) { widen(raw(p), out); }
 };
 
+// Gathers of 16-byte row segments of a dense operand. RowLoader<TB, false>
+// uses flat 64-bit addresses; RowLoader<TB, true> goes through a buffer
+// resource with 32-bit byte offsets (one VALU op and one VGPR per address
+// instead of 64-bit arithmetic and two), usable when the operand spans
+// less than 2 GiB; the hardware also range-checks every such load (reads
+// past num_records return zeros).
+constexpr int kBufferWord3 = 0x00020000;  // gfx9-family resource: 32-bit data format
+
+template <typename TB, bool kBuf> struct RowLoader;
+template <typename TB> struct RowLoader<TB, false> {
+  const TB *base;  // already offset to the lane's first column
+  int ld;          // elements per row
+  __device__ __forceinline__ RowLoader(const TB *B, int ldb, int col, int) : base(B + col), ld(ldb) {}
+  __device__ __forceinline__ typename Vec16<TB>::Raw operator()(int row) const {
+    return Vec16<TB>::raw(base + static_cast<size_t>(row) * ld);
+  }
+};
+template <typename TB> struct RowLoader<TB, true> {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int ld_bytes, col_bytes;
+  __device__ __forceinline__ RowLoader(const TB *B, int ldb, int col, int bytes)
+      : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<TB *>(B), static_cast<short>(0), bytes,
+                                               kBufferWord3)),
+        ld_bytes(ldb * static_cast<int>(sizeof(TB))),
+        col_bytes(col * static_cast<int>(sizeof(TB))) {}
+  __device__ __forceinline__ typename Vec16<TB>::Raw operator()(int row) const {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, row * ld_bytes + col_bytes, 0, 0);
+    return __builtin_bit_cast(typename Vec16<TB>::Raw, v);
+  }
+};
+
+// Byte span of rows [0, rows) of a [rows][ld] operand if a buffer resource
+// can address it (< 2^31), else 0.
+inline int buffer_span(long long rows, long long ld, int elem_bytes) {
+  const long long b = rows * ld * elem_bytes;
+  return (b > 0 && b < (1ll << 31)) ? static_cast<int>(b) : 0;
+}
+
 template <typename TB>
 __device__ __forceinline__ float load_scalar(const TB *p);
 template <> __device__ __forceinline__ float load_scalar<float>(const float *p) {

@@ -331,6 +331,135 @@ __device__ __forceinline__ void sddmm_wg(int wg, int M, int nnz, const int *__re
   }
 }
 
+// Row-scheduled SDDMM on a CSR plan: the rows run in the CSR classes
+// (subgroup / wave / workgroup per row by degree), and a row's entries are
+// dealt to its subgroups in chunks of G (lane l reads the chunk's entry l;
+// the next chunk's indices are requested before this chunk's gathers). Each
+// entry's dot product stays inside one subgroup (lane partials over EPL
+// columns, the same reduce-scatter + butterfly as sddmm_wg, hence the same
+// bits), so no cross-subgroup reduction exists. Column tiles beyond the
+// first add into out.
+template <typename TB, int G>
+__device__ __forceinline__ void sddmm_range(int e_begin, int e_end, int first_off, int stride,
+                                            int lane_sub, const int *__restrict__ colind,
+                                            const float (&y)[Vec16<TB>::kElems],
+                                            const TB *__restrict__ Xc, int ldx,
+                                            float *__restrict__ out, bool first_tile) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int L = G < 8 ? G : 8;
+  int e = e_begin + first_off;
+  int c = e + lane_sub < e_end ? colind[e + lane_sub] : 0;
+  for (; e < e_end; e += stride) {
+    const int n = min(G, e_end - e);
+    const int my_n = e + stride + lane_sub;
+    const int c_n = my_n < e_end ? colind[my_n] : 0;
+    for (int k = 0; k < n; k += L) {
+      const int nl = min(L, n - k);
+      typename Vec16<TB>::Raw xr[L];
+#pragma unroll
+      for (int u = 0; u < L; ++u) {
+        const int j = __shfl(c, k + u < G ? k + u : 0, G);  // slots past nl read a valid row
+        xr[u] = Vec16<TB>::raw(Xc + static_cast<size_t>(j) * ldx);
+      }
+      float p[L];
+#pragma unroll
+      for (int u = 0; u < L; ++u) {
+        float d = 0.f;
+        if (u < nl) {
+          float x[EPL];
+          Vec16<TB>::widen(xr[u], x);
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) d = fmaf(y[q], x[q], d);
+        }
+        p[u] = d;
+      }
+#pragma unroll
+      for (int m = L / 2; m >= 1; m >>= 1) {
+        const bool hi_half = (lane_sub & m) != 0;
+#pragma unroll
+        for (int i = 0; i < m; ++i) {
+          const float send = hi_half ? p[i] : p[i + m];
+          const float keep = hi_half ? p[i + m] : p[i];
+          p[i] = keep + __shfl_xor(send, m, G);
+        }
+      }
+#pragma unroll
+      for (int m = L; m < G; m <<= 1) p[0] += __shfl_xor(p[0], m, G);
+      if (lane_sub < nl) {
+        float *o = out + e + k + lane_sub;
+        if (first_tile) *o = p[0];
+        else *o += p[0];
+      }
+    }
+    c = c_n;
+  }
+}
+
+// The SDDMM work of virtual workgroup `wg` of a CSR plan's launch table.
+template <typename TB, int G>
+__device__ __forceinline__ void sddmm_row_wg(const LaunchTable &tab, int wg,
+                                             const int *__restrict__ rows,
+                                             const int *__restrict__ rowptr,
+                                             const int *__restrict__ colind,
+                                             const TB *__restrict__ dY, int lddy,
+                                             const TB *__restrict__ X, int ldx,
+                                             float *__restrict__ out, int ntile) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * EPL;
+  const Seg sg = find_seg(tab, wg);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1);
+  const int sub = lane / G;
+  const int gsub = wave * S + sub;
+  const int local_wg = wg - sg.wg_begin;
+  float y[EPL];
+  if (sg.kind == kSegA) {
+    const int R = sg.rows_per_sub;
+    const int subs_per_wg = kWavesPerWG * S;
+    const int end = sg.list_off + sg.count;
+    int pos = sg.list_off + local_wg * subs_per_wg * R + gsub;
+    int row = pos < end ? rows[pos] : -1;
+    int e0 = 0, e1 = 0;
+    if (row >= 0) {
+      e0 = rowptr[row];
+      e1 = rowptr[row + 1];
+    }
+    for (int k = 0; k < R && row >= 0; ++k) {
+      const int pos_n = pos + subs_per_wg;
+      const int row_n = (k + 1 < R && pos_n < end) ? rows[pos_n] : -1;
+      int e0n = 0, e1n = 0;
+      if (row_n >= 0) {
+        e0n = rowptr[row_n];
+        e1n = rowptr[row_n + 1];
+      }
+      for (int tt = 0; tt < ntile && e0 < e1; ++tt) {
+        Vec16<TB>::load(dY + static_cast<size_t>(row) * lddy + tt * CW + lane_sub * EPL, y);
+        sddmm_range<TB, G>(e0, e1, 0, G, lane_sub, colind, y, X + tt * CW + lane_sub * EPL, ldx,
+                           out, tt == 0);
+      }
+      row = row_n;
+      e0 = e0n;
+      e1 = e1n;
+      pos = pos_n;
+    }
+    return;
+  }
+  const bool wave_mode = sg.kind == kSegB;
+  const int pos = wave_mode ? sg.list_off + local_wg * kWavesPerWG + wave : sg.list_off + local_wg;
+  if (pos >= sg.list_off + sg.count) return;  // whole wave (no barrier here)
+  const int row = rows[pos];
+  const int e0 = rowptr[row], e1 = rowptr[row + 1];
+  const int part = wave_mode ? sub : gsub;
+  const int parts = wave_mode ? S : kWavesPerWG * S;
+  for (int tt = 0; tt < ntile; ++tt) {
+    Vec16<TB>::load(dY + static_cast<size_t>(row) * lddy + tt * CW + lane_sub * EPL, y);
+    sddmm_range<TB, G>(e0, e1, part * G, parts * G, lane_sub, colind, y,
+                       X + tt * CW + lane_sub * EPL, ldx, out, tt == 0);
+  }
+}
+
 // Host side.
 struct Geometry {
   int G = 0;    // lanes per row (0 = scalar kernel)

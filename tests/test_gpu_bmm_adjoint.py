@@ -11,6 +11,7 @@ import torch
 
 import oracle
 from spmm_eval_amd import api, graphs
+from spmm_eval_amd._lib import SpmmError
 
 pytestmark = pytest.mark.gpu
 HAND = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "b2sr_hand.json")))
@@ -161,3 +162,43 @@ def test_autograd_finite_differences(handle, dev, gcn):
         ref = (AT[j] @ G64[:, k]).item() if AT[j].nnz else 0.0
         assert abs(fd - ref) < 1e-9 + 1e-9 * abs(ref)
         assert abs(fd - gx[j, k]) <= 1e-5 * (np.abs(AT[j]).sum() * np.abs(G64[:, k]).max() + 1e-12)
+
+
+@pytest.mark.parametrize("M,K,dens,nl,ll,N", [
+    (2000, 1500, 0.004, 3, 1500, 128),   # hub rows: wave and workgroup classes
+    (700, 6000, 0.002, 2, 6000, 512),    # 4 column tiles of 128 (FP32)
+    (300, 300, 0.05, 1, 300, 24),        # G = 2 lanes
+    (50, 40, 0.2, 0, 0, 3),              # N not a multiple of 4: generic kernel
+])
+def test_sddmm_planned_equals_plan_free(handle, dev, M, K, dens, nl, ll, N):
+    """The row-scheduled SDDMM keeps each entry's dot in one subgroup with
+    the plan-free kernel's reduction tree: bitwise equal, FP32 and BF16,
+    and within 1e-5 of FP64."""
+    g = graphs.random_csr(M, K, dens, seed=M + N, long_rows=nl, long_len=ll)
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    dY = graphs.normal((M, N), 5, 1)
+    X = graphs.normal((K, N), 5, 2)
+    dYd, Xd = torch.from_numpy(dY).to(dev), torch.from_numpy(X).to(dev)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    a = api.sddmm_csr(handle, rp, ci, dYd, Xd)
+    b = api.sddmm_csr(handle, rp, ci, dYd, Xd, plan=plan)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    ref = oracle.sddmm_f64(g.rowptr, g.colind, dY, X)
+    assert np.linalg.norm(b.cpu().numpy() - ref) / np.linalg.norm(ref) <= 1e-5
+    if N % 8 == 0:
+        dYh, Xh = api.to_bf16_bits(dYd), api.to_bf16_bits(Xd)
+        ah = api.sddmm_csr(handle, rp, ci, dYh, Xh)
+        bh = api.sddmm_csr(handle, rp, ci, dYh, Xh, plan=plan)
+        assert torch.equal(ah, bh)
+
+
+def test_sddmm_planned_mismatch(handle, dev):
+    g = graphs.random_csr(100, 100, 0.05, seed=2)
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    g2 = graphs.random_csr(100, 100, 0.08, seed=3)
+    plan = api.CsrPlan(handle, torch.from_numpy(g2.rowptr).to(dev), g2.nnz)
+    X = torch.ones(100, 16, device=dev)
+    with pytest.raises(SpmmError) as ei:
+        api.sddmm_csr(handle, rp, ci, X, X, plan=plan)
+    assert ei.value.status == 5
