@@ -44,6 +44,7 @@ extern "C" {
 
 typedef struct spmm_handle_s *spmm_handle_t;       /* workspace + device id */
 typedef struct spmm_csr_plan_s *spmm_csr_plan_t;   /* row schedule of a CSR */
+typedef struct spmm_csrt_plan_s *spmm_csrt_plan_t; /* transposed CSR + schedule */
 typedef struct spmm_batch_plan_s *spmm_batch_plan_t;
 
 /* Launch geometry override (the autotuner's knobs). Zero fields = default.
@@ -204,6 +205,36 @@ int bmm32_xnor_bin(int M, int N, int Kbits, const uint32_t *A,
 int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
                   const int *colind, const float *val, int *tRowPtr,
                   int *tColInd, float *tVal, int *perm, void *stream);
+
+/* Transposed SpMM C[K][N] = A^T B, B [M][ldb], from A's own CSR (no
+ * counterpart in the reference; the transposed structure is csr2csc's,
+ * backend/readMtx.hpp:319-345). spmm_csr_t_analyze builds the stable
+ * transpose, its entry permutation and its row schedule once (SYNCHRONISES);
+ * the planned calls read A's values through the permutation, so val may
+ * change between calls (the backward pass dX = A^T dY of learnable edge
+ * weights). The drop-in forms analyse, run and free per call. */
+int spmm_csr_t_analyze(spmm_handle_t h, int M, int K, int nnz,
+                       const int *rowptr, const int *colind,
+                       spmm_csrt_plan_t *plan, void *stream);
+int spmm_csr_t_plan_destroy(spmm_csrt_plan_t plan);
+int spmm_csr_t_autotune(spmm_csrt_plan_t plan, int M, int K, int N, int nnz,
+                        const float *val, const void *B, int bf16, int ldb,
+                        float *C, int ldc, spmm_tune_t *best, void *stream);
+int spmm_csr_t_f32_planned(spmm_csrt_plan_t plan, int M, int K, int N,
+                           int nnz, const float *val, const float *B,
+                           int ldb, float *C, int ldc,
+                           const spmm_tune_t *tune, void *stream);
+int spmm_csr_t_bf16_planned(spmm_csrt_plan_t plan, int M, int K, int N,
+                            int nnz, const float *val, const uint16_t *B,
+                            int ldb, float *C, int ldc,
+                            const spmm_tune_t *tune, void *stream);
+int spmm_csr_t_f32(spmm_handle_t h, int M, int K, int N, int nnz,
+                   const int *rowptr, const int *colind, const float *val,
+                   const float *B, int ldb, float *C, int ldc, void *stream);
+int spmm_csr_t_bf16(spmm_handle_t h, int M, int K, int N, int nnz,
+                    const int *rowptr, const int *colind, const float *val,
+                    const uint16_t *B, int ldb, float *C, int ldc,
+                    void *stream);
 
 /* SDDMM at the stored positions: dval[e] = sum_k dY[i][k] * X[j][k] for
  * stored (i, j) = entry e, FP32 accumulation. dY [M][lddy], X [K][ldx]. */

@@ -68,6 +68,13 @@ _SIGS = {
     "csr_transpose": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sddmm_csr_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "sddmm_csr_bf16": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
+    "spmm_csr_t_analyze": [_vp, _i, _i, _i, _vp, _vp, C.POINTER(_vp), _vp],
+    "spmm_csr_t_plan_destroy": [_vp],
+    "spmm_csr_t_autotune": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp, _i, C.POINTER(Tune), _vp],
+    "spmm_csr_t_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp],
+    "spmm_csr_t_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp],
+    "spmm_csr_t_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
+    "spmm_csr_t_bf16": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "sddmm_csr_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "sddmm_csr_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_batch_plan_create": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, C.POINTER(_vp), _vp],

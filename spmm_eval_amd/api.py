@@ -100,6 +100,60 @@ class CsrPlan:
             pass
 
 
+class CsrTPlan:
+    """Plan of C = A^T B (spmm_csr_t_analyze): A's stable transpose, its entry
+    permutation and its row schedule; values are read through the
+    permutation at every call."""
+
+    def __init__(self, h: Handle, rowptr: torch.Tensor, colind: torch.Tensor, K: int,
+                 stream=None):
+        self.M = rowptr.numel() - 1
+        self.K = int(K)
+        self.nnz = colind.numel()
+        p = C.c_void_p()
+        call("spmm_csr_t_analyze", h.h, self.M, self.K, self.nnz, _p(rowptr), _p(colind),
+             C.byref(p), _stream(stream))
+        self.p = p
+
+    def autotune(self, val, B, C_out, stream=None) -> tuple:
+        best = Tune()
+        call("spmm_csr_t_autotune", self.p, self.M, self.K, B.shape[1], self.nnz, _p(val), _p(B),
+             int(_is_bf16(B)), B.stride(0), _p(C_out), C_out.stride(0), C.byref(best),
+             _stream(stream))
+        return (best.lanes_per_row, best.vec, best.rows_scale)
+
+    def close(self):
+        if self.p:
+            _lib.lib().spmm_csr_t_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spmm_csr_t(h: Handle, rowptr, colind, val, B, K: int, C_out=None,
+               plan: CsrTPlan | None = None, tune: tuple | None = None, stream=None):
+    """C = A^T B for CSR A [M][K] and row-major B [M][N] (FP32 or BF16);
+    FP32 C [K][N]."""
+    M = rowptr.numel() - 1
+    nnz = colind.numel()
+    N = B.shape[1]
+    if C_out is None:
+        C_out = torch.empty((K, N), dtype=torch.float32, device=B.device)
+    bf16 = _is_bf16(B)
+    if plan is None:
+        call("spmm_csr_t_bf16" if bf16 else "spmm_csr_t_f32", h.h, M, K, N, nnz, _p(rowptr),
+             _p(colind), _p(val), _p(B), B.stride(0), _p(C_out), C_out.stride(0), _stream(stream))
+    else:
+        t = C.byref(Tune(*tune)) if tune else None
+        call("spmm_csr_t_bf16_planned" if bf16 else "spmm_csr_t_f32_planned", plan.p, M, K, N, nnz,
+             _p(val), _p(B), B.stride(0), _p(C_out), C_out.stride(0), t, _stream(stream))
+    return C_out
+
+
 def spmm_csr(h: Handle, rowptr, colind, val, B, K: int | None = None, C_out=None,
              plan: CsrPlan | None = None, tune: tuple | None = None, stream=None):
     """C = A * B for CSR A (FP32 values) and row-major B (FP32, or BF16

@@ -30,12 +30,15 @@ struct LaunchTable {
 };
 
 // One subgroup (G lanes) accumulates entries [e_begin, e_end) taking chunks
-// of G entries at stride `chunk_stride`, starting at chunk `first`.
-template <typename TB, int G, int EPL>
+// of G entries at stride `chunk_stride`, starting at chunk `first`. With
+// kPerm the value of entry e is val[perm[e]] (a transposed structure whose
+// values stay in the source order, so learnable values need no re-gather).
+template <typename TB, int G, int EPL, bool kPerm = false>
 __device__ __forceinline__ void accumulate_range(
     int e_begin, int e_end, int first_chunk_off, int chunk_stride, int lane_sub,
     const int *__restrict__ colind, const float *__restrict__ val,
-    const TB *__restrict__ Bc, int ldb, float (&acc)[EPL]) {
+    const TB *__restrict__ Bc, int ldb, float (&acc)[EPL],
+    const int *__restrict__ perm = nullptr) {
   constexpr int U0 = EPL >= 16 ? 4 : 8;  // loads in flight per lane
   constexpr int U = G < U0 ? G : U0;
   constexpr int V = EPL / Vec16<TB>::kElems;
@@ -44,7 +47,7 @@ __device__ __forceinline__ void accumulate_range(
   float v = 0.f;
   if (e + lane_sub < e_end) {
     cidx = colind[e + lane_sub];
-    v = val[e + lane_sub];
+    v = val[kPerm ? perm[e + lane_sub] : e + lane_sub];
   }
   for (; e < e_end; e += chunk_stride) {
     const int n = min(G, e_end - e);
@@ -54,7 +57,7 @@ __device__ __forceinline__ void accumulate_range(
     float v_n = 0.f;
     if (my_n < e_end) {
       cidx_n = colind[my_n];
-      v_n = val[my_n];
+      v_n = val[kPerm ? perm[my_n] : my_n];
     }
     for (int k = 0; k < n; k += U) {
       int cc[U];
@@ -133,14 +136,15 @@ __device__ __forceinline__ Seg find_seg(const LaunchTable &tab, int wg) {
 // The work of one 256-thread workgroup of the CSR launch table: virtual
 // workgroup `wg`, column tile `ctile`. Used by csr_spmm_kernel and by the
 // fused batched kernel.
-template <typename TB, int G, int VEC>
+template <typename TB, int G, int VEC, bool kPerm = false>
 __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile,
                                        const int *__restrict__ rows,
                                        const int *__restrict__ rowptr,
                                        const int *__restrict__ colind,
                                        const float *__restrict__ val,
                                        const TB *__restrict__ B, int ldb,
-                                       float *__restrict__ C, int ldc) {
+                                       float *__restrict__ C, int ldc,
+                                       const int *__restrict__ perm = nullptr) {
   constexpr int EPL = VEC * Vec16<TB>::kElems;
   constexpr int S = kWave / G;
   constexpr int CW = G * EPL;  // columns covered per column tile
@@ -185,7 +189,8 @@ __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile
       float acc[EPL];
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-      accumulate_range<TB, G, EPL>(e0, e1, 0, G, lane_sub, colind, val, Bc, ldb, acc);
+      accumulate_range<TB, G, EPL, kPerm>(e0, e1, 0, G, lane_sub, colind, val, Bc, ldb, acc,
+                                          perm);
       store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
       row = row_n;
       e0 = e0n;
@@ -200,8 +205,8 @@ __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile
     float acc[EPL];
 #pragma unroll
     for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    accumulate_range<TB, G, EPL>(rowptr[row], rowptr[row + 1], sub * G, S * G,
-                                 lane_sub, colind, val, Bc, ldb, acc);
+    accumulate_range<TB, G, EPL, kPerm>(rowptr[row], rowptr[row + 1], sub * G, S * G,
+                                        lane_sub, colind, val, Bc, ldb, acc, perm);
     wave_reduce_subgroups<G, EPL>(acc);
     if (sub == 0) store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
   } else {
@@ -210,9 +215,9 @@ __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile
 #pragma unroll
     for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
     const int gsub = wave * S + sub;
-    accumulate_range<TB, G, EPL>(rowptr[row], rowptr[row + 1], gsub * G,
-                                 kWavesPerWG * S * G, lane_sub, colind, val, Bc,
-                                 ldb, acc);
+    accumulate_range<TB, G, EPL, kPerm>(rowptr[row], rowptr[row + 1], gsub * G,
+                                        kWavesPerWG * S * G, lane_sub, colind, val, Bc,
+                                        ldb, acc, perm);
     wave_reduce_subgroups<G, EPL>(acc);
     if (wave > 0 && sub == 0) {
 #pragma unroll

@@ -77,13 +77,24 @@ csr_spmm_kernel(LaunchTable tab, const int *__restrict__ rows,
   csr_wg<TB, G, VEC>(tab, blockIdx.x, blockIdx.y, rows, rowptr, colind, val, B, ldb, C, ldc);
 }
 
+// C = A^T B on the transposed structure of A, values read through perm.
+template <typename TB, int G, int VEC>
+__global__ void __launch_bounds__(kThreads)
+csr_spmm_perm_kernel(LaunchTable tab, const int *__restrict__ rows,
+                     const int *__restrict__ rowptr, const int *__restrict__ colind,
+                     const float *__restrict__ val, const int *__restrict__ perm,
+                     const TB *__restrict__ B, int ldb, float *__restrict__ C, int ldc) {
+  csr_wg<TB, G, VEC, true>(tab, blockIdx.x, blockIdx.y, rows, rowptr, colind, val, B, ldb, C,
+                           ldc, perm);
+}
+
 // Any N, any alignment: one wave per row, lanes stride over the columns.
 template <typename TB>
 __global__ void __launch_bounds__(kThreads)
 csr_spmm_generic_kernel(int M, int N, const int *__restrict__ rowptr,
                         const int *__restrict__ colind, const float *__restrict__ val,
                         const TB *__restrict__ B, int ldb, float *__restrict__ C,
-                        int ldc) {
+                        int ldc, const int *__restrict__ perm) {
   const int row = blockIdx.x * kWavesPerWG + threadIdx.x / kWave;
   if (row >= M) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -91,7 +102,8 @@ csr_spmm_generic_kernel(int M, int N, const int *__restrict__ rowptr,
   for (int c = lane; c < N; c += kWave) {
     float acc = 0.f;
     for (int e = e0; e < e1; ++e)
-      acc = fmaf(val[e], load_scalar<TB>(B + static_cast<size_t>(colind[e]) * ldb + c), acc);
+      acc = fmaf(val[perm ? perm[e] : e],
+                 load_scalar<TB>(B + static_cast<size_t>(colind[e]) * ldb + c), acc);
     C[static_cast<size_t>(row) * ldc + c] = acc;
   }
 }
@@ -179,33 +191,41 @@ void build_table(const int *counts, const int *offs, int G, int rows_scale, Laun
 template <typename TB, int G, int VEC>
 static void launch_vec(const LaunchTable &t, int col_tiles, const spmm_csr_plan_s *p,
                        const int *rowptr, const int *colind, const float *val,
-                       const TB *B, int ldb, float *C, int ldc, hipStream_t s) {
+                       const int *perm, const TB *B, int ldb, float *C, int ldc,
+                       hipStream_t s) {
   dim3 grid(t.total_wg, col_tiles);
-  hipLaunchKernelGGL((csr_spmm_kernel<TB, G, VEC>), grid, dim3(kThreads), 0, s, t,
-                     p->d_rows, rowptr, colind, val, B, ldb, C, ldc);
+  if (perm)
+    hipLaunchKernelGGL((csr_spmm_perm_kernel<TB, G, VEC>), grid, dim3(kThreads), 0, s, t,
+                       p->d_rows, rowptr, colind, val, perm, B, ldb, C, ldc);
+  else
+    hipLaunchKernelGGL((csr_spmm_kernel<TB, G, VEC>), grid, dim3(kThreads), 0, s, t,
+                       p->d_rows, rowptr, colind, val, B, ldb, C, ldc);
 }
 
 template <typename TB, int VEC>
 static void dispatch_G(int G, const LaunchTable &t, int col_tiles,
                        const spmm_csr_plan_s *p, const int *rowptr,
-                       const int *colind, const float *val, const TB *B, int ldb,
-                       float *C, int ldc, hipStream_t s) {
+                       const int *colind, const float *val, const int *perm, const TB *B,
+                       int ldb, float *C, int ldc, hipStream_t s) {
+#define SPMM_LV(g) launch_vec<TB, g, VEC>(t, col_tiles, p, rowptr, colind, val, perm, B, ldb, C, ldc, s)
   switch (G) {
-    case 1: launch_vec<TB, 1, VEC>(t, col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s); break;
-    case 2: launch_vec<TB, 2, VEC>(t, col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s); break;
-    case 4: launch_vec<TB, 4, VEC>(t, col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s); break;
-    case 8: launch_vec<TB, 8, VEC>(t, col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s); break;
-    case 16: launch_vec<TB, 16, VEC>(t, col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s); break;
-    case 32: launch_vec<TB, 32, VEC>(t, col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s); break;
-    default: launch_vec<TB, 64, VEC>(t, col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s); break;
+    case 1: SPMM_LV(1); break;
+    case 2: SPMM_LV(2); break;
+    case 4: SPMM_LV(4); break;
+    case 8: SPMM_LV(8); break;
+    case 16: SPMM_LV(16); break;
+    case 32: SPMM_LV(32); break;
+    default: SPMM_LV(64); break;
   }
+#undef SPMM_LV
 }
 
+// perm != nullptr: entry e's value is val[perm[e]] (transposed structure).
 template <typename TB>
 static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
                        const int *rowptr, const int *colind, const float *val,
                        const TB *B, int ldb, float *C, int ldc,
-                       const spmm_tune_t *tune, void *stream) {
+                       const spmm_tune_t *tune, void *stream, const int *perm = nullptr) {
   SPMM_CHECK_ARG(p != nullptr, "plan is null");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0 && nnz >= 0, "negative size");
   if (p->tile != 0) SPMM_FAIL(SPMM_ERR_MISMATCH, "a B2SR plan was given to a CSR call");
@@ -220,7 +240,7 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
   Geometry g = pick_geometry(p, N, sizeof(TB), B, ldb, C, ldc, tune);
   if (g.G == 0) {
     hipLaunchKernelGGL((csr_spmm_generic_kernel<TB>), dim3(ceil_div(M, kWavesPerWG)),
-                       dim3(kThreads), 0, s, M, N, rowptr, colind, val, B, ldb, C, ldc);
+                       dim3(kThreads), 0, s, M, N, rowptr, colind, val, B, ldb, C, ldc, perm);
     SPMM_LAUNCHED();
     return SPMM_OK;
   }
@@ -228,9 +248,9 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
   build_table(p->counts, p->offs, g.G, g.rows_scale, &t);
   if (t.total_wg == 0) return SPMM_OK;
   if (g.VEC == 2)
-    dispatch_G<TB, 2>(g.G, t, g.col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s);
+    dispatch_G<TB, 2>(g.G, t, g.col_tiles, p, rowptr, colind, val, perm, B, ldb, C, ldc, s);
   else
-    dispatch_G<TB, 1>(g.G, t, g.col_tiles, p, rowptr, colind, val, B, ldb, C, ldc, s);
+    dispatch_G<TB, 1>(g.G, t, g.col_tiles, p, rowptr, colind, val, perm, B, ldb, C, ldc, s);
   SPMM_LAUNCHED();
   return SPMM_OK;
 }
@@ -240,7 +260,7 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
 template <typename TB>
 static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int *rowptr,
                     const int *colind, const float *val, const TB *B, int ldb, float *C,
-                    int ldc, spmm_tune_t *best_out, void *stream) {
+                    int ldc, spmm_tune_t *best_out, void *stream, const int *perm = nullptr) {
   SPMM_CHECK_ARG(p != nullptr, "plan is null");
   hipStream_t s = as_stream(stream);
   const int e16 = 16 / static_cast<int>(sizeof(TB));
@@ -257,11 +277,13 @@ static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int 
       if (!tune_valid(N, e16, G, vec) || G * 4 < std::min(lanes, 64)) continue;  // <= 4 column tiles
       for (int rs = 1; rs <= 8 && st == SPMM_OK; rs <<= 1) {
         spmm_tune_t cand = {G, vec, rs};
-        st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, &cand, stream);
+        st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, &cand, stream,
+                             perm);
         if (st) break;
         if (hipEventRecord(e0, s) != hipSuccess) { st = SPMM_ERR_HIP; break; }
         for (int r = 0; r < 3 && st == SPMM_OK; ++r)
-          st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, &cand, stream);
+          st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, &cand,
+                               stream, perm);
         if (st) break;
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
           st = SPMM_ERR_HIP;
@@ -359,7 +381,128 @@ static int run_dropin(spmm_handle_s *h, int M, int K, int N, int nnz,
 
 }  // namespace spmm
 
+// Plan of C = A^T B: the stable transpose of A's structure (csr2csc order)
+// with the entry permutation, and the row schedule of that transpose. Values
+// are not copied: each call reads val[perm[e]], so A's values may change
+// between calls (learnable edge weights).
+struct spmm_csrt_plan_s {
+  int M = 0, K = 0, nnz = 0;
+  int *trowptr = nullptr, *tcol = nullptr, *perm = nullptr;
+  spmm_csr_plan_s *tplan = nullptr;
+};
+
+namespace spmm {
+
+template <typename TB>
+static int run_t_planned(spmm_csrt_plan_s *p, int M, int K, int N, int nnz, const float *val,
+                         const TB *B, int ldb, float *C, int ldc, const spmm_tune_t *tune,
+                         void *stream) {
+  SPMM_CHECK_ARG(p != nullptr, "plan is null");
+  if (p->M != M || p->K != K || p->nnz != nnz)
+    SPMM_FAIL(SPMM_ERR_MISMATCH, "transposed plan was built for M=" + std::to_string(p->M) +
+                                     " K=" + std::to_string(p->K) +
+                                     " nnz=" + std::to_string(p->nnz));
+  // the transposed problem: K output rows, B has M rows
+  return run_planned<TB>(p->tplan, K, M, N, nnz, p->trowptr, p->tcol, val, B, ldb, C, ldc, tune,
+                         stream, p->perm);
+}
+
+template <typename TB>
+static int run_t_dropin(spmm_handle_s *h, int M, int K, int N, int nnz, const int *rowptr,
+                        const int *colind, const float *val, const TB *B, int ldb, float *C,
+                        int ldc, void *stream) {
+  spmm_csrt_plan_s *p = nullptr;
+  int st = spmm_csr_t_analyze(h, M, K, nnz, rowptr, colind, &p, stream);
+  if (st != SPMM_OK) return st;
+  st = run_t_planned<TB>(p, M, K, N, nnz, val, B, ldb, C, ldc, nullptr, stream);
+  if (hipStreamSynchronize(as_stream(stream)) != hipSuccess && st == SPMM_OK) st = SPMM_ERR_HIP;
+  spmm_csr_t_plan_destroy(p);
+  return st;
+}
+
+}  // namespace spmm
+
 extern "C" {
+
+int spmm_csr_t_analyze(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
+                       const int *colind, spmm_csrt_plan_t *plan, void *stream) {
+  SPMM_CHECK_ARG(h && plan, "null handle or plan");
+  *plan = nullptr;
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && nnz >= 0, "negative size");
+  auto *p = new (std::nothrow) spmm_csrt_plan_s();
+  if (!p) SPMM_FAIL(SPMM_ERR_ALLOC, "host allocation failed");
+  p->M = M;
+  p->K = K;
+  p->nnz = nnz;
+  const size_t e = static_cast<size_t>(nnz > 0 ? nnz : 1);
+  if (hipMalloc(&p->trowptr, sizeof(int) * (static_cast<size_t>(K) + 1)) != hipSuccess ||
+      hipMalloc(&p->tcol, sizeof(int) * e) != hipSuccess ||
+      hipMalloc(&p->perm, sizeof(int) * e) != hipSuccess) {
+    spmm_csr_t_plan_destroy(p);
+    SPMM_FAIL(SPMM_ERR_ALLOC, "hipMalloc of the transposed structure failed");
+  }
+  int st = csr_transpose(h, M, K, nnz, rowptr, colind, nullptr, p->trowptr, p->tcol, nullptr,
+                         p->perm, stream);
+  if (st == SPMM_OK) st = spmm::analyze(h, K, nnz, p->trowptr, 0, &p->tplan, stream);
+  if (st != SPMM_OK) {
+    const std::string msg = spmm_last_error();
+    spmm_csr_t_plan_destroy(p);
+    spmm::set_error(msg);
+    return st;
+  }
+  *plan = p;
+  return SPMM_OK;
+}
+
+int spmm_csr_t_plan_destroy(spmm_csrt_plan_t plan) {
+  if (!plan) return SPMM_OK;
+  if (plan->trowptr) (void)hipFree(plan->trowptr);
+  if (plan->tcol) (void)hipFree(plan->tcol);
+  if (plan->perm) (void)hipFree(plan->perm);
+  spmm_csr_plan_destroy(plan->tplan);
+  delete plan;
+  return SPMM_OK;
+}
+
+int spmm_csr_t_autotune(spmm_csrt_plan_t plan, int M, int K, int N, int nnz, const float *val,
+                        const void *B, int bf16, int ldb, float *C, int ldc, spmm_tune_t *best,
+                        void *stream) {
+  SPMM_CHECK_ARG(plan != nullptr, "plan is null");
+  if (plan->M != M || plan->K != K || plan->nnz != nnz)
+    SPMM_FAIL(SPMM_ERR_MISMATCH, "transposed plan was built for other sizes");
+  if (bf16)
+    return spmm::autotune<uint16_t>(plan->tplan, K, M, N, nnz, plan->trowptr, plan->tcol, val,
+                                    static_cast<const uint16_t *>(B), ldb, C, ldc, best, stream,
+                                    plan->perm);
+  return spmm::autotune<float>(plan->tplan, K, M, N, nnz, plan->trowptr, plan->tcol, val,
+                               static_cast<const float *>(B), ldb, C, ldc, best, stream,
+                               plan->perm);
+}
+
+int spmm_csr_t_f32_planned(spmm_csrt_plan_t plan, int M, int K, int N, int nnz,
+                           const float *val, const float *B, int ldb, float *C, int ldc,
+                           const spmm_tune_t *tune, void *stream) {
+  return spmm::run_t_planned<float>(plan, M, K, N, nnz, val, B, ldb, C, ldc, tune, stream);
+}
+
+int spmm_csr_t_bf16_planned(spmm_csrt_plan_t plan, int M, int K, int N, int nnz,
+                            const float *val, const uint16_t *B, int ldb, float *C, int ldc,
+                            const spmm_tune_t *tune, void *stream) {
+  return spmm::run_t_planned<uint16_t>(plan, M, K, N, nnz, val, B, ldb, C, ldc, tune, stream);
+}
+
+int spmm_csr_t_f32(spmm_handle_t h, int M, int K, int N, int nnz, const int *rowptr,
+                   const int *colind, const float *val, const float *B, int ldb, float *C,
+                   int ldc, void *stream) {
+  return spmm::run_t_dropin<float>(h, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, stream);
+}
+
+int spmm_csr_t_bf16(spmm_handle_t h, int M, int K, int N, int nnz, const int *rowptr,
+                    const int *colind, const float *val, const uint16_t *B, int ldb, float *C,
+                    int ldc, void *stream) {
+  return spmm::run_t_dropin<uint16_t>(h, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc,
+                                      stream);
+}
 
 int spmm_csr_analyze(spmm_handle_t h, int M, int nnz, const int *rowptr,
                      spmm_csr_plan_t *plan, void *stream) {

@@ -25,7 +25,8 @@ def np_u32(t):
     return t.view(torch.int32).cpu().numpy().view(np.uint32)
 
 
-@pytest.mark.parametrize("M,N,Kw", [(1, 1, 1), (32, 32, 1), (64, 128, 4), (100, 70, 7), (256, 512, 33)])
+@pytest.mark.parametrize("M,N,Kw", [(1, 1, 1), (32, 32, 1), (64, 128, 4), (100, 70, 7), (256, 512, 33),
+                                    (128, 128, 4), (1024, 1024, 32), (4096, 4096, 128)])
 def test_bmm(dev, M, N, Kw):
     r = np.random.default_rng(M * N + Kw)
     A = r.integers(0, 2 ** 32, (Kw, M), dtype=np.uint64).astype(np.uint32)
@@ -201,4 +202,57 @@ def test_sddmm_planned_mismatch(handle, dev):
     X = torch.ones(100, 16, device=dev)
     with pytest.raises(SpmmError) as ei:
         api.sddmm_csr(handle, rp, ci, X, X, plan=plan)
+    assert ei.value.status == 5
+
+
+@pytest.mark.parametrize("M,K,dens,nl,ll,N", [
+    (2000, 1500, 0.004, 3, 1500, 128),
+    (777, 3000, 0.003, 2, 3000, 64),
+    (300, 200, 0.05, 2, 200, 7),        # generic kernel
+    (1, 5, 1.0, 0, 0, 4),
+])
+def test_spmm_csr_t(handle, dev, M, K, dens, nl, ll, N):
+    """C = A^T B straight from A's CSR: equals the CSR SpMM on the explicit
+    transpose bit for bit (same transposed entry order, same values), is
+    within 1e-5 of FP64, and follows new values without a new plan."""
+    g = graphs.random_csr(M, K, dens, seed=M + K + N, long_rows=nl, long_len=ll)
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    B = torch.from_numpy(graphs.normal((M, N), 3, N)).to(dev)
+    plan = api.CsrTPlan(handle, rp, ci, K)
+    C1 = api.spmm_csr_t(handle, rp, ci, v, B, K, plan=plan)
+    C2 = api.spmm_csr_t(handle, rp, ci, v, B, K)  # drop-in form
+    trp, tci, tval, _ = api.csr_transpose(handle, rp, ci, v, K)
+    tplan = api.CsrPlan(handle, trp, g.nnz)
+    C3 = api.spmm_csr(handle, trp, tci, tval, B, K=M, plan=tplan)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2) and torch.equal(C1, C3)
+    cp, ri, cv, _ = oracle.csr2csc(g.rowptr, g.colind, g.val, K)
+    ref = oracle.csr_spmm_f64(cp, ri, cv, B.cpu().numpy())
+    n = np.linalg.norm(ref)
+    assert np.linalg.norm(C1.cpu().numpy() - ref) / (n if n > 0 else 1.0) <= 1e-5
+    v2 = v * 2.0 - 0.5  # learnable values change; the plan stays
+    C4 = api.spmm_csr_t(handle, rp, ci, v2, B, K, plan=plan)
+    ref2 = oracle.csr_spmm_f64(cp, ri, (cv * 2.0 - 0.5).astype(np.float32), B.cpu().numpy())
+    n2 = np.linalg.norm(ref2)
+    assert np.linalg.norm(C4.cpu().numpy() - ref2) / (n2 if n2 > 0 else 1.0) <= 1e-5
+    if N % 8 == 0:
+        Bh = api.to_bf16_bits(B)
+        Ch = api.spmm_csr_t(handle, rp, ci, v, Bh, K, plan=plan)
+        C3h = api.spmm_csr(handle, trp, tci, tval, Bh, K=M, plan=tplan)
+        assert torch.equal(Ch, C3h)
+        G, vec, rs = plan.autotune(v, Bh, torch.empty_like(Ch))
+        assert G in (1, 2, 4, 8, 16, 32, 64) and vec in (1, 2) and rs in (1, 2, 4, 8)
+        Ct = api.spmm_csr_t(handle, rp, ci, v, Bh, K, plan=plan).cpu().numpy()  # tuned entry
+        Bq = oracle.bf16_to_f32(Bh.view(torch.int16).cpu().numpy().view(np.uint16))
+        refq = oracle.csr_spmm_f64(cp, ri, cv, Bq)
+        nq = np.linalg.norm(refq)
+        assert np.linalg.norm(Ct - refq) / (nq if nq > 0 else 1.0) <= 1e-5
+
+
+def test_spmm_csr_t_plan_mismatch(handle, dev):
+    g = graphs.random_csr(100, 80, 0.05, seed=5)
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    plan = api.CsrTPlan(handle, rp, ci, 80)
+    with pytest.raises(SpmmError) as ei:
+        api.spmm_csr_t(handle, rp, ci, v, torch.ones(100, 8, device=dev), 81, plan=plan)
     assert ei.value.status == 5
