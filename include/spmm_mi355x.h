@@ -258,6 +258,24 @@ int sddmm_csr_bf16_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
                            const uint16_t *dY, int lddy, const uint16_t *X,
                            int ldx, float *dval, void *stream);
 
+/* ---------------------------------------------------- Matrix Market
+ * The reference harness's ingest (test_spmm_full.cu:99-125): readMtx
+ * (backend/readMtx.hpp:351-422, mmio.hpp:128-226) then coo2csr
+ * (readMtx.hpp:246-304), on the host. As there: 1-based indices become
+ * 0-based, pattern entries read as 1.0, integer/real values as float (the
+ * rounding of fscanf "%f"), symmetric files are NOT expanded and duplicate
+ * pairs NOT merged, entries ordered by (row, col) (duplicate pairs keep file
+ * order). Coordinate files with pattern / integer / real values only (the
+ * reference reads no values for complex ones): SPMM_ERR_UNSUPPORTED
+ * otherwise; a truncated file or an index out of range is an error.
+ * spmm_mtx_read_info gives the sizes (kind = type + 4 * symmetry, type 0
+ * pattern / 1 integer / 2 real, symmetry 0 general / 1 symmetric / 2 skew)
+ * so the caller can allocate rowptr[M+1], colind[nnz] and val[nnz] (val
+ * may be NULL) for spmm_mtx_read_csr, which checks M, K, nnz again. */
+int spmm_mtx_read_info(const char *path, int *M, int *K, int *nnz, int *kind);
+int spmm_mtx_read_csr(const char *path, int M, int K, int nnz, int *rowptr,
+                      int *colind, float *val);
+
 /* ------------------------------------------------------------- batched
  * Many CSR graphs run as one block-diagonal problem. Graph g has rows
  * [row_off[g], row_off[g+1]) and entries [nnz_off[g], nnz_off[g+1]) of the

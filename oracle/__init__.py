@@ -86,6 +86,10 @@ def ref():
         R.ref_csr2csc.restype = None
         R.ref_coo2csr.argtypes = [_i, _i, _i, _i32p, _i32p, _f32p, _i32p, _i32p, _f32p]
         R.ref_coo2csr.restype = None
+        R.ref_read_mtx_info.argtypes = [C.c_char_p] + [C.POINTER(C.c_int)] * 3
+        R.ref_read_mtx_info.restype = C.c_int
+        R.ref_read_mtx_csr.argtypes = [C.c_char_p, _i32p, _i32p, _f32p]
+        R.ref_read_mtx_csr.restype = C.c_int
         _ref = R
     return _ref
 

@@ -12,6 +12,24 @@ void ref_csr2csc(int M, int K, const int *rowptr, const int *colind, const float
   csr2csc<float>(colptr, rowind, cval, rowptr, colind, val, M, K);
 }
 
+// backend/readMtx.hpp:351-422 then :246-304, as test_spmm_full.cu:99-113
+// runs them (directed = 0, no mtxinfo, no dat_name): sizes first...
+int ref_read_mtx_info(const char *path, int *M, int *K, int *nnz) {
+  std::vector<int> r, c;
+  std::vector<float> v;
+  return readMtx<float>(path, &r, &c, &v, M, K, nnz, 0, false, nullptr);
+}
+
+// ...then the CSR arrays (caller-allocated from the sizes above).
+int ref_read_mtx_csr(const char *path, int *rowptr, int *colind, float *val) {
+  std::vector<int> r, c;
+  std::vector<float> v;
+  int M = 0, K = 0, nnz = 0;
+  const int rc = readMtx<float>(path, &r, &c, &v, &M, &K, &nnz, 0, false, nullptr);
+  coo2csr<float>(rowptr, colind, val, r, c, v, M, K);
+  return rc;
+}
+
 // backend/readMtx.hpp:246-304 (sorts the COO tuples by (row, col) first)
 void ref_coo2csr(int M, int K, int nvals, const int *rows, const int *cols, const float *vals,
                  int *rowptr, int *colind, float *val) {

@@ -75,6 +75,8 @@ _SIGS = {
     "spmm_csr_t_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_csr_t_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_csr_t_bf16": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
+    "spmm_mtx_read_info": [C.c_char_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
+    "spmm_mtx_read_csr": [C.c_char_p, _i, _i, _i, _vp, _vp, _vp],
     "sddmm_csr_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "sddmm_csr_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_batch_plan_create": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, C.POINTER(_vp), _vp],

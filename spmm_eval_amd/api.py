@@ -300,6 +300,23 @@ def sddmm_csr(h: Handle, rowptr, colind, dY, X, out=None, plan: CsrPlan | None =
     return out
 
 
+def read_mtx(path: str):
+    """Matrix Market file -> host CSR (numpy), the reference harness's
+    ingest (spmm_mtx_read_info / spmm_mtx_read_csr of the C-ABI). Returns
+    (rowptr, colind, val, K, kind)."""
+    import numpy as np
+
+    M, K, nnz, kind = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    p = str(path).encode()
+    call("spmm_mtx_read_info", p, C.byref(M), C.byref(K), C.byref(nnz), C.byref(kind))
+    rowptr = np.empty(M.value + 1, np.int32)
+    colind = np.empty(max(nnz.value, 1), np.int32)[: nnz.value]
+    val = np.empty(max(nnz.value, 1), np.float32)[: nnz.value]
+    call("spmm_mtx_read_csr", p, M.value, K.value, nnz.value, rowptr.ctypes.data,
+         colind.ctypes.data, val.ctypes.data)
+    return rowptr, colind, val, K.value, kind.value
+
+
 def to_bf16_bits(x: torch.Tensor) -> torch.Tensor:
     """FP32 -> BF16 (round to nearest even) as a uint16 view."""
     return x.to(torch.bfloat16).view(torch.uint16)
