@@ -82,26 +82,32 @@ def profiled_traffic(profile: str, kernel: str):
 
 # ----------------------------------------------------------------- timing
 def time_launches(fn, steps, warmup):
-    """Runs fn() warmup times, then `steps` times between two events on the
-    current stream; also brackets every launch with its own event pair.
-    Returns (ms per step over the loop, mean ms per launch)."""
+    """Runs fn() warmup times, then `steps` times back to back between two
+    events on the current stream (the timed region: ms per step); then a
+    second pass of `steps` launches, each between its own event pair on the
+    same stream (mean kernel duration, kept out of the timed region because
+    the extra event records between launches cost a few microseconds each).
+    Returns (ms per step, mean ms per launch)."""
     import torch
 
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
-    pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record(s)
+    for _ in range(steps):
+        fn()
+    t1.record(s)
+    torch.cuda.synchronize()
+    total = t0.elapsed_time(t1) / steps
+    pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     for a, b in pairs:
         a.record(s)
         fn()
         b.record(s)
-    t1.record(s)
     torch.cuda.synchronize()
-    total = t0.elapsed_time(t1) / steps
     per = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
     return total, per
 

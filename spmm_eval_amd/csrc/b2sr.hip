@@ -449,8 +449,9 @@ b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
 // rows of a power-law graph) -> one subgroup per block row, streamed; up to
 // one wave chunk (256 tiles) -> one wave per block row, rows in turn, each
 // row's entries dealt round-robin to the subgroups; wider -> one workgroup
-// per block row, one wave per row (its subgroups take the tile chunks in
-// turn, decoding only their row's bits).
+// per row of the block row (its 16 subgroups take the tile chunks in turn,
+// decoding only that row's bits; the widest block rows set the kernel's
+// critical path, so they get the most lanes).
 template <int T> struct TileWord;
 template <> struct TileWord<4> { using type = uint32_t; };
 template <> struct TileWord<8> { using type = uint64_t; };
@@ -735,25 +736,43 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
                         br * T, M, col0);
     return;
   }
-  // class C: one workgroup per block row, wave w takes rows w, w + 4, ...
+  // class C: one workgroup per (block row, row): its 4S subgroups take the
+  // tile chunks in turn decoding only that row's bits, a butterfly sums each
+  // wave, and the waves' sums are added in a fixed order through LDS (the
+  // list area, free once every subgroup is done).
+  const int br = brows[sg.list_off + local_wg / T];
+  const int r = local_wg % T;
+  const int row = br * T + r;
+  if (row >= M) return;  // whole workgroup
   int *list = lds + gsub * CAP;
-  const int br = brows[sg.list_off + local_wg];
   const int t0 = brp[br], t1 = brp[br + 1];
   const auto *words = reinterpret_cast<const typename Word<T>::type *>(tw);
   constexpr int RC = Chunk<T, G>::kTiles;
-  for (int r = wave; r < T; r += kB2Waves) {
-    const int row = br * T + r;
-    if (row >= M) break;  // whole wave
-    float acc[EPL];
+  float acc[EPL];
 #pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    b2sr_range<T, TB, G>(t0 + sub * RC, t1, S * RC, r, lane_sub, bci, words, list, ld, acc);
+  for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+  b2sr_range<T, TB, G>(t0 + gsub * RC, t1, kB2Waves * S * RC, r, lane_sub, bci, words, list, ld,
+                       acc);
 #pragma unroll
-    for (int m = G; m < kWave; m <<= 1) {
+  for (int m = G; m < kWave; m <<= 1) {
 #pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
+    for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
+  }
+  static_assert((kB2Waves - 1) * CW <= kB2Waves * S * CAP, "reduction must fit the list area");
+  float *red = reinterpret_cast<float *>(lds);
+  __syncthreads();
+  if (wave > 0 && sub == 0) {
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) red[(wave - 1) * CW + lane_sub * EPL + q] = acc[q];
+  }
+  __syncthreads();
+  if (wave == 0 && sub == 0) {
+#pragma unroll
+    for (int w = 0; w < kB2Waves - 1; ++w) {
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] += red[w * CW + lane_sub * EPL + q];
     }
-    if (sub == 0) store_row<EPL>(C, ldc, row, col0, acc);
+    store_row<EPL>(C, ldc, row, col0, acc);
   }
 }
 
@@ -1053,8 +1072,8 @@ static void launch_brow(const spmm_csr_plan_s *p, int M, int K, const int *brp, 
   LaunchTable t;
   // bins wholly below one subgroup chunk of tiles: a subgroup per block row
   // (several only below 8 tiles); wholly below one wave chunk: a wave (the
-  // wave path decodes exactly one chunk); the rest: a workgroup
-  build_table_ex(p->counts, p->offs, S, TC, BrowChunk<G>::kWaveTiles, 8, 16, 1, &t);
+  // wave path decodes exactly one chunk); the rest: a workgroup per row
+  build_table_ex(p->counts, p->offs, S, TC, BrowChunk<G>::kWaveTiles, 8, 16, 1, &t, 1, T);
   if (t.total_wg == 0) return;
   const auto *tw = static_cast<const typename TileWord<T>::type *>(val);
   // B rows are read through a buffer resource when B spans < 2 GiB

@@ -483,11 +483,11 @@ struct TuneEntry {
 // Launch table from bin counts: bins whose degrees all lie below a_limit run
 // one subgroup per row (max(1, r_num / bound) rows per subgroup, r_empty for
 // empty rows, times rows_scale), bins at or above c_limit one workgroup per
-// row, the rest one wave per row (b_items waves per row when a unit holds
-// several rows, as a B2SR block row does).
+// row, the rest one wave per row (b_items waves / c_items workgroups per
+// unit when a unit holds several rows, as a B2SR block row does).
 void build_table_ex(const int *counts, const int *offs, int S, long long a_limit,
                     long long c_limit, int r_num, int r_empty, int rows_scale, LaunchTable *t,
-                    int b_items = 1);
+                    int b_items = 1, int c_items = 1);
 // Launch table for a CSR plan's bin counts and lane geometry G.
 void build_table(const int *counts, const int *offs, int G, int rows_scale, LaunchTable *t);
 

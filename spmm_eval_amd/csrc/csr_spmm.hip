@@ -148,7 +148,7 @@ static Geometry pick_geometry(const spmm_csr_plan_s *p, int N, int elem_bytes, c
 
 void build_table_ex(const int *counts, const int *offs, int S, long long a_limit,
                     long long c_limit, int r_num, int r_empty, int rows_scale, LaunchTable *t,
-                    int b_items) {
+                    int b_items, int c_items) {
   t->nseg = 0;
   int wg = 0;
   for (int b = kBins - 1; b >= 0; --b) {
@@ -169,7 +169,7 @@ void build_table_ex(const int *counts, const int *offs, int S, long long a_limit
     } else if (lo >= c_limit) {
       s.kind = kSegC;
       s.rows_per_sub = 1;
-      wg += cnt;
+      wg += cnt * c_items;
     } else {
       s.kind = kSegB;
       s.rows_per_sub = 1;
