@@ -71,8 +71,12 @@ int spmm_version(void);
  * C row-major instead of the column-major layout the harness converts to.
  *
  * spmm_csr_analyze builds the row schedule (rows binned by degree, longest
- * bins first). It launches two small kernels and SYNCHRONISES the stream
- * once to read the bin counts. The plan keeps no pointer to rowptr. */
+ * bins first) and keeps each row's entry span (rowptr[r], rowptr[r+1])
+ * beside its id, so the planned calls read the spans from the plan, not
+ * from rowptr: a plan is valid only for the structure it was built from
+ * (same rowptr contents), as a cuSPARSE analysis is. It launches three small
+ * kernels and SYNCHRONISES the stream once to read the bin counts. The
+ * plan keeps no pointer to rowptr. */
 int spmm_csr_analyze(spmm_handle_t h, int M, int nnz, const int *rowptr,
                      spmm_csr_plan_t *plan, void *stream);
 int spmm_csr_plan_destroy(spmm_csr_plan_t plan);
@@ -140,11 +144,12 @@ int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N,
                    const void *bsrVal, const uint16_t *B, int ldb, float *C,
                    int ldc, void *stream);
 
-/* Planned FBF SpMM. spmm_b2sr_analyze bins the M rows by the tile count of
- * their block row (SYNCHRONISES once); the planned call then runs short
- * rows one subgroup each, medium rows one wave each and hub rows one
- * workgroup each in a single launch (no list state, asynchronous). The
- * plan is a spmm_csr_plan_t; it keeps no pointer to bsrRowPtr. */
+/* Planned FBF SpMM. spmm_b2sr_analyze bins the block rows (T = 4, 8) or the
+ * rows (T = 16, 32) by tile count and keeps each unit's tile span beside
+ * it (SYNCHRONISES once); the planned call runs every class in a single
+ * launch (no list state, asynchronous) and reads the spans from the plan,
+ * so a plan is valid only for the bsrRowPtr it was built from. The plan is
+ * a spmm_csr_plan_t; it keeps no pointer to bsrRowPtr. */
 int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr,
                       spmm_csr_plan_t *plan, void *stream);
 int spmm_b2sr_f32_planned(spmm_csr_plan_t plan, int T, int M, int K, int N,

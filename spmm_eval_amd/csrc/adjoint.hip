@@ -153,10 +153,10 @@ static void launch_sddmm(int M, int nnz, const int *rowptr, const int *colind, c
 
 template <typename TB, int G>
 __global__ void __launch_bounds__(kAdjThreads)
-sddmm_row_kernel(LaunchTable tab, const int *__restrict__ rows, const int *__restrict__ rowptr,
+sddmm_row_kernel(LaunchTable tab, const int *__restrict__ rows, const int2 *__restrict__ span,
                  const int *__restrict__ colind, const TB *__restrict__ dY, int lddy,
                  const TB *__restrict__ X, int ldx, float *__restrict__ out, int ntile) {
-  sddmm_row_wg<TB, G>(tab, blockIdx.x, rows, rowptr, colind, dY, lddy, X, ldx, out, ntile);
+  sddmm_row_wg<TB, G>(tab, blockIdx.x, rows, span, colind, dY, lddy, X, ldx, out, ntile);
 }
 
 template <typename TB, int G>
@@ -167,7 +167,7 @@ static void launch_sddmm_rows(const spmm_csr_plan_s *p, const int *rowptr, const
   build_table(p->counts, p->offs, G, 1, &t);
   if (t.total_wg == 0) return;
   hipLaunchKernelGGL((sddmm_row_kernel<TB, G>), dim3(t.total_wg), dim3(kAdjThreads), 0, s, t,
-                     p->d_rows, rowptr, colind, dY, lddy, X, ldx, out, ntile);
+                     p->d_rows, p->d_span, colind, dY, lddy, X, ldx, out, ntile);
 }
 
 // plan == nullptr: entry-chunked (no schedule needed); else rows of the plan.

@@ -336,7 +336,7 @@ __device__ __forceinline__ void b2sr_range(int t, int t1, int stride, int r, int
 template <int T, typename TB, int G>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
-                    const int *__restrict__ brp, const int *__restrict__ bci,
+                    const int2 *__restrict__ span, const int *__restrict__ bci,
                     const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
                     int ldb, float *__restrict__ C, int ldc) {
   constexpr int EPL = Vec16<TB>::kElems;
@@ -362,19 +362,21 @@ b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
     const int subs_per_wg = kB2Waves * S;
     const int end = sg.list_off + sg.count;
     int pos = sg.list_off + local_wg * subs_per_wg * R + gsub;
-    int row = pos < end ? rows[pos] : -1;
-    int t0 = 0, t1 = 0;
-    if (row >= 0) {
-      t0 = brp[row / T];
-      t1 = brp[row / T + 1];
+    int row = -1, t0 = 0, t1 = 0;
+    if (pos < end) {  // the row and its block row's tile span arrive together
+      row = rows[pos];
+      const int2 sp = span[pos];
+      t0 = sp.x;
+      t1 = sp.y;
     }
     for (int k = 0; k < R && row >= 0; ++k) {
       const int pos_n = pos + subs_per_wg;
-      const int row_n = (k + 1 < R && pos_n < end) ? rows[pos_n] : -1;
-      int t0n = 0, t1n = 0;
-      if (row_n >= 0) {
-        t0n = brp[row_n / T];
-        t1n = brp[row_n / T + 1];
+      int row_n = -1, t0n = 0, t1n = 0;
+      if (k + 1 < R && pos_n < end) {
+        row_n = rows[pos_n];
+        const int2 sp = span[pos_n];
+        t0n = sp.x;
+        t1n = sp.y;
       }
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
@@ -394,7 +396,8 @@ b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
   const int pos = wave_mode ? sg.list_off + local_wg * kB2Waves + wave : sg.list_off + local_wg;
   if (wave_mode && pos >= sg.list_off + sg.count) return;  // whole wave (no barrier in this mode)
   const int row = rows[pos];
-  const int t0 = brp[row / T], t1 = brp[row / T + 1];
+  const int2 sp = span[pos];
+  const int t0 = sp.x, t1 = sp.y;
   const int part = wave_mode ? sub : gsub;
   const int parts = wave_mode ? S : kB2Waves * S;
 #pragma unroll
@@ -682,7 +685,8 @@ __device__ __forceinline__ void brow_wave(int t, int t1, int lane, const int *__
 template <int T, typename TB, int G, bool kBuf>
 __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *lds,
                                         const int *__restrict__ brows, int M,
-                                        const int *__restrict__ brp, const int *__restrict__ bci,
+                                        const int2 *__restrict__ span,
+                                        const int *__restrict__ bci,
                                         const typename TileWord<T>::type *__restrict__ tw,
                                         const TB *__restrict__ B, int ldb, int b_bytes,
                                         float *__restrict__ C, int ldc) {
@@ -706,24 +710,23 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
     const int subs_per_wg = kB2Waves * S;
     const int end = sg.list_off + sg.count;
     int pos = sg.list_off + local_wg * subs_per_wg * R + gsub;
-    int br = pos < end ? brows[pos] : -1;
-    int t0 = 0, t1 = 0;
-    if (br >= 0) {
-      t0 = brp[br];
-      t1 = brp[br + 1];
+    int br = -1;
+    int2 sp = make_int2(0, 0);
+    if (pos < end) {  // the block row and its tile span arrive together
+      br = brows[pos];
+      sp = span[pos];
     }
     for (int k = 0; k < R && br >= 0; ++k) {
       const int pos_n = pos + subs_per_wg;
-      const int br_n = (k + 1 < R && pos_n < end) ? brows[pos_n] : -1;
-      int t0n = 0, t1n = 0;
-      if (br_n >= 0) {
-        t0n = brp[br_n];
-        t1n = brp[br_n + 1];
+      int br_n = -1;
+      int2 sp_n = make_int2(0, 0);
+      if (k + 1 < R && pos_n < end) {
+        br_n = brows[pos_n];
+        sp_n = span[pos_n];
       }
-      brow_stream<T, TB, G>(t0, t1, lane_sub, bci, tw, list, ld, C, ldc, br * T, M, col0);
+      brow_stream<T, TB, G>(sp.x, sp.y, lane_sub, bci, tw, list, ld, C, ldc, br * T, M, col0);
       br = br_n;
-      t0 = t0n;
-      t1 = t1n;
+      sp = sp_n;
       pos = pos_n;
     }
     return;
@@ -732,8 +735,9 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
     const int pos = sg.list_off + local_wg * kB2Waves + wave;
     if (pos >= sg.list_off + sg.count) return;  // whole wave
     const int br = brows[pos];
-    brow_wave<T, TB, G>(brp[br], brp[br + 1], lane, bci, tw, lds + wave * S * CAP, ld, C, ldc,
-                        br * T, M, col0);
+    const int2 sp = span[pos];
+    brow_wave<T, TB, G>(sp.x, sp.y, lane, bci, tw, lds + wave * S * CAP, ld, C, ldc, br * T, M,
+                        col0);
     return;
   }
   // class C: one workgroup per (block row, row): its 4S subgroups take the
@@ -741,11 +745,12 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
   // wave, and the waves' sums are added in a fixed order through LDS (the
   // list area, free once every subgroup is done).
   const int br = brows[sg.list_off + local_wg / T];
+  const int2 sp = span[sg.list_off + local_wg / T];
   const int r = local_wg % T;
   const int row = br * T + r;
   if (row >= M) return;  // whole workgroup
   int *list = lds + gsub * CAP;
-  const int t0 = brp[br], t1 = brp[br + 1];
+  const int t0 = sp.x, t1 = sp.y;
   const auto *words = reinterpret_cast<const typename Word<T>::type *>(tw);
   constexpr int RC = Chunk<T, G>::kTiles;
   float acc[EPL];
@@ -779,14 +784,14 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
 template <int T, typename TB, int G, bool kBuf>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_brow_kernel(LaunchTable tab, const int *__restrict__ brows, int M,
-                 const int *__restrict__ brp, const int *__restrict__ bci,
+                 const int2 *__restrict__ span, const int *__restrict__ bci,
                  const typename TileWord<T>::type *__restrict__ tw, const TB *__restrict__ B,
                  int ldb, int b_bytes, float *__restrict__ C, int ldc) {
   constexpr int S = kWave / G;
   constexpr int CAP = BrowChunk<G>::kCap;
   static_assert(Chunk<T, G>::kCap == CAP, "all paths share the list area");
   __shared__ int lds[kB2Waves * S][CAP];
-  brow_wg<T, TB, G, kBuf>(tab, blockIdx.x, &lds[0][0], brows, M, brp, bci, tw, B, ldb, b_bytes,
+  brow_wg<T, TB, G, kBuf>(tab, blockIdx.x, &lds[0][0], brows, M, span, bci, tw, B, ldb, b_bytes,
                           C, ldc);
 }
 
@@ -1044,7 +1049,7 @@ static void launch_planned(const spmm_csr_plan_s *p, int M, const int *brp, cons
   build_table_ex(p->counts, p->offs, S, 2ll * TC, 2ll * S * TC, TC, 16, 1, &t);
   if (t.total_wg == 0) return;
   hipLaunchKernelGGL((b2sr_planned_kernel<T, TB, G>), dim3(t.total_wg, tiles), dim3(kB2Threads), 0,
-                     s, t, p->d_rows, M, brp, bci,
+                     s, t, p->d_rows, M, p->d_span, bci,
                      static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc);
 }
 
@@ -1080,11 +1085,12 @@ static void launch_brow(const spmm_csr_plan_s *p, int M, int K, const int *brp, 
   const int b_bytes = buffer_span(K, ldb, sizeof(TB));
   if (b_bytes)
     hipLaunchKernelGGL((b2sr_brow_kernel<T, TB, G, true>), dim3(t.total_wg, tiles),
-                       dim3(kB2Threads), 0, s, t, p->d_rows, M, brp, bci, tw, B, ldb, b_bytes, C,
-                       ldc);
+                       dim3(kB2Threads), 0, s, t, p->d_rows, M, p->d_span, bci, tw, B, ldb,
+                       b_bytes, C, ldc);
   else
     hipLaunchKernelGGL((b2sr_brow_kernel<T, TB, G, false>), dim3(t.total_wg, tiles),
-                       dim3(kB2Threads), 0, s, t, p->d_rows, M, brp, bci, tw, B, ldb, 0, C, ldc);
+                       dim3(kB2Threads), 0, s, t, p->d_rows, M, p->d_span, bci, tw, B, ldb, 0, C,
+                       ldc);
 }
 
 template <int T, typename TB>

@@ -56,6 +56,7 @@ struct FusedArgs {
   int sddmm_rows;             // SDDMM on the forward plan's rows (1) or entry-chunked (0)
   int M, N, nnz, sddmm_ntile;
   const int *rows_fwd, *rows_bwd;
+  const int2 *span_fwd, *span_bwd;
   const int *rowptr, *colind, *trowptr, *tcol;
   const float *val, *tval;
   const void *X, *dY;
@@ -70,20 +71,20 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
   const TB *X = static_cast<const TB *>(a.X);
   const TB *dY = static_cast<const TB *>(a.dY);
   if (item < a.n_fwd) {
-    csr_wg<TB, G, VEC>(a.fwd, item / a.ctiles, item % a.ctiles, a.rows_fwd, a.rowptr, a.colind,
-                       a.val, X, a.N, a.Y, a.N);
+    csr_wg<TB, G, VEC>(a.fwd, item / a.ctiles, item % a.ctiles, a.rows_fwd, a.span_fwd,
+                       a.colind, a.val, X, a.N, a.Y, a.N);
     return;
   }
   item -= a.n_fwd;
   if (item < a.n_bwd) {
-    csr_wg<TB, G, VEC>(a.bwd, item / a.ctiles, item % a.ctiles, a.rows_bwd, a.trowptr, a.tcol,
-                       a.tval, dY, a.N, a.dX, a.N);
+    csr_wg<TB, G, VEC>(a.bwd, item / a.ctiles, item % a.ctiles, a.rows_bwd, a.span_bwd,
+                       a.tcol, a.tval, dY, a.N, a.dX, a.N);
     return;
   }
   item -= a.n_bwd;
   if (kSddRows)
-    sddmm_row_wg<TB, GS>(a.sdd, item, a.rows_fwd, a.rowptr, a.colind, dY, a.N, X, a.N, a.dval,
-                         a.sddmm_ntile);
+    sddmm_row_wg<TB, GS>(a.sdd, item, a.rows_fwd, a.span_fwd, a.colind, dY, a.N, X, a.N,
+                         a.dval, a.sddmm_ntile);
   else
     sddmm_wg<TB, GS>(item, a.M, a.nnz, a.rowptr, a.colind, dY, a.N, X, a.N, a.dval,
                      a.sddmm_ntile);
@@ -193,6 +194,8 @@ static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, in
   a.nnz = p->nnz;
   a.rows_fwd = p->fwd->d_rows;
   a.rows_bwd = p->bwd->d_rows;
+  a.span_fwd = p->fwd->d_span;
+  a.span_bwd = p->bwd->d_span;
   a.rowptr = p->rowptr;
   a.colind = p->gcol;
   a.trowptr = p->trowptr;

@@ -139,7 +139,7 @@ __device__ __forceinline__ Seg find_seg(const LaunchTable &tab, int wg) {
 template <typename TB, int G, int VEC, bool kPerm = false>
 __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile,
                                        const int *__restrict__ rows,
-                                       const int *__restrict__ rowptr,
+                                       const int2 *__restrict__ span,
                                        const int *__restrict__ colind,
                                        const float *__restrict__ val,
                                        const TB *__restrict__ B, int ldb,
@@ -166,58 +166,55 @@ __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile
     const int base = sg.list_off + local_wg * subs_per_wg * R;
     const int end = sg.list_off + sg.count;
     const int my_sub = wave * S + sub;
-    // Two-stage software pipeline over the subgroup's R rows: the next row's
-    // row pointers and the id of the row after it are requested before the
-    // current row's gathers, so only colind -> gather stays on the critical
-    // path of each row.
+    // The plan keeps each row's entry span beside its id, so a row's id and
+    // span arrive together; the next row's are requested before this row's
+    // gathers, leaving colind -> gather on the critical path of each row.
     int pos = base + my_sub;
-    int row = pos < end ? rows[pos] : -1;
-    int e0 = 0, e1 = 0;
-    if (row >= 0) {
-      e0 = rowptr[row];
-      e1 = rowptr[row + 1];
+    int row = -1;
+    int2 sp = make_int2(0, 0);
+    if (pos < end) {
+      row = rows[pos];
+      sp = span[pos];
     }
-    int row_n = (R > 1 && pos + subs_per_wg < end) ? rows[pos + subs_per_wg] : -1;
     for (int k = 0; k < R && row >= 0; ++k) {
-      int e0n = 0, e1n = 0;
-      if (row_n >= 0) {
-        e0n = rowptr[row_n];
-        e1n = rowptr[row_n + 1];
+      const int pos_n = pos + subs_per_wg;
+      int row_n = -1;
+      int2 sp_n = make_int2(0, 0);
+      if (k + 1 < R && pos_n < end) {
+        row_n = rows[pos_n];
+        sp_n = span[pos_n];
       }
-      const int pos_nn = pos + 2 * subs_per_wg;
-      const int row_nn = (k + 2 < R && pos_nn < end) ? rows[pos_nn] : -1;
       float acc[EPL];
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-      accumulate_range<TB, G, EPL, kPerm>(e0, e1, 0, G, lane_sub, colind, val, Bc, ldb, acc,
+      accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, 0, G, lane_sub, colind, val, Bc, ldb, acc,
                                           perm);
       store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
       row = row_n;
-      e0 = e0n;
-      e1 = e1n;
-      row_n = row_nn;
-      pos += subs_per_wg;
+      sp = sp_n;
+      pos = pos_n;
     }
   } else if (sg.kind == kSegB) {
     const int pos = sg.list_off + local_wg * kWavesPerWG + wave;
     if (pos >= sg.list_off + sg.count) return;  // whole wave
     const int row = rows[pos];
+    const int2 sp = span[pos];
     float acc[EPL];
 #pragma unroll
     for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    accumulate_range<TB, G, EPL, kPerm>(rowptr[row], rowptr[row + 1], sub * G, S * G,
-                                        lane_sub, colind, val, Bc, ldb, acc, perm);
+    accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, sub * G, S * G, lane_sub, colind, val, Bc,
+                                        ldb, acc, perm);
     wave_reduce_subgroups<G, EPL>(acc);
     if (sub == 0) store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
   } else {
     const int row = rows[sg.list_off + local_wg];
+    const int2 sp = span[sg.list_off + local_wg];
     float acc[EPL];
 #pragma unroll
     for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
     const int gsub = wave * S + sub;
-    accumulate_range<TB, G, EPL, kPerm>(rowptr[row], rowptr[row + 1], gsub * G,
-                                        kWavesPerWG * S * G, lane_sub, colind, val, Bc,
-                                        ldb, acc, perm);
+    accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, gsub * G, kWavesPerWG * S * G, lane_sub,
+                                        colind, val, Bc, ldb, acc, perm);
     wave_reduce_subgroups<G, EPL>(acc);
     if (wave > 0 && sub == 0) {
 #pragma unroll
@@ -404,7 +401,7 @@ __device__ __forceinline__ void sddmm_range(int e_begin, int e_end, int first_of
 template <typename TB, int G>
 __device__ __forceinline__ void sddmm_row_wg(const LaunchTable &tab, int wg,
                                              const int *__restrict__ rows,
-                                             const int *__restrict__ rowptr,
+                                             const int2 *__restrict__ span,
                                              const int *__restrict__ colind,
                                              const TB *__restrict__ dY, int lddy,
                                              const TB *__restrict__ X, int ldx,
@@ -425,19 +422,21 @@ __device__ __forceinline__ void sddmm_row_wg(const LaunchTable &tab, int wg,
     const int subs_per_wg = kWavesPerWG * S;
     const int end = sg.list_off + sg.count;
     int pos = sg.list_off + local_wg * subs_per_wg * R + gsub;
-    int row = pos < end ? rows[pos] : -1;
-    int e0 = 0, e1 = 0;
-    if (row >= 0) {
-      e0 = rowptr[row];
-      e1 = rowptr[row + 1];
+    int row = -1, e0 = 0, e1 = 0;
+    if (pos < end) {
+      row = rows[pos];
+      const int2 sp = span[pos];
+      e0 = sp.x;
+      e1 = sp.y;
     }
     for (int k = 0; k < R && row >= 0; ++k) {
       const int pos_n = pos + subs_per_wg;
-      const int row_n = (k + 1 < R && pos_n < end) ? rows[pos_n] : -1;
-      int e0n = 0, e1n = 0;
-      if (row_n >= 0) {
-        e0n = rowptr[row_n];
-        e1n = rowptr[row_n + 1];
+      int row_n = -1, e0n = 0, e1n = 0;
+      if (k + 1 < R && pos_n < end) {
+        row_n = rows[pos_n];
+        const int2 sp = span[pos_n];
+        e0n = sp.x;
+        e1n = sp.y;
       }
       for (int tt = 0; tt < ntile && e0 < e1; ++tt) {
         Vec16<TB>::load(dY + static_cast<size_t>(row) * lddy + tt * CW + lane_sub * EPL, y);
@@ -455,7 +454,8 @@ __device__ __forceinline__ void sddmm_row_wg(const LaunchTable &tab, int wg,
   const int pos = wave_mode ? sg.list_off + local_wg * kWavesPerWG + wave : sg.list_off + local_wg;
   if (pos >= sg.list_off + sg.count) return;  // whole wave (no barrier here)
   const int row = rows[pos];
-  const int e0 = rowptr[row], e1 = rowptr[row + 1];
+  const int2 sp = span[pos];
+  const int e0 = sp.x, e1 = sp.y;
   const int part = wave_mode ? sub : gsub;
   const int parts = wave_mode ? S : kWavesPerWG * S;
   for (int tt = 0; tt < ntile; ++tt) {
@@ -507,6 +507,7 @@ struct spmm_csr_plan_s {
   int nnz = 0;
   int device = 0;
   int *d_rows = nullptr;
+  int2 *d_span = nullptr;  // (rowptr[u], rowptr[u + 1]) of unit d_rows[i], beside it
   int counts[32] = {0};
   int offs[32] = {0};
   int ntuned = 0;

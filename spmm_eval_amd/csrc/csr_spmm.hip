@@ -68,23 +68,33 @@ bin_scatter_kernel(const int *__restrict__ rowptr, int M, int shift, int *__rest
   }
 }
 
+// span[i] = the entry range of unit rows[i] (of block row rows[i] >> shift).
+__global__ void __launch_bounds__(256)
+span_kernel(const int *__restrict__ rowptr, int M, int shift, const int *__restrict__ rows,
+            int2 *__restrict__ span) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+    const int q = rows[i] >> shift;
+    span[i] = make_int2(rowptr[q], rowptr[q + 1]);
+  }
+}
+
 template <typename TB, int G, int VEC>
 __global__ void __launch_bounds__(kThreads)
 csr_spmm_kernel(LaunchTable tab, const int *__restrict__ rows,
-                const int *__restrict__ rowptr, const int *__restrict__ colind,
+                const int2 *__restrict__ span, const int *__restrict__ colind,
                 const float *__restrict__ val, const TB *__restrict__ B, int ldb,
                 float *__restrict__ C, int ldc) {
-  csr_wg<TB, G, VEC>(tab, blockIdx.x, blockIdx.y, rows, rowptr, colind, val, B, ldb, C, ldc);
+  csr_wg<TB, G, VEC>(tab, blockIdx.x, blockIdx.y, rows, span, colind, val, B, ldb, C, ldc);
 }
 
 // C = A^T B on the transposed structure of A, values read through perm.
 template <typename TB, int G, int VEC>
 __global__ void __launch_bounds__(kThreads)
 csr_spmm_perm_kernel(LaunchTable tab, const int *__restrict__ rows,
-                     const int *__restrict__ rowptr, const int *__restrict__ colind,
+                     const int2 *__restrict__ span, const int *__restrict__ colind,
                      const float *__restrict__ val, const int *__restrict__ perm,
                      const TB *__restrict__ B, int ldb, float *__restrict__ C, int ldc) {
-  csr_wg<TB, G, VEC, true>(tab, blockIdx.x, blockIdx.y, rows, rowptr, colind, val, B, ldb, C,
+  csr_wg<TB, G, VEC, true>(tab, blockIdx.x, blockIdx.y, rows, span, colind, val, B, ldb, C,
                            ldc, perm);
 }
 
@@ -196,10 +206,10 @@ static void launch_vec(const LaunchTable &t, int col_tiles, const spmm_csr_plan_
   dim3 grid(t.total_wg, col_tiles);
   if (perm)
     hipLaunchKernelGGL((csr_spmm_perm_kernel<TB, G, VEC>), grid, dim3(kThreads), 0, s, t,
-                       p->d_rows, rowptr, colind, val, perm, B, ldb, C, ldc);
+                       p->d_rows, p->d_span, colind, val, perm, B, ldb, C, ldc);
   else
     hipLaunchKernelGGL((csr_spmm_kernel<TB, G, VEC>), grid, dim3(kThreads), 0, s, t,
-                       p->d_rows, rowptr, colind, val, B, ldb, C, ldc);
+                       p->d_rows, p->d_span, colind, val, B, ldb, C, ldc);
 }
 
 template <typename TB, int VEC>
@@ -324,8 +334,9 @@ int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
   p->nnz = nnz;
   p->device = h->device;
   if (M == 0) { *out = p; return SPMM_OK; }
-  if (hipMalloc(&p->d_rows, sizeof(int) * static_cast<size_t>(M)) != hipSuccess) {
-    delete p;
+  if (hipMalloc(&p->d_rows, sizeof(int) * static_cast<size_t>(M)) != hipSuccess ||
+      hipMalloc(&p->d_span, sizeof(int2) * static_cast<size_t>(M)) != hipSuccess) {
+    spmm_csr_plan_destroy(p);
     SPMM_FAIL(SPMM_ERR_ALLOC, "hipMalloc of the row list failed");
   }
   int *ws = static_cast<int *>(workspace(h, sizeof(int) * kBins));
@@ -359,6 +370,9 @@ int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(kThreads), 0, s, rowptr, M, shift, ws,
                      p->d_rows);
   if (hipGetLastError() != hipSuccess) return fail("bin_scatter_kernel launch");
+  hipLaunchKernelGGL(span_kernel, dim3(grid), dim3(kThreads), 0, s, rowptr, M, shift, p->d_rows,
+                     p->d_span);
+  if (hipGetLastError() != hipSuccess) return fail("span_kernel launch");
   // The pinned buffer and workspace are reused by the next call.
   if (hipStreamSynchronize(s) != hipSuccess) return fail("stream sync");
   *out = p;
@@ -512,6 +526,7 @@ int spmm_csr_analyze(spmm_handle_t h, int M, int nnz, const int *rowptr,
 int spmm_csr_plan_destroy(spmm_csr_plan_t plan) {
   if (!plan) return SPMM_OK;
   if (plan->d_rows) (void)hipFree(plan->d_rows);
+  if (plan->d_span) (void)hipFree(plan->d_span);
   delete plan;
   return SPMM_OK;
 }
