@@ -16,13 +16,16 @@
  * (test_spmm_full.cu:347-348).
  *
  * B2SR ("bit-block sparse row") layout, as produced by the reference's
- * csr2bsr_batch_{4,8,16,32} + ToBit{4,8,16,32}Col
- * (backend/csr2bsr_batch_bsrbmv.cu:207-397,403-579,654-830,894-1070;
- * backend/spmm.cu:165-194,226-246,272-291,318-335):
+ * csr2bsr_batch_{4,8,16,32,64} + ToBit{4,8,16,32,64}Col
+ * (backend/csr2bsr_batch_bsrbmv.cu:207-397,403-579,654-830,894-1070,
+ * 1076-1253; backend/spmm.cu:165-194,226-246,272-291,318-335,357-383):
  *   bsrRowPtr[MB+1], bsrColInd[nb] ascending within a block row, and
  *   bsrVal[nb*T] words; word bsrVal[T*b + r] has bit (T-1-c) set iff
- *   A[T*br + r][T*bc + c] is a stored entry. Word type: uint8 for T=4 (low
- *   nibble) and T=8, uint16 for T=16, uint32 for T=32.  MB = ceil(M/T).
+ *   A[T*br + r][T*bc + c] is a stored entry. Word type: uint8 for T=2 (bits
+ *   1..0), T=4 (low nibble) and T=8, uint16 for T=16, uint32 for T=32,
+ *   uint64 for T=64.  MB = ceil(M/T). (For T=2 the reference's ToBit2Col,
+ *   spmm.cu:133-162, overlaps the stores of neighbouring lanes; the rule
+ *   above is used.)
  *
  * BF16 operands are uint16 bit patterns (round-to-nearest-even from FP32).
  */
@@ -121,7 +124,7 @@ int spmm_csr_autotune(spmm_csr_plan_t plan, int M, int K, int N, int nnz,
  * device with column indices ascending within each row (as coo2csr emits,
  * backend/readMtx.hpp:246-304); values are ignored: every stored entry is a
  * set bit (the reference forces values to 1.0 before packing,
- * csr2bsr_batch_bsrbmv.cu:296-297). T in {4, 8, 16, 32}.
+ * csr2bsr_batch_bsrbmv.cu:296-297). T in {2, 4, 8, 16, 32, 64}.
  * csr2b2sr_nnzb writes bsrRowPtr[MB+1] and SYNCHRONISES to return nnzb. */
 int csr2b2sr_nnzb(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
                   const int *colind, int T, int *bsrRowPtr, int *nnzb,
@@ -144,8 +147,8 @@ int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N,
                    const void *bsrVal, const uint16_t *B, int ldb, float *C,
                    int ldc, void *stream);
 
-/* Planned FBF SpMM. spmm_b2sr_analyze bins the block rows (T = 4, 8) or the
- * rows (T = 16, 32) by tile count and keeps each unit's tile span beside
+/* Planned FBF SpMM. spmm_b2sr_analyze bins the block rows (T = 2, 4, 8) or
+ * the rows (T = 16, 32, 64) by tile count and keeps each unit's tile span beside
  * it (SYNCHRONISES once); the planned call runs every class in a single
  * launch (no list state, asynchronous) and reads the spans from the plan,
  * so a plan is valid only for the bsrRowPtr it was built from. The plan is

@@ -98,7 +98,7 @@ def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
 
 
-WORD_DTYPE = {4: np.uint8, 8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+WORD_DTYPE = {2: np.uint8, 4: np.uint8, 8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
 
 
 def omp_threads() -> int:

@@ -160,7 +160,10 @@ void orc_csr_spmm_i64(int M, int N, const int *rowptr, const int *colind,
  * word for block row r (backend/spmm.cu:165-194 (T=4, low nibble of a
  * uchar), :226-246 (T=8, uchar), :272-291 (T=16, ushort), :318-335 (T=32,
  * uint32), :357-383 (T=64, uint64)). Words are stored bsrVal[T*b + r]
- * (csr2bsr_batch_bsrbmv.cu:339). */
+ * (csr2bsr_batch_bsrbmv.cu:339). T=2 follows the same rule with a uchar per
+ * tile row (bits 1..0): the reference's ToBit2Col (spmm.cu:133-162) stores 8
+ * bytes per lane at a 4-byte stride, so neighbouring lanes overwrite each
+ * other and its layout is not a usable definition. */
 static int cmp_int(const void *a, const void *b) {
   int x = *(const int *)a, y = *(const int *)b;
   return (x > y) - (x < y);
@@ -193,7 +196,7 @@ static int max_block_row_entries(int M, const int *rowptr, int T) {
 /* Returns nb (or -1 on bad T); writes bsrRowPtr[MB+1]. */
 int orc_csr2b2sr_nnzb(int M, const int *rowptr, const int *colind, int T,
                       int *bsrRowPtr) {
-  if (T != 4 && T != 8 && T != 16 && T != 32 && T != 64) return -1;
+  if (T != 2 && T != 4 && T != 8 && T != 16 && T != 32 && T != 64) return -1;
   int MB = (M + T - 1) / T;
   int *buf = (int *)malloc(sizeof(int) * (size_t)(max_block_row_entries(M, rowptr, T) + 1));
   bsrRowPtr[0] = 0;
@@ -205,6 +208,7 @@ int orc_csr2b2sr_nnzb(int M, const int *rowptr, const int *colind, int T,
 
 static void set_bit(void *bsrVal, int T, int64_t word, int bit) {
   switch (T) {
+    case 2:
     case 4:
     case 8: ((uint8_t *)bsrVal)[word] |= (uint8_t)(1u << bit); break;
     case 16: ((uint16_t *)bsrVal)[word] |= (uint16_t)(1u << bit); break;
@@ -242,6 +246,7 @@ void orc_csr2b2sr(int M, const int *rowptr, const int *colind, int T,
 
 static inline uint64_t get_word(const void *bsrVal, int T, int64_t w) {
   switch (T) {
+    case 2:
     case 4:
     case 8: return ((const uint8_t *)bsrVal)[w];
     case 16: return ((const uint16_t *)bsrVal)[w];
@@ -254,7 +259,7 @@ static inline uint64_t get_word(const void *bsrVal, int T, int64_t w) {
  * FBF SpMM: C[T*br + r][n] = sum over tiles t of block row br, over c with
  * bit (T-1-c) of bsrVal[T*t + r] set, of B[T*bsrColInd[t] + c][n]
  * (spmm4_full_full_new2, backend/spmm.cu:877-956, decode at :905-909,
- * gather-accumulate at :931-943), generalised to T in {4,8,16,32,64}. Rows
+ * gather-accumulate at :931-943), generalised to T in {2,4,8,16,32,64}. Rows
  * past M (padding of the last block row) are not written. Double sums. */
 void orc_b2sr_spmm_f64(int T, int M, int N, const int *bsrRowPtr,
                        const int *bsrColInd, const void *bsrVal,

@@ -17,7 +17,8 @@ import torch
 from . import _lib
 from ._lib import Tune, call
 
-WORD_DTYPE = {4: torch.uint8, 8: torch.uint8, 16: torch.uint16, 32: torch.uint32}
+WORD_DTYPE = {2: torch.uint8, 4: torch.uint8, 8: torch.uint8, 16: torch.uint16, 32: torch.uint32,
+              64: torch.uint64}
 
 
 def _p(t):

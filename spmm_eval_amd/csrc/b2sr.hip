@@ -25,20 +25,36 @@
 #include "csr_core.hpp"
 
 #include <climits>
+#include <type_traits>
 
 namespace spmm {
 
 template <int T> struct Word;
+template <> struct Word<2> { using type = uint8_t; };
 template <> struct Word<4> { using type = uint8_t; };
 template <> struct Word<8> { using type = uint8_t; };
 template <> struct Word<16> { using type = uint16_t; };
 template <> struct Word<32> { using type = uint32_t; };
+template <> struct Word<64> { using type = uint64_t; };
+
+// A tile row's bits as a register: 64-bit only for T = 64.
+template <int T> using Bits = typename std::conditional<(T > 32), uint64_t, uint32_t>::type;
+template <typename U> __device__ __forceinline__ int popcount(U x) {
+  return sizeof(U) == 8 ? __popcll(static_cast<unsigned long long>(x))
+                        : __popc(static_cast<uint32_t>(x));
+}
+template <typename U> __device__ __forceinline__ int top_bit(U x) {  // x != 0
+  return sizeof(U) == 8 ? 63 - __clzll(static_cast<unsigned long long>(x))
+                        : 31 - __clz(static_cast<uint32_t>(x));
+}
 
 constexpr int kB2Threads = 256;
 constexpr int kB2Waves = kB2Threads / kWave;
 constexpr int kLongChunks = 2;  // block rows above this many tile chunks go to b2sr_long_kernel
 
-static bool valid_T(int T) { return T == 4 || T == 8 || T == 16 || T == 32; }
+static bool valid_T(int T) {
+  return T == 2 || T == 4 || T == 8 || T == 16 || T == 32 || T == 64;
+}
 
 // ------------------------------------------------------------ conversion
 template <int T, bool kFill>
@@ -74,12 +90,12 @@ csr2b2sr_kernel(int M, int K, const int *__restrict__ rowptr, const int *__restr
     for (int d = 1; d < T; d <<= 1) m = min(m, __shfl_xor(m, d));
     if (__all(m == INT_MAX)) break;
     if (m != INT_MAX) {
-      uint32_t bits = 0;
+      Bits<T> bits = 0;
       while (pos < end) {
         const int c = colind[pos];
         if (c < 0 || c >= K || c < prev) { bad = 1; pos = end; break; }
         if (c / T != m) break;
-        bits |= 1u << (T - 1 - (c % T));
+        bits |= static_cast<Bits<T>>(1) << (T - 1 - (c % T));
         prev = c;
         ++pos;
       }
@@ -115,7 +131,7 @@ __device__ __forceinline__ int decode_chunk(int t, int t1, int r, int lane_sub,
   constexpr int TC = Chunk<T, G>::kTiles;
   constexpr int TPL = Chunk<T, G>::kPerLane;
   int bc[TPL];
-  uint32_t w[TPL];
+  Bits<T> w[TPL];
   int cnt = 0;
 #pragma unroll
   for (int k = 0; k < TPL; ++k) {
@@ -127,7 +143,7 @@ __device__ __forceinline__ int decode_chunk(int t, int t1, int r, int lane_sub,
       bc[k] = bci[tile];
       w[k] = words[static_cast<size_t>(tile) * T + r];
     }
-    cnt += __popc(w[k]);
+    cnt += popcount(w[k]);
   }
   int inc = cnt;
 #pragma unroll
@@ -140,10 +156,10 @@ __device__ __forceinline__ int decode_chunk(int t, int t1, int r, int lane_sub,
 #pragma unroll
   for (int k = 0; k < TPL; ++k) {
     const int base = bc[k] * T + T - 1;
-    uint32_t x = w[k];
+    Bits<T> x = w[k];
     while (x) {
-      const int p = 31 - __clz(x);
-      x ^= 1u << p;
+      const int p = top_bit(x);
+      x ^= static_cast<Bits<T>>(1) << p;
       list[off++] = base - p;
     }
   }
@@ -247,7 +263,7 @@ b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
     for (int r0 = 0; r0 < T; r0 += kB2Waves) {
       const int r = r0 + wave;
       const int row = br * T + r;
-      if (row >= M) continue;  // whole wave
+      if (r >= T || row >= M) continue;  // whole wave (T = 2 leaves waves 2, 3 idle)
       float acc[EPL];
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
@@ -456,6 +472,7 @@ b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
 // decoding only that row's bits; the widest block rows set the kernel's
 // critical path, so they get the most lanes).
 template <int T> struct TileWord;
+template <> struct TileWord<2> { using type = uint16_t; };
 template <> struct TileWord<4> { using type = uint32_t; };
 template <> struct TileWord<8> { using type = uint64_t; };
 
@@ -808,11 +825,11 @@ b2sr_spmm_generic_kernel(int M, int N, const int *__restrict__ brp, const int *_
   for (int c = lane; c < N; c += kWave) {
     float acc = 0.f;
     for (int t = brp[br]; t < brp[br + 1]; ++t) {
-      uint32_t w = words[static_cast<size_t>(t) * T + r];
+      Bits<T> w = words[static_cast<size_t>(t) * T + r];
       const int base = bci[t] * T + T - 1;
       while (w) {
-        const int p = 31 - __clz(w);
-        w ^= 1u << p;
+        const int p = top_bit(w);
+        w ^= static_cast<Bits<T>>(1) << p;
         acc += load_scalar<TB>(B + static_cast<size_t>(base - p) * ldb + c);
       }
     }
@@ -927,10 +944,12 @@ template <bool kFill>
 static void convert(int T, int M, int K, const int *rowptr, const int *colind, int *brp,
                     int *bci, void *val, int *err, hipStream_t s) {
   switch (T) {
+    case 2: launch_convert<2, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
     case 4: launch_convert<4, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
     case 8: launch_convert<8, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
     case 16: launch_convert<16, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
-    default: launch_convert<32, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
+    case 32: launch_convert<32, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
+    default: launch_convert<64, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
   }
 }
 
@@ -1007,7 +1026,7 @@ static int b2sr_state(spmm_handle_s *h, int MB, int **out) {
 template <typename TB>
 static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp, const int *bci,
                    const void *val, const TB *B, int ldb, float *C, int ldc, void *stream) {
-  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
   if (M == 0 || N == 0) return SPMM_OK;
   SPMM_CHECK_ARG(brp && C, "null bsrRowPtr or C");
@@ -1026,12 +1045,16 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
     SPMM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     long_grid = 8 * cus;
   }
+#define SPMM_FBF(t) dispatch_fbf<t, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s)
   switch (T) {
-    case 4: dispatch_fbf<4, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s); break;
-    case 8: dispatch_fbf<8, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s); break;
-    case 16: dispatch_fbf<16, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s); break;
-    default: dispatch_fbf<32, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s); break;
+    case 2: SPMM_FBF(2); break;
+    case 4: SPMM_FBF(4); break;
+    case 8: SPMM_FBF(8); break;
+    case 16: SPMM_FBF(16); break;
+    case 32: SPMM_FBF(32); break;
+    default: SPMM_FBF(64); break;
   }
+#undef SPMM_FBF
   SPMM_LAUNCHED();
   return SPMM_OK;
 }
@@ -1113,7 +1136,7 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
                            const int *bci, const void *val, const TB *B, int ldb, float *C,
                            int ldc, void *stream) {
   SPMM_CHECK_ARG(p != nullptr, "plan is null");
-  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
   if (p->tile != T || p->M != M)
     SPMM_FAIL(SPMM_ERR_MISMATCH, "plan was not built by spmm_b2sr_analyze for this T and M");
@@ -1121,28 +1144,34 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
   SPMM_CHECK_ARG(brp && C, "null bsrRowPtr or C");
   SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldb >= N), "ldb/ldc smaller than N");
   SPMM_CHECK_ARG(K == 0 || B, "null B");
-  // T = 4 / 8 tiles are read as one 32 / 64-bit word; list entries keep
+  // T = 2 / 4 / 8 tiles are read as one 16 / 32 / 64-bit word; list entries keep
   // the B row in 27 bits
   SPMM_CHECK_ARG(T > 8 || reinterpret_cast<uintptr_t>(val) % T == 0,
                  "bsrVal must be aligned to T bytes for T = 4 and 8");
-  SPMM_CHECK_ARG(T > 8 || K < (1 << 27), "planned T = 4 / 8 needs K < 2^27");
+  SPMM_CHECK_ARG(T > 8 || K < (1 << 27), "planned T <= 8 needs K < 2^27");
   hipStream_t s = as_stream(stream);
   const Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
   if (g.G == 0) {  // scalar path needs no schedule
+#define SPMM_FBF(t) dispatch_fbf<t, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s)
     switch (T) {
-      case 4: dispatch_fbf<4, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s); break;
-      case 8: dispatch_fbf<8, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s); break;
-      case 16: dispatch_fbf<16, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s); break;
-      default: dispatch_fbf<32, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s); break;
+      case 2: SPMM_FBF(2); break;
+      case 4: SPMM_FBF(4); break;
+      case 8: SPMM_FBF(8); break;
+      case 16: SPMM_FBF(16); break;
+      case 32: SPMM_FBF(32); break;
+      default: SPMM_FBF(64); break;
     }
+#undef SPMM_FBF
     SPMM_LAUNCHED();
     return SPMM_OK;
   }
   switch (T) {
+    case 2: dispatch_brow<2, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 16: dispatch_planned<16, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
-    default: dispatch_planned<32, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 32: dispatch_planned<32, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    default: dispatch_planned<64, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
   }
   SPMM_LAUNCHED();
   return SPMM_OK;
@@ -1177,7 +1206,7 @@ extern "C" {
 int csr2b2sr_nnzb(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const int *colind,
                   int T, int *bsrRowPtr, int *nnzb, void *stream) {
   SPMM_CHECK_ARG(h && bsrRowPtr && nnzb, "null handle, bsrRowPtr or nnzb");
-  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && nnz >= 0, "negative size");
   SPMM_CHECK_ARG(M == 0 || rowptr, "null rowptr");
   SPMM_CHECK_ARG(nnz == 0 || colind, "null colind");
@@ -1209,7 +1238,7 @@ int csr2b2sr_nnzb(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, con
 int csr2b2sr(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const int *colind, int T,
              const int *bsrRowPtr, int *bsrColInd, void *bsrVal, void *stream) {
   SPMM_CHECK_ARG(h && bsrRowPtr, "null handle or bsrRowPtr");
-  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && nnz >= 0, "negative size");
   SPMM_CHECK_ARG(M == 0 || rowptr, "null rowptr");
   SPMM_CHECK_ARG(nnz == 0 || (colind && bsrColInd && bsrVal), "null colind or outputs");
@@ -1237,7 +1266,7 @@ int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRo
 int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr, spmm_csr_plan_t *plan,
                       void *stream) {
   SPMM_CHECK_ARG(h && plan && bsrRowPtr, "null handle, plan or bsrRowPtr");
-  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M > 0, "M must be positive");
   int st;
   if (T <= 8) {
@@ -1272,7 +1301,7 @@ int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const 
                              const int *bsrColInd, const void *bsrVal, const uint32_t *Bw,
                              int ldbw, int32_t *C, int ldc, void *stream) {
   (void)h;
-  SPMM_CHECK_ARG(valid_T(T), "T must be 4, 8, 16 or 32");
+  SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
   if (M == 0 || N == 0) return SPMM_OK;
   SPMM_CHECK_ARG(bsrRowPtr && C, "null bsrRowPtr or C");
@@ -1280,10 +1309,12 @@ int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const 
   SPMM_CHECK_ARG(K == 0 || Bw, "null Bw");
   hipStream_t s = as_stream(stream);
   switch (T) {
+    case 2: dispatch_sign<2>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
     case 4: dispatch_sign<4>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
     case 8: dispatch_sign<8>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
     case 16: dispatch_sign<16>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
-    default: dispatch_sign<32>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
+    case 32: dispatch_sign<32>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
+    default: dispatch_sign<64>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
   }
   SPMM_LAUNCHED();
   return SPMM_OK;

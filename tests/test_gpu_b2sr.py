@@ -34,7 +34,12 @@ def assert_b2sr_equal(dev_out, ref):
     brp, bci, words = dev_out
     np.testing.assert_array_equal(brp.cpu().numpy(), ref[0])
     np.testing.assert_array_equal(bci.cpu().numpy(), ref[1])
-    w = words.view(torch.int32).cpu().numpy().view(np.uint32) if words.dtype == torch.uint32 else np_words(words)
+    signed = {torch.uint32: (torch.int32, np.uint32), torch.uint64: (torch.int64, np.uint64)}
+    if words.dtype in signed:
+        st, nt = signed[words.dtype]
+        w = words.view(st).cpu().numpy().view(nt)
+    else:
+        w = np_words(words)
     np.testing.assert_array_equal(w, ref[2])
 
 
@@ -56,7 +61,7 @@ def test_hand_vectors(handle, dev, T):
     assert C2.cpu().tolist() == HAND["C"]
 
 
-@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("shape", [(1, 1), (37, 101), (300, 300), (1000, 777)])
 def test_csr2b2sr_random(handle, dev, T, shape):
     M, K = shape
@@ -66,7 +71,7 @@ def test_csr2b2sr_random(handle, dev, T, shape):
     assert_b2sr_equal(api.csr2b2sr(handle, rp, ci, M, K, T), ref)
 
 
-@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 def test_csr2b2sr_workload(handle, dev, T):
     g = graphs.workload_b2sr_graph()
     ref = oracle.csr2b2sr(g.rowptr, g.colind, T)
@@ -91,7 +96,7 @@ def b2sr_graph():
     return graphs.workload_b2sr_graph()
 
 
-@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("N", [32, 64, 128])
 def test_fbf_workload(handle, dev, b2sr_graph, T, N):
     g = b2sr_graph
@@ -121,7 +126,7 @@ def test_fbf_workload(handle, dev, b2sr_graph, T, N):
     assert np.linalg.norm(Chp - refq) / np.linalg.norm(refq) <= 1e-5
 
 
-@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("N", [1, 7, 100])
 def test_fbf_odd_widths(handle, dev, T, N):
     g = graphs.random_csr(333, 444, 0.02, seed=N, long_rows=1, long_len=444)
@@ -136,7 +141,7 @@ def test_fbf_odd_widths(handle, dev, T, N):
     assert np.linalg.norm(Cp - ref) / max(np.linalg.norm(ref), 1e-30) <= 1e-5
 
 
-@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("N", [32, 64, 70, 128])
 def test_sign_packed_popcount(handle, dev, b2sr_graph, T, N):
     g = b2sr_graph
@@ -170,7 +175,7 @@ def test_dense_packers(dev, H, W):
     assert np_u32(api.pack_bin_rows(Xh)).reshape(-1).tolist() == sign["bin"]
 
 
-@pytest.mark.parametrize("T", [4, 32])
+@pytest.mark.parametrize("T", [2, 4, 32, 64])
 @pytest.mark.parametrize("N", [64, 512])
 def test_fbf_long_block_rows_repeated(handle, dev, T, N):
     """Block rows long enough to go through the long-row list (N=512 adds a
@@ -193,7 +198,7 @@ def test_fbf_long_block_rows_repeated(handle, dev, T, N):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
-@pytest.mark.parametrize("T", [4, 16])
+@pytest.mark.parametrize("T", [2, 4, 16, 64])
 def test_fbf_planned_hub_rows_and_mismatch(handle, dev, T):
     """Rows of 3000-4000 entries reach the one-workgroup-per-row class; a
     plan for another T or M, or a CSR plan, is refused."""
@@ -217,7 +222,7 @@ def test_fbf_planned_hub_rows_and_mismatch(handle, dev, T):
         api.spmm_csr(handle, rp, ci, torch.ones(g.nnz, device=dev), Bd, plan=plan)
 
 
-@pytest.mark.parametrize("T", [4, 8, 16, 32])
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("N,bf16", [(64, False), (32, False), (256, True)])
 def test_fbf_planned_dense_tiles(handle, dev, T, N, bf16):
     """Nearly full tiles: a chunk's set bits overflow one list window (the

@@ -148,7 +148,7 @@ def test_bf16_rounding_matches_torch():
 
 
 # ---------------------------------------------------------------- B2SR
-@pytest.mark.parametrize("T", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("shape", [(1, 1), (13, 29), (70, 70), (257, 100)])
 def test_csr2b2sr_matches_brute_force(T, shape):
     M, K = shape
@@ -172,7 +172,7 @@ def test_planted_bit_order_is_caught():
     assert lsb_first.tolist() != HAND["T4"]["bsrVal"]
 
 
-@pytest.mark.parametrize("T", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 def test_b2sr_spmm_matches_dense(T):
     g = chung_lu(300, 3000, seed=T)
     B = np.random.default_rng(T).uniform(-1, 1, (300, 40)).astype(np.float32)
