@@ -11,7 +11,8 @@ reads one full B row), with GFLOP/s = 2 nnz N / t.
 16,384-row graph (N=32) and on the 232,965-row ~114.6M-nnz graph (N=64 and
 128, FP32 and BF16 B); GCN-style fwd+bwd (Y = A X, dX = A^T dY,
 dval = SDDMM) in FP32 and BF16 features; 512 uneven graphs batched in one
-captured hipGraph vs 512 separate FP32 launches. Errors there are measured
+captured hipGraph vs 512 separate FP32 launches; aligned B2SR-8/16 binary
+SpMV (binary and FP32 x) on the headline graph beside CSR at N = 1. Errors there are measured
 against an FP64 numpy recomputation on sampled rows.
 
 `cpu_baseline`: the OpenMP FP32 CSR SpMM of the oracle library on the same
@@ -271,6 +272,52 @@ def extra_csr(args, h, dev):
     return out
 
 
+def extra_spmv_aligned(args, h, dev):
+    """Aligned B2SR-8/16 repack and the binary SpMV over it on the headline
+    graph, beside CSR SpMM at N = 1 (the csrmv the reference checks against)."""
+    import torch
+
+    from spmm_eval_amd import api, graphs
+
+    out = {}
+    g = graphs.workload_b2sr_graph()
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    rng = np.random.default_rng(11)
+    xf = torch.from_numpy(rng.uniform(-1, 1, g.K).astype(np.float32)).to(dev)
+    plan = api.CsrPlan(h, rp, g.nnz)
+    y = torch.empty((g.M, 1), device=dev)
+    ms_csr = time_loop(lambda: api.spmm_csr(h, rp, ci, v, xf[:, None], C_out=y, plan=plan), 200, 20)
+    out["spmv_csr_f32_89k"] = {"ms": round(ms_csr, 5),
+                               "GBps": round(csr_bytes(g.M, g.nnz, 1) / ms_csr / 1e6, 1)}
+    del plan
+    for T in (8, 16):
+        t0 = time.time()
+        brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
+        nrp, nci, nval = api.b2sr_aligned(h, T, brp, bci, words)
+        torch.cuda.synchronize()
+        conv_ms = (time.time() - t0) * 1e3
+        MB, KB, nt = (g.M + T - 1) // T, (g.K + T - 1) // T, nci.numel()
+        xb = torch.from_numpy(rng.integers(0, 1 << T, KB).astype(np.int32)).to(dev)
+        xb = xb.to(torch.uint8) if T == 8 else xb.to(torch.int16).view(torch.uint16)
+        cnt = torch.empty(g.M, dtype=torch.int32, device=dev)
+        yb = torch.empty(MB, dtype=xb.dtype, device=dev)
+        yf = torch.empty(g.M, device=dev)
+        ms_bin = time_loop(lambda: api.spmv_b2sr_aligned(h, T, g.M, g.K, nrp, nci, nval, xb,
+                                                         y_count=cnt, y_bin=yb), 200, 20)
+        ms_f32 = time_loop(lambda: api.spmv_b2sr_aligned(h, T, g.M, g.K, nrp, nci, nval, xf,
+                                                         y=yf), 200, 20)
+        struct = (MB + 1) * 4 + nt * 4 + nval.numel() * 4
+        b_bin = struct + KB * T // 8 + g.M * 4 + MB * T // 8
+        b_f32 = struct + g.K * 4 + g.M * 4
+        out[f"spmv_b2sr{T}_aligned_89k"] = {
+            "tiles_padded": nt, "tiles": int(bci.numel()), "convert_ms_host_clock": round(conv_ms, 2),
+            "bin_ms": round(ms_bin, 5), "bin_GBps": round(b_bin / ms_bin / 1e6, 1),
+            "f32_ms": round(ms_f32, 5), "f32_GBps": round(b_f32 / ms_f32 / 1e6, 1)}
+        del brp, bci, words, nrp, nci, nval
+    torch.cuda.empty_cache()
+    return out
+
+
 def extra_fwd_bwd(args, h, dev):
     import torch
 
@@ -375,7 +422,7 @@ def main():
     ap.add_argument("--big-steps", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["csr", "fwd_bwd", "batched"], default=None,
+    ap.add_argument("--only", choices=["csr", "fwd_bwd", "batched", "spmv_aligned"], default=None,
                     help="run one extra workload alone and print its result (for profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
@@ -402,7 +449,8 @@ def main():
 
     h = api.Handle()
     if args.only:
-        fn = {"csr": extra_csr, "fwd_bwd": extra_fwd_bwd, "batched": extra_batched}[args.only]
+        fn = {"csr": extra_csr, "fwd_bwd": extra_fwd_bwd, "batched": extra_batched,
+              "spmv_aligned": extra_spmv_aligned}[args.only]
         print(json.dumps(fn(args, h, dev)))
         h.close()
         return
@@ -453,7 +501,7 @@ def main():
         result["cpu_baseline"] = cpu_baseline(hd)
     if not args.quick:
         extra = {}
-        for fn in (extra_csr, extra_fwd_bwd, extra_batched):
+        for fn in (extra_csr, extra_fwd_bwd, extra_batched, extra_spmv_aligned):
             log(fn.__name__)
             try:
                 extra.update(fn(args, h, dev))

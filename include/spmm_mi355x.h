@@ -173,6 +173,43 @@ int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N,
                              const void *bsrVal, const uint32_t *Bw, int ldbw,
                              int32_t *C, int ldc, void *stream);
 
+/* ------------------------------------------------------ aligned B2SR-8/16
+ * Regroups a B2SR-8 or B2SR-16 matrix into uint32 words: each block row's
+ * tiles in groups of 32/T (padded with empty tiles of column 0), word
+ * newVal[T*g + j] = row j of the group's tiles, first tile in the highest
+ * bits. b2sr_aligned_nnzb replaces countUcharUnsignedSize /
+ * countUshortUnsignedSize (backend/csr2bsr_batch_bsrbmv.cu:586-600,
+ * 837-851) and also writes newRowPtr[MB+1] (padded tile offsets); it
+ * SYNCHRONISES to return the padded tile count (words = nnzb_aligned /
+ * (32/T) * T). b2sr_aligned_pack replaces packUchar2AlignedUnsigned /
+ * packUshort2AlignedUnsigned (:607-647, :858-888). */
+int b2sr_aligned_nnzb(spmm_handle_t h, int T, int MB, const int *bsrRowPtr,
+                      int *newRowPtr, int *nnzb_aligned, void *stream);
+int b2sr_aligned_pack(spmm_handle_t h, int T, int MB, int nnzb_aligned,
+                      const int *bsrRowPtr, const int *bsrColInd,
+                      const void *bsrVal, const int *newRowPtr,
+                      int *newColInd, uint32_t *newVal, void *stream);
+
+/* Binary SpMV on the aligned layout, y = A_bin * x for the M rows of A.
+ * The reference's bsrbmv kernels are absent from its tree
+ * (csr2bsr_batch_bsrbmv.cu:1); its checks of their output are
+ * backend/utility.cu:135-185 (full-value vectors against cuSPARSE csrmv,
+ * and the binarised vector through bin2full).
+ * _bin: x_bits holds one T-bit word per block column (bit T-1-c =
+ * x[T*bc + c]); y_count[M] (int32, may be NULL) gets exact counts and
+ * y_bin[ceil(M/T)] (T-bit words, may be NULL) gets bit T-1-j of word br set
+ * iff row T*br + j has a non-zero count.
+ * _f32: x[K] FP32, y[M] FP32 (sum over the row's set bits). */
+int spmv_b2sr_aligned_bin(spmm_handle_t h, int T, int M, int K,
+                          int nnzb_aligned, const int *newRowPtr,
+                          const int *newColInd, const uint32_t *newVal,
+                          const void *x_bits, int32_t *y_count, void *y_bin,
+                          void *stream);
+int spmv_b2sr_aligned_f32(spmm_handle_t h, int T, int M, int K,
+                          int nnzb_aligned, const int *newRowPtr,
+                          const int *newColInd, const uint32_t *newVal,
+                          const float *x, float *y, void *stream);
+
 /* Dense bit packers (backend/spmm.cu:57-130).
  * pack_sign_rows_u32 / pack_bin_rows_u32: X[H][W] (row stride ldx) ->
  *   out[H][ceil(W/32)], bit (31-i) of word w = X[h][32w+i] >= 0 (sign) or

@@ -67,6 +67,9 @@ def lib():
             "orc_bmm32_xnor_bin": ([_i, _i, _i, _u32p, _u32p, _u32p], None),
             "orc_csr2csc": ([_i, _i, _i32p, _i32p, _vp, _i32p, _i32p, _vp, _vp], None),
             "orc_sddmm_f64": ([_i, _i, _i32p, _i32p, _f32p, _i, _f32p, _i, _f64p], None),
+            "orc_b2sr_aligned_size": ([_i, _i, _i32p], _i),
+            "orc_b2sr_aligned_pack": ([_i, _i, _i32p, _i32p, _vp, _i32p, _i32p, _u32p], None),
+            "orc_spmv_b2sr_aligned": ([_i, _i, _i32p, _i32p, _u32p, _vp, _i32p, _vp], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -259,3 +262,33 @@ def sddmm_f64(rowptr, colind, dY, X) -> np.ndarray:
     out = np.empty(max(int(rowptr[-1]), 1), np.float64)
     lib().orc_sddmm_f64(M, N, rowptr, colind if len(colind) else np.zeros(1, np.int32), dY, N, X, X.shape[1], out)
     return out[: int(rowptr[-1])]
+
+
+def b2sr_aligned_pack(T, brp, bci, words):
+    """Aligned B2SR-8/16 repack -> (new_rowptr, new_colind, new_val uint32)."""
+    brp = _c(brp, np.int32)
+    MB = len(brp) - 1
+    n = lib().orc_b2sr_aligned_size(T, MB, brp)
+    if n < 0:
+        raise ValueError(f"aligned repack needs T = 8 or 16, got {T}")
+    nrp = np.empty(MB + 1, np.int32)
+    nci = np.empty(max(n, 1), np.int32)
+    nval = np.zeros(max(n // (32 // T) * T, 1), np.uint32)
+    words = _c(words, WORD_DTYPE[T])
+    lib().orc_b2sr_aligned_pack(T, MB, brp, _c(bci, np.int32) if len(bci) else np.zeros(1, np.int32),
+                                words.ctypes.data, nrp, nci, nval)
+    return nrp, nci[:n], nval[: n // (32 // T) * T]
+
+
+def spmv_b2sr_aligned(T, nrp, nci, nval, x_words):
+    """Binary SpMV on the aligned layout -> (counts int32 [MB*T], binarised words [MB])."""
+    nrp = _c(nrp, np.int32)
+    MB = len(nrp) - 1
+    xw = _c(x_words, WORD_DTYPE[T])
+    cnt = np.empty(MB * T, np.int32)
+    yb = np.empty(max(MB, 1), WORD_DTYPE[T])
+    lib().orc_spmv_b2sr_aligned(T, MB, nrp, _c(nci, np.int32) if len(nci) else np.zeros(1, np.int32),
+                                _c(nval, np.uint32) if len(nval) else np.zeros(1, np.uint32),
+                                xw.ctypes.data, cnt, yb.ctypes.data)
+    return cnt, yb[:MB]
+

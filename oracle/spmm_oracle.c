@@ -431,3 +431,82 @@ void orc_sddmm_f64(int M, int N, const int *rowptr, const int *colind,
       out[e] = s;
     }
 }
+
+/* ------------------------------------------------- aligned B2SR repack --
+ * B2SR-8 / 16 regrouped into 32-bit words, the reference's
+ * packUchar2AlignedUnsigned / packUshort2AlignedUnsigned with their size
+ * estimates countUcharUnsignedSize / countUshortUnsignedSize
+ * (csr2bsr_batch_bsrbmv.cu:586-646, 837-888): each block row's tiles are
+ * taken in groups of GS = 32 / T (4 for T = 8, 2 for T = 16), padded with
+ * empty tiles of column 0; the group's word for tile row j is
+ * tile(i)[j] << (32 - T) | tile(i+1)[j] << (32 - 2T) | ..., first tile in the
+ * highest bits (:623-627, :874-876). new_rowptr counts tiles (a multiple of
+ * GS per block row), new_colind[tiles] the padded columns, new_val[tiles/GS
+ * * T] the words (group-major, then row). Returns the padded tile count. */
+int orc_b2sr_aligned_size(int T, int MB, const int *rowptr) {
+  if (T != 8 && T != 16) return -1;
+  int GS = 32 / T, n = 0;
+  for (int br = 0; br < MB; ++br) {
+    int t = rowptr[br + 1] - rowptr[br];
+    n += (t + GS - 1) / GS * GS;
+  }
+  return n;
+}
+
+void orc_b2sr_aligned_pack(int T, int MB, const int *rowptr, const int *colind,
+                           const void *val, int *new_rowptr, int *new_colind,
+                           uint32_t *new_val) {
+  int GS = 32 / T;
+  new_rowptr[0] = 0;
+  int64_t wcnt = 0;
+  int ccnt = 0;
+  for (int br = 0; br < MB; ++br) {
+    int t0 = rowptr[br], n = rowptr[br + 1] - t0;
+    int padded = (n + GS - 1) / GS * GS;
+    for (int i = 0; i < padded; i += GS) {
+      for (int j = 0; j < T; ++j) {
+        uint32_t w = 0;
+        for (int k = 0; k < GS; ++k) {
+          uint32_t part = 0;
+          if (i + k < n) part = (uint32_t)get_word(val, T, (int64_t)(t0 + i + k) * T + j);
+          w |= part << (32 - T * (k + 1));
+        }
+        new_val[wcnt++] = w;
+      }
+    }
+    for (int i = 0; i < padded; ++i) new_colind[ccnt++] = i < n ? colind[t0 + i] : 0;
+    new_rowptr[br + 1] = new_rowptr[br] + padded;
+  }
+}
+
+/* Binary SpMV on the aligned layout (the consumer the reference's absent
+ * bsrbmv would be; its checks compare counts and the binarised vector,
+ * utility.cu:135-185): x is bit-packed per block column (x_words[bc], bit
+ * T-1-c = x[T*bc + c], the tile rows' own bit order). For row r = T*br + j:
+ * count[r] = sum over groups of popc(word_j & xw), xw the group's GS x words
+ * in the same positions as its tiles. y_count[MB*T] (int32) and y_bin[MB]
+ * (bit T-1-j set iff count[T*br + j] > 0, the layout bin2full decodes,
+ * utility.cu:154-178); either may be NULL. */
+void orc_spmv_b2sr_aligned(int T, int MB, const int *new_rowptr, const int *new_colind,
+                           const uint32_t *new_val, const void *x_words, int32_t *y_count,
+                           void *y_bin) {
+  int GS = 32 / T;
+  for (int br = 0; br < MB; ++br) {
+    uint32_t bin = 0;
+    for (int j = 0; j < T; ++j) {
+      int32_t c = 0;
+      for (int t = new_rowptr[br]; t < new_rowptr[br + 1]; t += GS) {
+        uint32_t xw = 0;
+        for (int k = 0; k < GS; ++k)
+          xw |= (uint32_t)get_word(x_words, T, new_colind[t + k]) << (32 - T * (k + 1));
+        c += __builtin_popcount(new_val[(int64_t)(t / GS) * T + j] & xw);
+      }
+      if (y_count) y_count[(int64_t)br * T + j] = c;
+      if (c > 0) bin |= 1u << (T - 1 - j);
+    }
+    if (y_bin) {
+      if (T == 8) ((uint8_t *)y_bin)[br] = (uint8_t)bin;
+      else ((uint16_t *)y_bin)[br] = (uint16_t)bin;
+    }
+  }
+}

@@ -59,6 +59,10 @@ _SIGS = {
     "spmm_b2sr_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_b2sr_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_b2sr_signpacked_i32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
+    "b2sr_aligned_nnzb": [_vp, _i, _i, _vp, _vp, C.POINTER(_i), _vp],
+    "b2sr_aligned_pack": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "spmv_b2sr_aligned_bin": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "spmv_b2sr_aligned_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "pack_sign_rows_u32": [_i, _i, _vp, _i, _vp, _vp],
     "pack_bin_rows_u32": [_i, _i, _vp, _i, _vp, _vp],
     "pack_sign_cols_u32": [_i, _i, _vp, _i, _vp, _vp],

@@ -229,6 +229,45 @@ def spmm_b2sr_sign(h: Handle, T: int, M: int, K: int, N: int, brp, bci, words, B
     return C_out
 
 
+def b2sr_aligned(h: Handle, T: int, brp, bci, words, stream=None):
+    """B2SR-8/16 -> aligned 32-bit groups (newRowPtr, newColInd, newVal uint32):
+    countUcharUnsignedSize + packUchar2AlignedUnsigned and their 16-bit forms
+    (backend/csr2bsr_batch_bsrbmv.cu:586-647, 837-888)."""
+    MB = brp.numel() - 1
+    dev = brp.device
+    nrp = torch.empty(MB + 1, dtype=torch.int32, device=dev)
+    n = C.c_int()
+    call("b2sr_aligned_nnzb", h.h, T, MB, _p(brp), _p(nrp), C.byref(n), _stream(stream))
+    nwords = n.value // (32 // T) * T
+    nci = torch.empty(max(n.value, 1), dtype=torch.int32, device=dev)
+    nval = torch.empty(max(nwords, 1), dtype=torch.uint32, device=dev)
+    call("b2sr_aligned_pack", h.h, T, MB, n.value, _p(brp), _p(bci), _p(words), _p(nrp),
+         _p(nci), _p(nval), _stream(stream))
+    return nrp, nci[: n.value], nval[:nwords]
+
+
+def spmv_b2sr_aligned(h: Handle, T: int, M: int, K: int, nrp, nci, nval, x, y_count=None,
+                      y_bin=None, y=None, stream=None):
+    """Binary SpMV on the aligned layout. x of T-bit words (one per block
+    column) gives (y_count int32 [M], y_bin T-bit words [ceil(M/T)]); FP32 x
+    [K] gives y FP32 [M]."""
+    nnzb = nci.numel()
+    if x.dtype == torch.float32:
+        if y is None:
+            y = torch.empty(M, dtype=torch.float32, device=x.device)
+        call("spmv_b2sr_aligned_f32", h.h, T, M, K, nnzb, _p(nrp), _p(nci), _p(nval), _p(x),
+             _p(y), _stream(stream))
+        return y
+    _dev(x, WORD_DTYPE[T])
+    if y_count is None:
+        y_count = torch.empty(M, dtype=torch.int32, device=x.device)
+    if y_bin is None:
+        y_bin = torch.empty((M + T - 1) // T, dtype=WORD_DTYPE[T], device=x.device)
+    call("spmv_b2sr_aligned_bin", h.h, T, M, K, nnzb, _p(nrp), _p(nci), _p(nval), _p(x),
+         _p(y_count), _p(y_bin), _stream(stream))
+    return y_count, y_bin
+
+
 def _pack(fn: str, X, cols: bool, stream=None):
     H, W = X.shape
     Wc = (W + 31) // 32

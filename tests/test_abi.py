@@ -40,6 +40,12 @@ NULL = C.c_void_p(0)
     ("sddmm_csr_f32", (NULL, 2, 2, 4, 2, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), 2,
                        C.c_void_p(16), 4, C.c_void_p(16), NULL)),  # lddy < N
     ("spmm_batch_plan_create", (NULL, 1, NULL, NULL, NULL, NULL, NULL, 8, 0, None, NULL)),
+    ("b2sr_aligned_nnzb", (C.c_void_p(16), 4, 3, C.c_void_p(16), C.c_void_p(16), None, NULL)),
+    ("b2sr_aligned_pack", (C.c_void_p(16), 16, 3, 3, NULL, NULL, NULL, NULL, NULL, NULL, NULL)),  # 3 % 2
+    ("spmv_b2sr_aligned_bin", (C.c_void_p(16), 8, 16, 16, 4, C.c_void_p(16), C.c_void_p(16),
+                               C.c_void_p(16), C.c_void_p(16), NULL, NULL, NULL)),  # no output
+    ("spmv_b2sr_aligned_f32", (C.c_void_p(16), 32, 16, 16, 4, NULL, NULL, NULL, NULL,
+                               C.c_void_p(16), NULL)),  # T = 32 has no aligned form
 ])
 def test_invalid_arguments_rejected_on_host(call, args):
     L = _lib.lib()

@@ -321,3 +321,94 @@ def test_generator_csr_matches_reference_coo2csr():
                   np.ascontiguousarray(g.val[perm]), rp, ci, cv)
     np.testing.assert_array_equal(rp, g.rowptr)
     np.testing.assert_array_equal(ci, g.colind)
+
+
+# ---- aligned B2SR-8/16 repack and the binary SpMV over it ----
+
+def test_aligned_pack_hand_vector():
+    """One T = 8 block row of 5 tiles -> 2 groups of 4 (3 empty column-0
+    tiles of padding); word j of a group is its tiles' row-j bytes, first
+    tile highest (hand-computed)."""
+    T = 8
+    brp = np.array([0, 5, 5], np.int32)  # second block row empty
+    bci = np.array([1, 3, 4, 6, 9], np.int32)
+    tiles = np.zeros((5, T), np.uint8)
+    tiles[0, 0], tiles[1, 0], tiles[2, 0], tiles[3, 0] = 0x80, 0x01, 0xFF, 0x10
+    tiles[4, 7] = 0xA5
+    nrp, nci, nval = oracle.b2sr_aligned_pack(T, brp, bci, tiles.reshape(-1))
+    np.testing.assert_array_equal(nrp, [0, 8, 8])
+    np.testing.assert_array_equal(nci, [1, 3, 4, 6, 9, 0, 0, 0])
+    assert nval.shape == (16,)
+    assert nval[0] == 0x8001FF10 and (nval[1:8] == 0).all()
+    assert (nval[8:15] == 0).all() and nval[15] == 0xA5000000
+
+
+def _unaligned(T, nrp, nval):
+    """Independent reading of the layout: a uint32 word in big-endian byte
+    order is its GS tiles' row pieces in order."""
+    GS = 32 // T
+    be = nval.astype(">u4").view(np.uint8 if T == 8 else ">u2")  # [groups*T*GS]
+    g = be.reshape(-1, T, GS).transpose(0, 2, 1)                   # [groups][GS][T]
+    return g.reshape(-1, T).astype(oracle.WORD_DTYPE[T])           # padded tiles x rows
+
+
+@pytest.mark.parametrize("T", [8, 16])
+@pytest.mark.parametrize("shape", [(1, 1), (70, 70), (300, 517)])
+def test_aligned_pack_round_trip(T, shape):
+    g = random_csr(*shape, density=0.05, seed=shape[0] + T)
+    brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    nrp, nci, nval = oracle.b2sr_aligned_pack(T, brp, bci, words)
+    GS = 32 // T
+    n_tiles = np.diff(brp)
+    np.testing.assert_array_equal(np.diff(nrp), -(-n_tiles // GS) * GS)
+    back = _unaligned(T, nrp, nval) if len(nval) else np.zeros((0, T))
+    old = words.reshape(-1, T)
+    for br in range(len(brp) - 1):
+        a, n, p = brp[br], n_tiles[br], nrp[br]
+        np.testing.assert_array_equal(back[p:p + n], old[a:a + n])
+        np.testing.assert_array_equal(nci[p:p + n], bci[a:a + n])
+        assert (back[p + n:nrp[br + 1]] == 0).all() and (nci[p + n:nrp[br + 1]] == 0).all()
+
+
+def test_aligned_pack_rejects_other_T():
+    with pytest.raises(ValueError):
+        oracle.b2sr_aligned_pack(4, np.array([0, 1], np.int32), np.array([0], np.int32),
+                                 np.zeros(4, np.uint8))
+
+
+def _pack_x(x01, T):
+    """x bit-packed per block column, bit T-1-c = x[T*bc + c] (numpy.packbits)."""
+    KB = -(-len(x01) // T)
+    xp = np.zeros(KB * T, np.uint8)
+    xp[:len(x01)] = x01
+    b = np.packbits(xp.reshape(KB, T), axis=1)  # big-endian bytes per block column
+    return b[:, 0] if T == 8 else b.view(">u2")[:, 0].astype(np.uint16)
+
+
+@pytest.mark.parametrize("T", [8, 16])
+@pytest.mark.parametrize("shape", [(1, 1), (33, 70), (250, 250), (400, 1030)])
+def test_aligned_spmv_matches_dense(T, shape):
+    g = random_csr(*shape, density=0.04, seed=7 + T)
+    x = (np.random.default_rng(T).random(g.K) < 0.5).astype(np.uint8)
+    brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    nrp, nci, nval = oracle.b2sr_aligned_pack(T, brp, bci, words)
+    cnt, yb = oracle.spmv_b2sr_aligned(T, nrp, nci, nval, _pack_x(x, T))
+    want = dense(g, binary=True).astype(np.int64) @ x
+    MB = len(brp) - 1
+    full = np.zeros(MB * T, np.int64)
+    full[:g.M] = want
+    np.testing.assert_array_equal(cnt, full)
+    np.testing.assert_array_equal(yb, _pack_x((full > 0).astype(np.uint8), T))
+
+
+def test_planted_group_order_is_caught():
+    """Reading the group's tiles lowest-first (a plausible slip) breaks the
+    dense check above."""
+    T = 8
+    g = random_csr(64, 256, density=0.2, seed=1)
+    brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    nrp, nci, nval = oracle.b2sr_aligned_pack(T, brp, bci, words)
+    bad = nval.astype(">u4").view("<u4").astype(np.uint32)  # byte-reversed words
+    x = (np.random.default_rng(0).random(g.K) < 0.5).astype(np.uint8)
+    cnt, _ = oracle.spmv_b2sr_aligned(T, nrp, nci, bad, _pack_x(x, T))
+    assert not np.array_equal(cnt[:g.M], dense(g, binary=True).astype(np.int64) @ x)
