@@ -15,8 +15,8 @@
 // any other (the reference's packers are single-thread loops over all block
 // rows).
 //
-// SpMV y = A_bin * x: a subgroup of L lanes per block row (L a multiple of T,
-// chosen from the average group count). Lane (gl, j) keeps row j fixed and
+// SpMV y = A_bin * x: L lanes per block row (a multiple of T, up to a whole
+// workgroup, chosen from the average group count). Lane (gl, j) keeps row j fixed and
 // strides over the block row's groups gl, gl + L/T, ...; consecutive lanes
 // read consecutive words. Binary x (bit-packed per block column, the tile
 // rows' bit order) gives exact int32 counts and the binarised block-row word
@@ -89,58 +89,120 @@ aligned_pack_kernel(const int *__restrict__ rp, const int *__restrict__ ci,
 
 // kF32 = false: x is one T-bit word per block column, out = counts / bits.
 // kF32 = true:  x is K floats, out = y (FP32).
+// L lanes per block row (L <= 64: 64/L block rows per wave; L = 256: one
+// block row per workgroup, the waves' partial sums meet in LDS). Each lane
+// takes U groups per round with all their word and column loads in flight.
 template <int T, int L, bool kF32>
 __global__ void __launch_bounds__(kAThreads)
 aligned_spmv_kernel(const int *__restrict__ nrp, const int *__restrict__ nci,
                     const uint32_t *__restrict__ nval, const void *__restrict__ xv, int M, int MB,
                     int32_t *__restrict__ y_count, typename AWord<T>::type *__restrict__ y_bin,
                     float *__restrict__ y) {
-  constexpr int GS = 32 / T, GPP = L / T;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int sub = lane / L, sl = lane % L, j = sl % T, gl = sl / T;
-  const int br = (blockIdx.x * kAThreads + threadIdx.x) / kWave * (kWave / L) + sub;
+  static_assert(L <= kWave || L == kAThreads, "a block row spans part of a wave or a workgroup");
+  constexpr int GS = 32 / T, GPP = L / T, U = 4;
+  constexpr int WL = L < kWave ? L : kWave;  // a subgroup's lanes inside one wave
+  using W = typename AWord<T>::type;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int sl = tid % L, j = sl % T, gl = sl / T;
+  const int br = blockIdx.x * (kAThreads / L) + tid / L;
   const bool live = br < MB;
   int cnt = 0;
   float acc = 0.f;
   if (live) {
     const int g1 = nrp[br + 1] / GS;
-    for (int g = nrp[br] / GS + gl; g < g1; g += GPP) {
-      const uint32_t a = nval[static_cast<int64_t>(g) * T + j];
-      if constexpr (!kF32) {
-        const auto *x = static_cast<const typename AWord<T>::type *>(xv);
-        uint32_t xw = 0;
+    for (int g = nrp[br] / GS + gl; g < g1; g += GPP * U) {
+      // Every slot loads from a valid group (past the end: the block row's
+      // last, whose word is then dropped), so no load sits behind a branch
+      // and all of them are in flight together.
+      uint32_t a[U];
+      int col[U][GS];
 #pragma unroll
-        for (int k = 0; k < GS; ++k)
-          xw |= static_cast<uint32_t>(x[nci[g * GS + k]]) << (32 - T * (k + 1));
-        cnt += __popc(a & xw);
+      for (int u = 0; u < U; ++u) {
+        const int gg = min(g + u * GPP, g1 - 1);
+        a[u] = nval[static_cast<int64_t>(gg) * T + j];
+#pragma unroll
+        for (int k = 0; k < GS; ++k) col[u][k] = nci[gg * GS + k];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (g + u * GPP >= g1) a[u] = 0u;
+      if constexpr (!kF32) {
+        const W *x = static_cast<const W *>(xv);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          uint32_t xw = 0;
+#pragma unroll
+          for (int k = 0; k < GS; ++k)
+            xw |= static_cast<uint32_t>(x[col[u][k]]) << (32 - T * (k + 1));
+          cnt += __popc(a[u] & xw);
+        }
       } else {
+        // Each round takes the highest remaining set bit of every word, so
+        // the U gathers of a round go out together; rounds repeat while a
+        // word has bits left (most words hold none or one).
         const float *x = static_cast<const float *>(xv);
-        uint32_t bits = a;
-        while (bits) {
-          const int b = 31 - __clz(bits);
-          bits &= ~(1u << b);
-          const int k = (31 - b) / T, c = T - 1 - (b % T);
-          acc += x[static_cast<int64_t>(nci[g * GS + k]) * T + c];
+        bool more = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) more = more || a[u] != 0;
+        while (more) {
+          float v[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int b = a[u] ? 31 - __clz(a[u]) : 31;
+            const int k = (31 - b) / T, c = T - 1 - (b % T);
+            int ck = col[u][0];
+#pragma unroll
+            for (int q = 1; q < GS; ++q) ck = k == q ? col[u][q] : ck;
+            v[u] = x[static_cast<int64_t>(ck) * T + c];
+          }
+          more = false;
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if (a[u]) {
+              acc += v[u];
+              a[u] &= ~(1u << (31 - __clz(a[u])));
+            }
+            more = more || a[u] != 0;
+          }
         }
       }
     }
   }
-  // lanes of one subgroup that share j differ in the bits at T, 2T, ... L/2
+  // lanes of one wave that share j differ in the bits at T, 2T, ... WL/2
 #pragma unroll
-  for (int o = T; o < L; o <<= 1) {
+  for (int o = T; o < WL; o <<= 1) {
     if constexpr (!kF32) cnt += __shfl_xor(cnt, o, kWave);
     else acc += __shfl_xor(acc, o, kWave);
   }
+  if constexpr (L > kWave) {  // one block row per workgroup: add the waves' sums
+    __shared__ int red_i[kAThreads / kWave][T];
+    __shared__ float red_f[kAThreads / kWave][T];
+    if (lane < T) {
+      if constexpr (!kF32) red_i[tid / kWave][lane] = cnt;
+      else red_f[tid / kWave][lane] = acc;
+    }
+    __syncthreads();
+    if (tid < T) {
+#pragma unroll
+      for (int w = 1; w < kAThreads / kWave; ++w) {
+        if constexpr (!kF32) cnt += red_i[w][tid];
+        else acc += red_f[w][tid];
+      }
+    }
+    if (tid >= kWave) return;  // wave 0 writes the block row
+  }
   const int row = br * T + j;
+  const bool writer = live && sl < T;  // one lane per row
   if constexpr (!kF32) {
-    const uint64_t nz = __ballot(live && gl == 0 && cnt > 0);
-    if (live && gl == 0 && y_count && row < M) y_count[row] = cnt;
+    const uint64_t nz = __ballot(writer && cnt > 0);
+    if (writer && y_count && row < M) y_count[row] = cnt;
     if (live && sl == 0 && y_bin) {
-      const uint32_t m = static_cast<uint32_t>(nz >> (sub * L)) & ((1u << T) - 1);
-      y_bin[br] = static_cast<typename AWord<T>::type>(__brev(m) >> (32 - T));
+      const int shift = lane / WL * WL;
+      const uint32_t m = static_cast<uint32_t>(nz >> shift) & ((1u << T) - 1);
+      y_bin[br] = static_cast<W>(__brev(m) >> (32 - T));
     }
   } else {
-    if (live && gl == 0 && row < M) y[row] = acc;
+    if (writer && row < M) y[row] = acc;
   }
 }
 
@@ -158,18 +220,24 @@ void launch_spmv(int L, const int *nrp, const int *nci, const uint32_t *nval, co
     case 8: if constexpr (T <= 8) { SPMM_ASPMV(8); break; } [[fallthrough]];
     case 16: SPMM_ASPMV(16); break;
     case 32: SPMM_ASPMV(32); break;
-    default: SPMM_ASPMV(64); break;
+    case 64: SPMM_ASPMV(64); break;
+    default: SPMM_ASPMV(256); break;
   }
 #undef SPMM_ASPMV
 }
 
-// Lanes per block row: enough to give each lane about one group per pass.
-int lanes_for(int T, int MB, int nnzb) {
+// Lanes per block row. Bit x: enough that U = 4 rounds cover the average
+// block row, 128 rounded up to a whole workgroup (measured on a 89k-row
+// power-law graph: 8.5 us at T = 8 with 64 lanes, 14.4 us at T = 16 with a
+// workgroup). FP32 x: always a workgroup -- its set bits are gathered one
+// round each, and the dense words of hub block rows set the critical path.
+int lanes_for(int T, int MB, int nnzb, bool f32) {
+  if (f32) return kAThreads;
   const int GS = 32 / T;
   const double groups = MB > 0 ? static_cast<double>(nnzb) / GS / MB : 0.0;
   int L = T;
-  while (L < kWave && L / T < groups) L *= 2;
-  return L;
+  while (L < kAThreads && 4.0 * L / T < groups) L *= 2;
+  return L > kWave ? kAThreads : L;
 }
 
 template <bool kF32>
@@ -187,7 +255,7 @@ int run_spmv(spmm_handle_t h, int T, int M, int K, int nnzb, const int *nrp, con
                  "aligned B2SR above 2^31 words");
   if (MB == 0) return SPMM_OK;
   hipStream_t s = as_stream(stream);
-  const int L = lanes_for(T, MB, nnzb);
+  const int L = lanes_for(T, MB, nnzb, kF32);
   if (T == 8) launch_spmv<8, kF32>(L, nrp, nci, nval, x, M, MB, y_count, y_bin, y, s);
   else launch_spmv<16, kF32>(L, nrp, nci, nval, x, M, MB, y_count, y_bin, y, s);
   SPMM_LAUNCHED();

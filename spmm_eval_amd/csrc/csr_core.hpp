@@ -467,7 +467,7 @@ __device__ __forceinline__ void sddmm_row_wg(const LaunchTable &tab, int wg,
 
 // Host side.
 struct Geometry {
-  int G = 0;    // lanes per row (0 = scalar kernel)
+  int G = 0;    // lanes per row (0 = narrow kernel)
   int VEC = 1;  // 16-byte vectors per lane
   int col_tiles = 1;
   int rows_scale = 1;

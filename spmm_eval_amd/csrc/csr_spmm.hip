@@ -98,23 +98,106 @@ csr_spmm_perm_kernel(LaunchTable tab, const int *__restrict__ rows,
                            ldc, perm);
 }
 
-// Any N, any alignment: one wave per row, lanes stride over the columns.
-template <typename TB>
+// Narrow or unaligned N (N not a multiple of one 16-byte vector, or B / C /
+// ld not 16-byte aligned): a row's entries are split across lanes rather
+// than its columns. A subgroup is NC column lanes (a power of two covering
+// min(N, 64); wider N loops over column chunks) times ES entry slots, and
+// the plan's row classes set ES: short rows a subgroup of NC x
+// min(4, 64 / NC) lanes each, medium rows a wave (ES = 64 / NC), long rows a
+// workgroup (ES = 256 / NC) whose waves meet in LDS. A lane takes
+// kNarrowU entries a round, all loads from clamped (in range) indices, so a
+// round's gathers are in flight together; slots past the row's end are
+// dropped before the FMA. Sums: per lane in entry order, then a butterfly
+// over the slots, then the waves in order -- the same for every call.
+constexpr int kNarrowU = 4;
+
+template <typename TB, int NC, int ES, bool kPerm>
+__device__ __forceinline__ float narrow_range(int e0, int e1, int es, int cc,
+                                              const int *__restrict__ colind,
+                                              const float *__restrict__ val,
+                                              const int *__restrict__ perm,
+                                              const TB *__restrict__ B, int ldb) {
+  float acc = 0.f;
+  for (int e = e0 + es; e < e1; e += ES * kNarrowU) {
+    int ci[kNarrowU];
+    float v[kNarrowU], b[kNarrowU];
+#pragma unroll
+    for (int u = 0; u < kNarrowU; ++u) {
+      const int eu = min(e + u * ES, e1 - 1);
+      ci[u] = colind[eu];
+      v[u] = val[kPerm ? perm[eu] : eu];
+    }
+#pragma unroll
+    for (int u = 0; u < kNarrowU; ++u)
+      b[u] = load_scalar<TB>(B + static_cast<size_t>(ci[u]) * ldb + cc);
+#pragma unroll
+    for (int u = 0; u < kNarrowU; ++u)
+      if (e + u * ES < e1) acc = fmaf(v[u], b[u], acc);
+  }
+  return acc;
+}
+
+// Entry slots of a class-A subgroup: 4, or all the wave has beside NC.
+template <int NC> constexpr int narrow_slots_a() { return kWave / NC < 4 ? kWave / NC : 4; }
+
+template <typename TB, int NC, bool kPerm>
 __global__ void __launch_bounds__(kThreads)
-csr_spmm_generic_kernel(int M, int N, const int *__restrict__ rowptr,
-                        const int *__restrict__ colind, const float *__restrict__ val,
-                        const TB *__restrict__ B, int ldb, float *__restrict__ C,
-                        int ldc, const int *__restrict__ perm) {
-  const int row = blockIdx.x * kWavesPerWG + threadIdx.x / kWave;
-  if (row >= M) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int e0 = rowptr[row], e1 = rowptr[row + 1];
-  for (int c = lane; c < N; c += kWave) {
-    float acc = 0.f;
-    for (int e = e0; e < e1; ++e)
-      acc = fmaf(val[perm ? perm[e] : e],
-                 load_scalar<TB>(B + static_cast<size_t>(colind[e]) * ldb + c), acc);
-    C[static_cast<size_t>(row) * ldc + c] = acc;
+csr_narrow_kernel(LaunchTable tab, const int *__restrict__ rows, const int2 *__restrict__ span,
+                  const int *__restrict__ colind, const float *__restrict__ val,
+                  const int *__restrict__ perm, const TB *__restrict__ B, int ldb,
+                  float *__restrict__ C, int ldc, int N) {
+  constexpr int ESA = narrow_slots_a<NC>();
+  constexpr int GA = NC * ESA, SA = kWave / GA;
+  __shared__ float red[kWavesPerWG][NC];
+  const Seg sg = find_seg(tab, blockIdx.x);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int local_wg = blockIdx.x - sg.wg_begin;
+  const int cl = lane % NC;
+  for (int c0 = 0; c0 < N; c0 += NC) {
+    const int c = c0 + cl, cc = min(c, N - 1);
+    if (sg.kind == kSegA) {
+      // R rows per subgroup; every lane runs all R rounds (subgroups past the
+      // end take an empty span) so the shuffles stay wave-uniform.
+      const int R = sg.rows_per_sub;
+      const int sub = lane / GA, es = (lane % GA) / NC;
+      const int subs_per_wg = kWavesPerWG * SA;
+      const int end = sg.list_off + sg.count;
+      int pos = sg.list_off + local_wg * subs_per_wg * R + wave * SA + sub;
+      for (int k = 0; k < R; ++k, pos += subs_per_wg) {
+        const bool ok = pos < end;
+        const int2 sp = ok ? span[pos] : make_int2(0, 0);
+        float acc = narrow_range<TB, NC, ESA, kPerm>(sp.x, sp.y, es, cc, colind, val, perm, B, ldb);
+#pragma unroll
+        for (int o = NC; o < GA; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+        if (ok && es == 0 && c < N) C[static_cast<size_t>(rows[pos]) * ldc + c] = acc;
+      }
+    } else if (sg.kind == kSegB) {
+      const int pos = sg.list_off + local_wg * kWavesPerWG + wave;
+      if (pos >= sg.list_off + sg.count) return;  // whole wave, no barrier below
+      const int2 sp = span[pos];
+      const int es = lane / NC;
+      float acc = narrow_range<TB, NC, kWave / NC, kPerm>(sp.x, sp.y, es, cc, colind, val, perm,
+                                                          B, ldb);
+#pragma unroll
+      for (int o = NC; o < kWave; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+      if (es == 0 && c < N) C[static_cast<size_t>(rows[pos]) * ldc + c] = acc;
+    } else {
+      const int pos = sg.list_off + local_wg;
+      const int2 sp = span[pos];
+      const int es = threadIdx.x / NC;
+      float acc = narrow_range<TB, NC, kThreads / NC, kPerm>(sp.x, sp.y, es, cc, colind, val,
+                                                             perm, B, ldb);
+#pragma unroll
+      for (int o = NC; o < kWave; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+      if (lane < NC) red[wave][lane] = acc;
+      __syncthreads();
+      if (wave == 0 && lane < NC && c < N) {
+#pragma unroll
+        for (int w = 1; w < kWavesPerWG; ++w) acc += red[w][lane];
+        C[static_cast<size_t>(rows[pos]) * ldc + c] = acc;
+      }
+      __syncthreads();  // red[] is reused by the next column chunk
+    }
   }
 }
 
@@ -127,7 +210,7 @@ static bool tune_valid(int N, int e16, int G, int VEC) {
 
 // Lane geometry for one call: an explicit tune, else the plan's autotuned
 // entry for (N, dtype), else the widest power-of-two subgroup that tiles N.
-// G == 0 selects the scalar kernel (unaligned operands or N % e16 != 0).
+// G == 0 selects the narrow kernel (unaligned operands or N % e16 != 0).
 static Geometry pick_geometry(const spmm_csr_plan_s *p, int N, int elem_bytes, const void *B,
                               int ldb, const void *C, int ldc, const spmm_tune_t *tune) {
   Geometry g;
@@ -230,6 +313,40 @@ static void dispatch_G(int G, const LaunchTable &t, int col_tiles,
 #undef SPMM_LV
 }
 
+template <typename TB, int NC>
+static void launch_narrow_nc(const spmm_csr_plan_s *p, int N, const int *colind,
+                             const float *val, const int *perm, const TB *B, int ldb, float *C,
+                             int ldc, hipStream_t s) {
+  constexpr int ESA = narrow_slots_a<NC>();
+  LaunchTable t;
+  // class A: rows that fit one round of a subgroup; class C: rows needing
+  // more than 8 rounds of a wave
+  build_table_ex(p->counts, p->offs, kWave / (NC * ESA), ESA * kNarrowU,
+                 static_cast<long long>(kWave / NC) * kNarrowU * 8, 8, 16, 1, &t);
+  if (t.total_wg == 0) return;
+  if (perm)
+    hipLaunchKernelGGL((csr_narrow_kernel<TB, NC, true>), dim3(t.total_wg), dim3(kThreads), 0, s,
+                       t, p->d_rows, p->d_span, colind, val, perm, B, ldb, C, ldc, N);
+  else
+    hipLaunchKernelGGL((csr_narrow_kernel<TB, NC, false>), dim3(t.total_wg), dim3(kThreads), 0,
+                       s, t, p->d_rows, p->d_span, colind, val, perm, B, ldb, C, ldc, N);
+}
+
+template <typename TB>
+static void launch_narrow(const spmm_csr_plan_s *p, int N, const int *colind, const float *val,
+                          const int *perm, const TB *B, int ldb, float *C, int ldc,
+                          hipStream_t s) {
+#define SPMM_LN(nc) launch_narrow_nc<TB, nc>(p, N, colind, val, perm, B, ldb, C, ldc, s)
+  if (N <= 1) SPMM_LN(1);
+  else if (N <= 2) SPMM_LN(2);
+  else if (N <= 4) SPMM_LN(4);
+  else if (N <= 8) SPMM_LN(8);
+  else if (N <= 16) SPMM_LN(16);
+  else if (N <= 32) SPMM_LN(32);
+  else SPMM_LN(64);
+#undef SPMM_LN
+}
+
 // perm != nullptr: entry e's value is val[perm[e]] (transposed structure).
 template <typename TB>
 static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
@@ -249,8 +366,7 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
   hipStream_t s = as_stream(stream);
   Geometry g = pick_geometry(p, N, sizeof(TB), B, ldb, C, ldc, tune);
   if (g.G == 0) {
-    hipLaunchKernelGGL((csr_spmm_generic_kernel<TB>), dim3(ceil_div(M, kWavesPerWG)),
-                       dim3(kThreads), 0, s, M, N, rowptr, colind, val, B, ldb, C, ldc, perm);
+    launch_narrow<TB>(p, N, colind, val, perm, B, ldb, C, ldc, s);
     SPMM_LAUNCHED();
     return SPMM_OK;
   }

@@ -208,7 +208,8 @@ def test_sddmm_planned_mismatch(handle, dev):
 @pytest.mark.parametrize("M,K,dens,nl,ll,N", [
     (2000, 1500, 0.004, 3, 1500, 128),
     (777, 3000, 0.003, 2, 3000, 64),
-    (300, 200, 0.05, 2, 200, 7),        # generic kernel
+    (300, 200, 0.05, 2, 200, 7),        # narrow kernel
+    (400, 3000, 0.01, 2, 3000, 1),      # narrow kernel, SpMV shape
     (1, 5, 1.0, 0, 0, 4),
 ])
 def test_spmm_csr_t(handle, dev, M, K, dens, nl, ll, N):

@@ -48,7 +48,7 @@ def check_f32(g, B, C):
 CASES = [
     # (M, K, density, long_rows, long_len, N)
     (1, 1, 1.0, 0, 0, 1),
-    (50, 40, 0.2, 0, 0, 3),       # N not a multiple of 4 -> generic kernel
+    (50, 40, 0.2, 0, 0, 3),       # N not a multiple of 4 -> narrow kernel
     (300, 200, 0.05, 2, 200, 4),
     (513, 700, 0.02, 1, 700, 8),
     (1000, 1000, 0.01, 3, 1000, 32),
@@ -114,11 +114,57 @@ def test_csr_tune_variants(handle, dev, tune):
     check_f32(g, B, C.cpu().numpy())
 
 
+NARROW = [
+    # (M, K, density, long_rows, long_len, N): every row class of the narrow
+    # kernel (subgroup, wave, workgroup rows of 3000-5000 entries) and each
+    # column-lane width, including N > 64 (column chunks)
+    (500, 5000, 0.002, 3, 5000, 1),
+    (600, 3000, 0.004, 4, 3000, 2),
+    (400, 5000, 0.003, 2, 5000, 5),
+    (300, 2000, 0.01, 2, 2000, 7),
+    (300, 1000, 0.01, 2, 1000, 17),
+    (200, 3000, 0.005, 2, 3000, 33),
+    (150, 800, 0.02, 1, 800, 100),
+]
+
+
+@pytest.mark.parametrize("M,K,dens,nl,ll,N", NARROW)
+def test_csr_narrow_widths(handle, dev, M, K, dens, nl, ll, N):
+    g = graphs.random_csr(M, K, dens, seed=3 * M + N, long_rows=nl, long_len=ll)
+    B = graphs.uniform_pm1((K, N), seed=N + 1)
+    rp, ci, v = to_dev(g, dev)
+    Bd = torch.from_numpy(B).to(dev)
+    C = api.spmm_csr(handle, rp, ci, v, Bd).cpu().numpy()
+    check_f32(g, B, C)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    Cp = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan).cpu().numpy()
+    assert np.array_equal(Cp, C)  # same schedule, same sums
+    Bh = api.to_bf16_bits(Bd)
+    Ch = api.spmm_csr(handle, rp, ci, v, Bh, plan=plan).cpu().numpy()
+    refq = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, Bh.view(torch.int16).cpu().numpy().view(np.uint16))
+    assert rel_fro(Ch, refq) <= 1e-5
+
+
+@pytest.mark.parametrize("N", [1, 3])
+def test_csr_narrow_workload_graph(handle, dev, N):
+    """SpMV-shaped products on the 89k-row power-law graph (hub rows of
+    ~900 entries): f32 bound, and +-1 B with unit values bit-exact."""
+    g = graphs.workload_b2sr_graph()
+    rp, ci, v = to_dev(g, dev)
+    B = graphs.uniform_pm1((g.K, N), seed=7 + N)
+    C = api.spmm_csr(handle, rp, ci, v, torch.from_numpy(B).to(dev)).cpu().numpy()
+    check_f32(g, B, C)
+    S = np.random.default_rng(N).choice([-1, 1], size=(g.K, N)).astype(np.int32)
+    ones = torch.ones(g.nnz, device=dev)
+    Cs = api.spmm_csr(handle, rp, ci, ones, torch.from_numpy(S.astype(np.float32)).to(dev)).cpu().numpy()
+    np.testing.assert_array_equal(Cs, oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.float32))
+
+
 def test_csr_strided_and_empty(handle, dev):
     g = graphs.random_csr(300, 200, 0.05, seed=1)
     rp, ci, v = to_dev(g, dev)
     Bfull = torch.from_numpy(graphs.uniform_pm1((200, 70), seed=2)).to(dev)
-    B = Bfull[:, :64]  # ldb = 70: not 16-byte aligned rows -> generic kernel
+    B = Bfull[:, :64]  # ldb = 70: not 16-byte aligned rows -> narrow kernel
     Cfull = torch.full((300, 80), 7.0, device=dev)
     api.spmm_csr(handle, rp, ci, v, B, C_out=Cfull[:, :64])
     check_f32(g, B.cpu().numpy(), Cfull[:, :64].cpu().numpy())
