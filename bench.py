@@ -493,11 +493,11 @@ def main():
     hd = headline(args, h, dev)
     gbs = hd["nbytes"] / (hd["ms_step"] * 1e-3) / 1e9
     ach = hd["nbytes"] / (hd["ms_launch"] * 1e-3) / 1e9
-    if hd["T"] <= 8:  # block-row kernel; B read through a buffer resource below 2 GiB
-        kbuf = "true" if hd["g"].K * hd["N"] * 4 < 2 ** 31 else "false"
+    kbuf = "true" if hd["g"].K * hd["N"] * 4 < 2 ** 31 else "false"  # B through a buffer resource
+    if hd["T"] <= 8:  # block-row kernels: narrow tiles, wide tiles
         kname = f"b2sr_brow_kernel<{hd['T']}, float, {b2sr_lanes(hd['N'])}, {kbuf}>"
     else:
-        kname = f"b2sr_planned_kernel<{hd['T']}, float, {b2sr_lanes(hd['N'])}>"
+        kname = f"b2sr_wide_kernel<{hd['T']}, float, {min(32, b2sr_lanes(hd['N']))}, {kbuf}>"
     traffic = profiled_traffic(f"b2sr_fbf_t{hd['T']}_n{hd['N']}_89k.md", kname)
     result = {
         "metric": "SpMM HBM GB/s (algorithmic) -- B2SR-4 FBF SpMM, FP32 B",

@@ -314,12 +314,8 @@ b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
   }
 }
 
-// Planned FBF SpMM for T = 16 and 32: rows binned by their block row's tile
-// count (the plan of spmm_b2sr_analyze) run in one launch with the CSR classes: short rows
-// one subgroup per row (several rows per subgroup, the next row's tile range
-// requested before this row's gathers), medium rows one wave per row (its
-// subgroups take tile chunks in turn, butterfly reduction), hub rows one
-// workgroup per row (LDS reduction in a fixed order).
+// Row r's share of tiles [t, t1) taken in chunks at `stride`: each chunk's
+// bits of row r decoded into the subgroup's list, then 8 gathers in flight.
 template <int T, typename TB, int G, class LD>
 __device__ __forceinline__ void b2sr_range(int t, int t1, int stride, int r, int lane_sub,
                                            const int *__restrict__ bci,
@@ -349,6 +345,12 @@ __device__ __forceinline__ void b2sr_range(int t, int t1, int stride, int r, int
   }
 }
 
+// Planned FBF SpMM for T = 16: rows binned by their block row's tile count
+// run in one launch with the CSR classes: short rows one subgroup per row
+// (several rows per subgroup, the next row's tile range requested before
+// this row's gathers), medium rows one wave per row (its subgroups take tile
+// chunks in turn, butterfly reduction), hub rows one workgroup per row (LDS
+// reduction in a fixed order). Each row decodes only its own tile rows.
 template <int T, typename TB, int G>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
@@ -888,6 +890,154 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
     if (col0 + q < N) Crow[col0 + q] = 2 * cnt[q] - deg;
 }
 
+// ------------------------------ block-row FBF, wide tiles (T = 32 and 64)
+// A tile is T words of T bits (128 or 512 bytes) and power-law graphs
+// leave about one bit per tile, so the words, not the gathers, are most of
+// the bytes. One workgroup per block row (the plan's block-row order):
+// chunks of tiles are loaded with coalesced 16-byte loads into LDS (each tile
+// padded by 16 bytes, so the strided reads of one tile row across tiles hit
+// distinct banks), then each subgroup decodes the rows it owns (rows gsub,
+// gsub + SW, ...; RPS of them) from LDS: a subgroup scan of the lanes' bit
+// counts places (B row << 3 | local row) entries in the subgroup's staging
+// list (16 G entries, 16 KB per workgroup), which is streamed with 8 gathers
+// in flight into per-row accumulators kept across chunks (windows if a
+// chunk's entries exceed the list). Every
+// word is read from HBM once and fully coalesced.
+template <int T> struct WideChunk {
+  using Wd = typename Word<T>::type;
+  static constexpr int kTileBytes = T * T / 8;
+  static constexpr int kSlot = kTileBytes + 16;  // padded LDS tile slot
+  static constexpr int kTiles = T == 32 ? 256 : 64;
+  static constexpr int kVecs = kTiles * kTileBytes / 16;  // 16-byte loads per chunk
+  static constexpr int kLoads = kVecs / kB2Threads;       // per thread
+  static_assert(kVecs % kB2Threads == 0, "whole loads per thread");
+};
+
+template <int T, typename TB, int G, bool kBuf>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span, int M,
+                 const int *__restrict__ bci, const typename Word<T>::type *__restrict__ words,
+                 const TB *__restrict__ B, int ldb, int b_bytes, float *__restrict__ C, int ldc) {
+  using Wd = typename Word<T>::type;
+  using WC = WideChunk<T>;
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int S = kWave / G, SW = kB2Waves * S;
+  constexpr int RPS = T >= SW ? T / SW : 1;  // rows per subgroup
+  constexpr int kStage = 16 * G;             // staged entries per subgroup
+  static_assert(RPS <= 8, "local row index takes 3 bits");
+  __shared__ __align__(16) unsigned char wbuf[WC::kTiles * WC::kSlot];
+  __shared__ int bcs[WC::kTiles];
+  __shared__ int stage[SW][kStage];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int lane_sub = lane & (G - 1), sub = lane / G, gsub = wave * S + sub;
+  const int col0 = blockIdx.y * G * EPL + lane_sub * EPL;
+  const RowLoader<TB, kBuf> ld(B, ldb, col0, b_bytes);
+  const int br = brows[blockIdx.x];
+  const int2 sp = span[blockIdx.x];
+  const bool owner = gsub < T;  // subgroups past T rows (SW > T) only load
+  int *st = stage[gsub];
+  float acc[RPS][EPL];
+#pragma unroll
+  for (int i = 0; i < RPS; ++i)
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[i][q] = 0.f;
+
+  for (int tc = sp.x; tc < sp.y; tc += WC::kTiles) {
+    const int nt = min(WC::kTiles, sp.y - tc);
+    const int nvec = nt * WC::kTileBytes / 16;
+    const uint4 *src = reinterpret_cast<const uint4 *>(words + static_cast<int64_t>(tc) * T);
+    uint4 v[WC::kLoads];
+#pragma unroll
+    for (int k = 0; k < WC::kLoads; ++k) {
+      const int q = k * kB2Threads + tid;
+      v[k] = q < nvec ? src[q] : make_uint4(0, 0, 0, 0);
+    }
+    for (int k = tid; k < nt; k += kB2Threads) bcs[k] = bci[tc + k];
+#pragma unroll
+    for (int k = 0; k < WC::kLoads; ++k) {
+      const int q = k * kB2Threads + tid;
+      if (q < nvec) {
+        const int byte = q * 16;
+        *reinterpret_cast<uint4 *>(wbuf + byte / WC::kTileBytes * WC::kSlot +
+                                   byte % WC::kTileBytes) = v[k];
+      }
+    }
+    __syncthreads();
+    if (owner) {
+      // this lane's tiles: lane_sub, lane_sub + G, ...; its bit count over
+      // the subgroup's rows
+      int cnt = 0;
+      for (int t = lane_sub; t < nt; t += G) {
+#pragma unroll
+        for (int i = 0; i < RPS; ++i) {
+          const int r = gsub + i * SW;
+          cnt += popcount(*reinterpret_cast<const Wd *>(wbuf + t * WC::kSlot + r * sizeof(Wd)));
+        }
+      }
+      int inc = cnt;
+#pragma unroll
+      for (int d = 1; d < G; d <<= 1) {
+        const int y = __shfl_up(inc, d, G);
+        if (lane_sub >= d) inc += y;
+      }
+      const int total = __shfl(inc, G - 1, G);
+      for (int base = 0; base < total; base += kStage) {
+        int pos = inc - cnt - base;
+        for (int t = lane_sub; t < nt && pos < kStage; t += G) {
+          const int b0 = bcs[t] * T + T - 1;
+#pragma unroll
+          for (int i = 0; i < RPS; ++i) {
+            Wd x = *reinterpret_cast<const Wd *>(wbuf + t * WC::kSlot +
+                                                 (gsub + i * SW) * sizeof(Wd));
+            while (x) {
+              const int b = top_bit(x);
+              x ^= static_cast<Wd>(1) << b;
+              if (pos >= 0 && pos < kStage) st[pos] = ((b0 - b) << 3) | i;
+              ++pos;
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int n = min(kStage, total - base);
+        for (int k = 0; k < n; k += 8) {
+          typename Vec16<TB>::Raw raw[8];
+          uint32_t rows = 0;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int e = st[k + u < n ? k + u : k];
+            rows |= static_cast<uint32_t>(e & 7) << (3 * u);
+            raw[u] = ld(e >> 3);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (k + u < n) {
+              float b[EPL];
+              Vec16<TB>::widen(raw[u], b);
+              const int i = (rows >> (3 * u)) & 7;
+#pragma unroll
+              for (int ii = 0; ii < RPS; ++ii)
+                if (ii == i) {
+#pragma unroll
+                  for (int q = 0; q < EPL; ++q) acc[ii][q] += b[q];
+                }
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();  // the staging list is rewritten next
+      }
+    }
+    __syncthreads();  // wbuf and bcs are rewritten by the next chunk
+  }
+  if (!owner) return;
+#pragma unroll
+  for (int i = 0; i < RPS; ++i) {
+    const int row = br * T + gsub + i * SW;
+    if (gsub + i * SW < T && row < M) store_row<EPL>(C, ldc, row, col0, acc[i]);
+  }
+}
+
 // ------------------------------------------------------- dense packers
 // One wave per (row, 64-column chunk): a 64-bit ballot gives two words.
 // cols == false: out[h][w] (row-major, Wc words per row);
@@ -1131,6 +1281,37 @@ static void dispatch_brow(const Geo &g, const spmm_csr_plan_s *p, int M, int K, 
   }
 }
 
+template <int T, typename TB, int G>
+static void launch_wide(const spmm_csr_plan_s *p, int K, const int *bci, const void *val,
+                        const TB *B, int ldb, float *C, int ldc, int tiles, int M,
+                        hipStream_t s) {
+  const int MB = (M + T - 1) / T;
+  const auto *w = static_cast<const typename Word<T>::type *>(val);
+  const int b_bytes = buffer_span(K, ldb, sizeof(TB));
+  if (b_bytes)
+    hipLaunchKernelGGL((b2sr_wide_kernel<T, TB, G, true>), dim3(MB, tiles), dim3(kB2Threads), 0,
+                       s, p->d_rows, p->d_span, M, bci, w, B, ldb, b_bytes, C, ldc);
+  else
+    hipLaunchKernelGGL((b2sr_wide_kernel<T, TB, G, false>), dim3(MB, tiles), dim3(kB2Threads),
+                       0, s, p->d_rows, p->d_span, M, bci, w, B, ldb, 0, C, ldc);
+}
+
+// G = 64 runs as G = 32 over twice the column tiles (eight rows per subgroup
+// at most).
+template <int T, typename TB>
+static void dispatch_wide(const Geo &g, const spmm_csr_plan_s *p, int M, int K, const int *bci,
+                          const void *val, const TB *B, int ldb, float *C, int ldc,
+                          hipStream_t s) {
+  switch (g.G) {
+    case 1: launch_wide<T, TB, 1>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
+    case 2: launch_wide<T, TB, 2>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
+    case 4: launch_wide<T, TB, 4>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
+    case 8: launch_wide<T, TB, 8>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
+    case 16: launch_wide<T, TB, 16>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
+    default: launch_wide<T, TB, 32>(p, K, bci, val, B, ldb, C, ldc, g.tiles * (g.G / 32), M, s); break;
+  }
+}
+
 template <typename TB>
 static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const int *brp,
                            const int *bci, const void *val, const TB *B, int ldb, float *C,
@@ -1147,8 +1328,10 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
   // T = 2 / 4 / 8 tiles are read as one 16 / 32 / 64-bit word; list entries keep
   // the B row in 27 bits
   SPMM_CHECK_ARG(T > 8 || reinterpret_cast<uintptr_t>(val) % T == 0,
-                 "bsrVal must be aligned to T bytes for T = 4 and 8");
-  SPMM_CHECK_ARG(T > 8 || K < (1 << 27), "planned T <= 8 needs K < 2^27");
+                 "bsrVal must be aligned to T bytes for T = 2, 4 and 8");
+  SPMM_CHECK_ARG(T <= 8 || reinterpret_cast<uintptr_t>(val) % 16 == 0,
+                 "bsrVal must be 16-byte aligned for T = 16, 32 and 64");
+  SPMM_CHECK_ARG(K < (1 << 27), "planned B2SR needs K < 2^27");
   hipStream_t s = as_stream(stream);
   const Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
   if (g.G == 0) {  // scalar path needs no schedule
@@ -1170,8 +1353,8 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 16: dispatch_planned<16, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 32: dispatch_planned<32, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
-    default: dispatch_planned<64, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 32: dispatch_wide<32, TB>(g, p, M, K, bci, val, B, ldb, C, ldc, s); break;
+    default: dispatch_wide<64, TB>(g, p, M, K, bci, val, B, ldb, C, ldc, s); break;
   }
   SPMM_LAUNCHED();
   return SPMM_OK;
@@ -1269,15 +1452,14 @@ int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr, spmm_
   SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M > 0, "M must be positive");
   int st;
-  if (T <= 8) {
-    // block rows binned by tile count (b2sr_brow_kernel's units)
+  if (T != 16) {
+    // block rows binned by tile count (the units of b2sr_brow_kernel and
+    // b2sr_wide_kernel), heaviest first
     st = analyze(h, (M + T - 1) / T, 0, bsrRowPtr, 0, plan, stream);
     if (st == SPMM_OK) (*plan)->M = M;
   } else {
     // rows binned by their block row's tile count (b2sr_planned_kernel's units)
-    int shift = 0;
-    while ((1 << shift) < T) ++shift;
-    st = analyze(h, M, 0, bsrRowPtr, shift, plan, stream);
+    st = analyze(h, M, 0, bsrRowPtr, 4, plan, stream);
   }
   if (st == SPMM_OK) (*plan)->tile = T;
   return st;
