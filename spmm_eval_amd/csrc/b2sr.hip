@@ -814,28 +814,100 @@ b2sr_brow_kernel(LaunchTable tab, const int *__restrict__ brows, int M,
                           C, ldc);
 }
 
-// Any N / alignment: one wave per output row, lanes stride over columns.
-template <int T, typename TB>
+// Narrow or unaligned N (N not a multiple of one 16-byte vector, or B / C /
+// ld not 16-byte aligned; N = 1 is a binary-A SpMV): L lanes per block row
+// (64, or a workgroup for T >= 16), lane (slot, j) keeps tile row j and takes
+// the block row's tiles slot, slot + L/T, ... four at a time (word and block
+// column loads clamped to a valid tile, so they are in flight together).
+// Each round takes the highest remaining set bit of each of the four words
+// and gathers its NB columns of B together; a butterfly over the slots (and
+// LDS across waves) sums each row, slot 0 stores. Wider N runs in passes of
+// NB columns.
+template <int T, int L, int NB, typename TB>
 __global__ void __launch_bounds__(kB2Threads)
-b2sr_spmm_generic_kernel(int M, int N, const int *__restrict__ brp, const int *__restrict__ bci,
-                         const typename Word<T>::type *__restrict__ words,
-                         const TB *__restrict__ B, int ldb, float *__restrict__ C, int ldc) {
-  const int row = blockIdx.x * kB2Waves + threadIdx.x / kWave;
-  if (row >= M) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int br = row / T, r = row % T;
-  for (int c = lane; c < N; c += kWave) {
-    float acc = 0.f;
-    for (int t = brp[br]; t < brp[br + 1]; ++t) {
-      Bits<T> w = words[static_cast<size_t>(t) * T + r];
-      const int base = bci[t] * T + T - 1;
-      while (w) {
-        const int p = top_bit(w);
-        w ^= static_cast<Bits<T>>(1) << p;
-        acc += load_scalar<TB>(B + static_cast<size_t>(base - p) * ldb + c);
+b2sr_narrow_kernel(int M, int N, const int *__restrict__ brp, const int *__restrict__ bci,
+                   const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
+                   int ldb, float *__restrict__ C, int ldc) {
+  static_assert(L == kWave || L == kB2Threads, "a wave or a workgroup per block row");
+  constexpr int SLOTS = L / T, U = 4;
+  constexpr int WL = L < kWave ? L : kWave;
+  __shared__ float red[kB2Waves][T][NB];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int sl = tid % L, j = sl % T, slot = sl / T;
+  const int MB = (M + T - 1) / T;
+  const int br = blockIdx.x * (kB2Threads / L) + tid / L;
+  const bool live = br < MB;
+  int t0 = 0, t1 = 0;
+  if (live) {
+    t0 = brp[br];
+    t1 = brp[br + 1];
+  }
+  const int row = br * T + j;
+  for (int n0 = 0; n0 < N; n0 += NB) {
+    float acc[NB];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) acc[n] = 0.f;
+    for (int t = t0 + slot; t < t1; t += SLOTS * U) {
+      Bits<T> a[U];
+      int bc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int tt = min(t + u * SLOTS, t1 - 1);
+        a[u] = words[static_cast<size_t>(tt) * T + j];
+        bc[u] = bci[tt];
+      }
+      bool more = false;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (t + u * SLOTS >= t1) a[u] = 0;
+        more = more || a[u] != 0;
+      }
+      while (more) {
+        float v[U][NB];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int b = a[u] ? top_bit(a[u]) : T - 1;  // empty words read column 0
+          const TB *Brow = B + static_cast<size_t>(bc[u] * T + T - 1 - b) * ldb;
+#pragma unroll
+          for (int n = 0; n < NB; ++n) v[u][n] = load_scalar<TB>(Brow + min(n0 + n, N - 1));
+        }
+        more = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (a[u]) {
+#pragma unroll
+            for (int n = 0; n < NB; ++n) acc[n] += v[u][n];
+            a[u] ^= static_cast<Bits<T>>(1) << top_bit(a[u]);
+          }
+          more = more || a[u] != 0;
+        }
       }
     }
-    C[static_cast<size_t>(row) * ldc + c] = acc;
+#pragma unroll
+    for (int o = T; o < WL; o <<= 1) {
+#pragma unroll
+      for (int n = 0; n < NB; ++n) acc[n] += __shfl_xor(acc[n], o, kWave);
+    }
+    if constexpr (L > kWave) {  // the waves' partial sums meet in LDS
+      if (lane < T) {
+#pragma unroll
+        for (int n = 0; n < NB; ++n) red[tid / kWave][lane][n] = acc[n];
+      }
+      __syncthreads();
+      if (tid < T) {
+#pragma unroll
+        for (int w = 1; w < kB2Waves; ++w) {
+#pragma unroll
+          for (int n = 0; n < NB; ++n) acc[n] += red[w][tid][n];
+        }
+      }
+      __syncthreads();  // red[] is reused by the next pass
+    }
+    if (live && slot == 0 && (L <= kWave || tid < T) && row < M) {
+#pragma unroll
+      for (int n = 0; n < NB; ++n)
+        if (n0 + n < N) C[static_cast<size_t>(row) * ldc + n0 + n] = acc[n];
+    }
   }
 }
 
@@ -1131,16 +1203,36 @@ static void launch_fbf(int M, const int *brp, const int *bci, const void *val, c
                      state);
 }
 
+template <int T, int L, typename TB>
+static void launch_narrow_l(int M, int N, const int *brp, const int *bci, const void *val,
+                            const TB *B, int ldb, float *C, int ldc, hipStream_t s) {
+  const int MB = (M + T - 1) / T;
+  const dim3 grid(ceil_div(MB, kB2Threads / L));
+  const auto *w = static_cast<const typename Word<T>::type *>(val);
+#define SPMM_NW(nb)                                                                           \
+  hipLaunchKernelGGL((b2sr_narrow_kernel<T, L, nb, TB>), grid, dim3(kB2Threads), 0, s, M, N, \
+                     brp, bci, w, B, ldb, C, ldc)
+  if (N <= 1) SPMM_NW(1);
+  else if (N <= 4) SPMM_NW(4);
+  else SPMM_NW(8);
+#undef SPMM_NW
+}
+
+// A wave per block row for T <= 8, a workgroup for the larger tiles (their
+// block rows hold several times more tiles on power-law graphs).
+template <int T, typename TB>
+static void launch_narrow(int M, int N, const int *brp, const int *bci, const void *val,
+                          const TB *B, int ldb, float *C, int ldc, hipStream_t s) {
+  if (T <= 8) launch_narrow_l<T, kWave, TB>(M, N, brp, bci, val, B, ldb, C, ldc, s);
+  else launch_narrow_l<T, kB2Threads, TB>(M, N, brp, bci, val, B, ldb, C, ldc, s);
+}
+
 template <int T, typename TB>
 static void dispatch_fbf(const Geo &g, int M, int N, const int *brp, const int *bci,
                          const void *val, const TB *B, int ldb, float *C, int ldc,
                          int *state, int long_grid, hipStream_t s) {
   switch (g.G) {
-    case 0:
-      hipLaunchKernelGGL((b2sr_spmm_generic_kernel<T, TB>), dim3(ceil_div(M, kB2Waves)),
-                         dim3(kB2Threads), 0, s, M, N, brp, bci,
-                         static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc);
-      break;
+    case 0: launch_narrow<T, TB>(M, N, brp, bci, val, B, ldb, C, ldc, s); break;
     case 1: launch_fbf<T, TB, 1>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
     case 2: launch_fbf<T, TB, 2>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
     case 4: launch_fbf<T, TB, 4>(M, brp, bci, val, B, ldb, C, ldc, g.tiles, state, long_grid, s); break;
@@ -1334,7 +1426,7 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
   SPMM_CHECK_ARG(K < (1 << 27), "planned B2SR needs K < 2^27");
   hipStream_t s = as_stream(stream);
   const Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
-  if (g.G == 0) {  // scalar path needs no schedule
+  if (g.G == 0) {  // the narrow path needs no schedule
 #define SPMM_FBF(t) dispatch_fbf<t, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s)
     switch (T) {
       case 2: SPMM_FBF(2); break;
