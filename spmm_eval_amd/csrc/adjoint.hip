@@ -124,23 +124,66 @@ sddmm_kernel(int M, int nnz, const int *__restrict__ rowptr, const int *__restri
   sddmm_wg<TB, G>(blockIdx.x, M, nnz, rowptr, colind, dY, lddy, X, ldx, out, ntile);
 }
 
+// Narrow or unaligned N: one lane per stored entry, 64 consecutive entries a
+// wave. The wave finds the rows of its first and last entries by a uniform
+// binary search over rowptr; when those are fewer than 64 rows apart each
+// lane holds one of the rows' end offsets and every lane finds its row in
+// six shuffle steps, else each lane searches rowptr itself. The dot product
+// takes 8 columns at a time, all 16 loads in flight, summed in column order.
 template <typename TB>
 __global__ void __launch_bounds__(kAdjThreads)
-sddmm_generic_kernel(int M, int N, const int *__restrict__ rowptr, const int *__restrict__ colind,
-                     const TB *__restrict__ dY, int lddy, const TB *__restrict__ X, int ldx,
-                     float *__restrict__ out) {
-  const int row = blockIdx.x * 4 + threadIdx.x / kWave;
-  if (row >= M) return;
+sddmm_narrow_kernel(int M, int N, int nnz, const int *__restrict__ rowptr,
+                    const int *__restrict__ colind, const TB *__restrict__ dY, int lddy,
+                    const TB *__restrict__ X, int ldx, float *__restrict__ out) {
   const int lane = threadIdx.x & (kWave - 1);
-  for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) {
-    const TB *x = X + static_cast<size_t>(colind[e]) * ldx;
-    const TB *y = dY + static_cast<size_t>(row) * lddy;
-    float s = 0.f;
-    for (int k = lane; k < N; k += kWave) s = fmaf(load_scalar<TB>(y + k), load_scalar<TB>(x + k), s);
+  const long long first = (static_cast<long long>(blockIdx.x) * kAdjThreads + threadIdx.x) -
+                          lane;
+  if (first >= nnz) return;  // whole wave
+  const int e0 = static_cast<int>(first);
+  const int e = e0 + lane;
+  const int elast = min(e0 + kWave - 1, nnz - 1);
+  // the row holding entry x: the last r with rowptr[r] <= x
+  auto row_of = [&](int x) {
+    int lo = 0, hi = M - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rowptr[mid] <= x) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  };
+  const int rlo = row_of(e0), rhi = row_of(elast);
+  int row;
+  if (rhi - rlo < kWave) {
+    const int n = rhi - rlo;                           // ends of rows rlo .. rhi-1
+    const int end = rowptr[min(rlo + 1 + lane, M)];    // lane l: end of row rlo + l
+    int k = 0;                                         // rows of the window ending at or before e
 #pragma unroll
-    for (int m = 1; m < kWave; m <<= 1) s += __shfl_xor(s, m);
-    if (lane == 0) out[e] = s;
+    for (int step = 32; step >= 1; step >>= 1) {
+      const int v = __shfl(end, min(k + step - 1, kWave - 1), kWave);
+      if (k + step <= n && v <= e) k += step;
+    }
+    row = rlo + k;
+  } else {
+    row = e < nnz ? row_of(e) : rhi;
   }
+  if (e >= nnz) return;
+  const TB *y = dY + static_cast<size_t>(row) * lddy;
+  const TB *x = X + static_cast<size_t>(colind[e]) * ldx;
+  float acc = 0.f;
+  for (int k0 = 0; k0 < N; k0 += 8) {
+    float a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = min(k0 + u, N - 1);
+      a[u] = load_scalar<TB>(y + k);
+      b[u] = load_scalar<TB>(x + k);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + u < N) acc = fmaf(a[u], b[u], acc);
+  }
+  out[e] = acc;
 }
 
 template <typename TB, int G>
@@ -200,8 +243,9 @@ static int run_sddmm(const spmm_csr_plan_s *plan, int M, int K, int N, int nnz,
       if (t <= lanes && lanes % t == 0) { G = t; break; }
   }
   if (G == 0) {
-    hipLaunchKernelGGL((sddmm_generic_kernel<TB>), dim3(ceil_div(M, 4)), dim3(kAdjThreads), 0, s,
-                       M, N, rowptr, colind, dY, lddy, X, ldx, out);
+    hipLaunchKernelGGL((sddmm_narrow_kernel<TB>),
+                       dim3(static_cast<unsigned>(ceil_div_ll(nnz, kAdjThreads))),
+                       dim3(kAdjThreads), 0, s, M, N, nnz, rowptr, colind, dY, lddy, X, ldx, out);
   } else if (plan) {
     const int ntile = N / (G * e16);
     switch (G) {

@@ -169,7 +169,10 @@ def test_autograd_finite_differences(handle, dev, gcn):
     (2000, 1500, 0.004, 3, 1500, 128),   # hub rows: wave and workgroup classes
     (700, 6000, 0.002, 2, 6000, 512),    # 4 column tiles of 128 (FP32)
     (300, 300, 0.05, 1, 300, 24),        # G = 2 lanes
-    (50, 40, 0.2, 0, 0, 3),              # N not a multiple of 4: generic kernel
+    (50, 40, 0.2, 0, 0, 3),              # N not a multiple of 4: narrow kernel
+    (2000, 1500, 0.004, 3, 1500, 1),     # narrow, N = 1, hub rows
+    (5000, 200, 0.0005, 0, 0, 5),        # narrow, mostly empty rows: 64 entries span > 64 rows
+    (300, 300, 0.05, 1, 300, 33),        # narrow, several 8-column steps
 ])
 def test_sddmm_planned_equals_plan_free(handle, dev, M, K, dens, nl, ll, N):
     """The row-scheduled SDDMM keeps each entry's dot in one subgroup with
@@ -192,6 +195,19 @@ def test_sddmm_planned_equals_plan_free(handle, dev, M, K, dens, nl, ll, N):
         ah = api.sddmm_csr(handle, rp, ci, dYh, Xh)
         bh = api.sddmm_csr(handle, rp, ci, dYh, Xh, plan=plan)
         assert torch.equal(ah, bh)
+
+
+def test_sddmm_narrow_bf16(handle, dev):
+    """BF16 operands with N = 12 (not a multiple of 8 bf16): the narrow
+    kernel, within 1e-5 of FP64 of the rounded operands."""
+    g = graphs.random_csr(800, 900, 0.01, seed=4, long_rows=2, long_len=900)
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    dYh = api.to_bf16_bits(torch.from_numpy(graphs.normal((800, 12), 5, 3)).to(dev))
+    Xh = api.to_bf16_bits(torch.from_numpy(graphs.normal((900, 12), 5, 4)).to(dev))
+    out = api.sddmm_csr(handle, rp, ci, dYh, Xh).cpu().numpy()
+    q = lambda t: oracle.bf16_to_f32(t.view(torch.int16).cpu().numpy().view(np.uint16))
+    ref = oracle.sddmm_f64(g.rowptr, g.colind, q(dYh), q(Xh))
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) <= 1e-5
 
 
 def test_sddmm_planned_mismatch(handle, dev):
