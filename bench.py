@@ -12,7 +12,8 @@ reads one full B row), with GFLOP/s = 2 nnz N / t.
 128, FP32 and BF16 B); GCN-style fwd+bwd (Y = A X, dX = A^T dY,
 dval = SDDMM) in FP32 and BF16 features; 512 uneven graphs batched in one
 captured hipGraph vs 512 separate FP32 launches; aligned B2SR-8/16 binary
-SpMV (binary and FP32 x) on the headline graph beside CSR at N = 1. Errors there are measured
+SpMV (binary and FP32 x) on the headline graph beside CSR at N = 1; the B2SR
+grid (tiles 2..64 x N = 1/32/64/128) beside CSR. Errors there are measured
 against an FP64 numpy recomputation on sampled rows.
 
 `cpu_baseline`: the OpenMP FP32 CSR SpMM of the oracle library on the same
@@ -45,7 +46,7 @@ def csr_bytes(M, nnz, N, s_val=4, s_b=4):
 
 def b2sr_bytes(M, T, nb, nnz, N, s_b=4):
     MB = (M + T - 1) // T
-    tile = {4: 4, 8: 8, 16: 32, 32: 128}[T]
+    tile = {2: 2, 4: 4, 8: 8, 16: 32, 32: 128, 64: 512}[T]
     return 4 * (MB + 1) + nb * (4 + tile) + nnz * N * s_b + M * N * 4
 
 
@@ -353,6 +354,39 @@ def extra_spmv_aligned(args, h, dev):
     return out
 
 
+def extra_b2sr_grid(args, h, dev):
+    """The B2SR workload's grid on the headline graph: planned FBF SpMM for
+    tiles 4/8/16/32 (and 2/64) x hidden 32/64/128, plus N = 1 (the
+    narrow kernel), beside CSR at the same N; hipGraph-replay timing."""
+    import torch
+
+    from spmm_eval_amd import api, graphs
+
+    g = graphs.workload_b2sr_graph()
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    v = torch.ones(g.nnz, device=dev)
+    cp = api.CsrPlan(h, rp, g.nnz)
+    widths = (1, 32, 64, 128)
+    Bs = {N: torch.from_numpy(graphs.uniform_pm1((g.K, N), seed=N + 17)).to(dev) for N in widths}
+    Cs = {N: torch.empty((g.M, N), device=dev) for N in widths}
+    out = {"csr_us": {}, "b2sr_us": {}}
+    for N in widths:
+        ms = time_graph(lambda: api.spmm_csr(h, rp, ci, v, Bs[N], C_out=Cs[N], plan=cp), Cs[N],
+                        steps=50, replays=3)
+        out["csr_us"][f"N{N}"] = round(ms * 1e3, 2)
+    for T in (2, 4, 8, 16, 32, 64):
+        brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
+        plan = api.B2srPlan(h, T, g.M, brp)
+        for N in widths:
+            ms = time_graph(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bs[N], C_out=Cs[N],
+                                                  plan=plan), Cs[N], steps=50, replays=3)
+            out["b2sr_us"][f"T{T}_N{N}"] = round(ms * 1e3, 2)
+        plan.close()
+        del brp, bci, words
+    torch.cuda.empty_cache()
+    return {"b2sr_grid_89k": out}
+
+
 def extra_fwd_bwd(args, h, dev):
     import torch
 
@@ -457,7 +491,7 @@ def main():
     ap.add_argument("--big-steps", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["csr", "fwd_bwd", "batched", "spmv_aligned"], default=None,
+    ap.add_argument("--only", choices=["csr", "fwd_bwd", "batched", "spmv_aligned", "b2sr_grid"], default=None,
                     help="run one extra workload alone and print its result (for profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
@@ -485,7 +519,7 @@ def main():
     h = api.Handle()
     if args.only:
         fn = {"csr": extra_csr, "fwd_bwd": extra_fwd_bwd, "batched": extra_batched,
-              "spmv_aligned": extra_spmv_aligned}[args.only]
+              "spmv_aligned": extra_spmv_aligned, "b2sr_grid": extra_b2sr_grid}[args.only]
         print(json.dumps(fn(args, h, dev)))
         h.close()
         return
@@ -536,7 +570,7 @@ def main():
         result["cpu_baseline"] = cpu_baseline(hd)
     if not args.quick:
         extra = {}
-        for fn in (extra_csr, extra_fwd_bwd, extra_batched, extra_spmv_aligned):
+        for fn in (extra_csr, extra_fwd_bwd, extra_batched, extra_spmv_aligned, extra_b2sr_grid):
             log(fn.__name__)
             try:
                 extra.update(fn(args, h, dev))
