@@ -967,12 +967,12 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
 // leave about one bit per tile, so the words, not the gathers, are most of
 // the bytes. One workgroup per block row (the plan's block-row order). Lane
 // (slot, j) -- j = tid mod T, SLOTS = 256 / T -- keeps tile row j and loads
-// row j's word of tiles slot, slot + SLOTS, ... U at a time, so consecutive
-// lanes read consecutive words and one round covers 64 tiles. The round's
-// set bits get positions in a row-segmented LDS list (row 0's B rows, then
-// row 1's, ..., ascending in column within each row) from one strided scan
-// over the slots (per-tile counts packed in 16-bit pairs) and a scan over
-// the T row totals. Subgroup g owns rows [g RPS, (g+1) RPS) and streams that
+// row j's word of U consecutive tiles (slot s takes tiles s U .. s U + U - 1
+// of the round), so consecutive lanes read consecutive words and one round
+// covers 64 tiles. The round's set bits get positions in a row-segmented
+// LDS list (row 0's B rows, then row 1's, ..., ascending in column within
+// each row) from one strided scan of the lanes' bit counts over the slots
+// and a scan over the T row totals. Subgroup g owns rows [g RPS, (g+1) RPS) and streams that
 // contiguous stretch of the list with 8 gathers in flight into per-row
 // accumulators kept across rounds. Rounds whose bits exceed the list run in
 // windows. Every word is read once, coalesced; rows are summed in ascending
@@ -989,10 +989,8 @@ b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span, i
   constexpr int S = kWave / G, SW = kB2Waves * S;
   constexpr int RPS = T >= SW ? T / SW : 1;  // rows per subgroup
   constexpr int SLOTS = kB2Threads / T;
-  constexpr int U = 64 / SLOTS;              // tiles per lane per round (64 tiles a round)
-  constexpr int NP = (U + 1) / 2;            // packed count pairs
-  static_assert(U % 2 == 0 || U == 1, "counts are packed in pairs");
-  __shared__ uint32_t wtot[kB2Waves][T][NP];
+  constexpr int U = 64 / SLOTS;              // consecutive tiles per lane per round
+  __shared__ int wtot[kB2Waves][T];
   __shared__ int seg[T + 1];
   __shared__ int list[kWideCap];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -1011,55 +1009,41 @@ b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span, i
     for (int q = 0; q < EPL; ++q) acc[i][q] = 0.f;
 
   for (int rb = sp.x; rb < sp.y; rb += SLOTS * U) {
+    // slot s takes tiles rb + s U .. rb + s U + U - 1, so slot-major order is
+    // tile order and one count per lane places all of its entries
     Bits<T> a[U];
     int bc[U];
-    uint32_t pk[NP], inc[NP];
+    const int t0 = rb + slot * U;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int tt = min(rb + slot + u * SLOTS, sp.y - 1);
+      const int tt = min(t0 + u, sp.y - 1);
       a[u] = words[static_cast<size_t>(tt) * T + j];
       bc[u] = bci[tt];
     }
-#pragma unroll
-    for (int p = 0; p < NP; ++p) pk[p] = 0;
+    int cnt = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (rb + slot + u * SLOTS >= sp.y) a[u] = 0;
-      pk[u / 2] += static_cast<uint32_t>(popcount(a[u])) << (16 * (u & 1));
+      if (t0 + u >= sp.y) a[u] = 0;
+      cnt += popcount(a[u]);
     }
     // strided inclusive scan over the slots of this wave (lanes j, j+T, ...)
-#pragma unroll
-    for (int p = 0; p < NP; ++p) inc[p] = pk[p];
+    int inc = cnt;
 #pragma unroll
     for (int d = T; d < kWave; d <<= 1) {
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const uint32_t v = __shfl_up(inc[p], d, kWave);
-        if (lane >= d) inc[p] += v;
-      }
+      const int v = __shfl_up(inc, d, kWave);
+      if (lane >= d) inc += v;
     }
-    if (lane >= kWave - T) {
-#pragma unroll
-      for (int p = 0; p < NP; ++p) wtot[wave][j][p] = inc[p];
-    }
+    if (lane >= kWave - T) wtot[wave][j] = inc;
     __syncthreads();
-    uint32_t before[NP], total[NP];  // earlier waves; all waves
+    int before = 0, total = 0;  // row j's entries in earlier waves; in all
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      before[p] = 0;
-      total[p] = 0;
-#pragma unroll
-      for (int w = 0; w < kB2Waves; ++w) {
-        const uint32_t x = wtot[w][j][p];
-        if (w < wave) before[p] += x;
-        total[p] += x;
-      }
+    for (int w = 0; w < kB2Waves; ++w) {
+      const int x = wtot[w][j];
+      if (w < wave) before += x;
+      total += x;
     }
     if (tid < T) {  // row totals -> row segments (one wave, T <= 64 lanes)
-      int rt = 0;
-#pragma unroll
-      for (int p = 0; p < NP; ++p) rt += static_cast<int>((total[p] & 0xffffu) + (total[p] >> 16));
-      int x = rt;
+      int x = total;
 #pragma unroll
       for (int d = 1; d < T; d <<= 1) {
         const int y = __shfl_up(x, d, kWave);
@@ -1070,16 +1054,13 @@ b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span, i
     }
     __syncthreads();
     const int ntot = seg[T];
-    // position of this lane's first entry of tile u within row j
-    int pos[U];
+    int pos[U];  // this lane's first entry of tile u within row j's segment
     {
-      int run = seg[j];
+      int run = seg[j] + before + inc - cnt;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int sh = 16 * (u & 1);
-        const uint32_t ex = inc[u / 2] - pk[u / 2] + before[u / 2];
-        pos[u] = run + static_cast<int>((ex >> sh) & 0xffffu);
-        run += static_cast<int>((total[u / 2] >> sh) & 0xffffu);
+        pos[u] = run;
+        run += popcount(a[u]);
       }
     }
     int lo = 0, hi = 0;  // this subgroup's stretch of the list
