@@ -268,10 +268,12 @@ def extra_csr(args, h, dev):
     plan = api.CsrPlan(h, rp, g.nnz)
     Bd = torch.from_numpy(B).to(dev)
     C = torch.empty((g.M, 32), device=dev)
-    ms = time_loop(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=C, plan=plan), 200, 20)
+    geo = plan.autotune(rp, ci, v, Bd, C)
+    # ~8 us a call: timed as back-to-back launches in a captured graph
+    ms = time_graph(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=C, plan=plan), C, steps=200)
     nbytes = csr_bytes(g.M, g.nnz, 32)
     rows = np.random.default_rng(5).choice(g.M, 2000, replace=False)
-    out["csr_f32_16k_n32"] = {
+    out["csr_f32_16k_n32"] = {"geometry": geo,
         "nnz": g.nnz, "ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1),
         "pct_hbm_peak": round(100 * nbytes / ms / 1e6 / HBM_PEAK_GBS, 1),
         "GFLOPs": round(2 * g.nnz * 32 / ms / 1e6, 1),
