@@ -55,3 +55,40 @@ def test_sddmm_vs_rocsparse(handle, dev, graph):
     plan = api.CsrPlan(handle, rp, g.nnz)
     out = api.sddmm_csr(handle, rp, ci, dY, X, plan=plan)
     assert rel(out, ref) <= 1e-5
+
+
+def pack_blocks(vals: np.ndarray, T: int) -> np.ndarray:
+    """[nb][T][T] block values -> [nb*T] words, bit T-1-c of word r set iff
+    block[r][c] != 0 (numpy.packbits, big-endian within each row)."""
+    nz = (vals != 0).astype(np.uint8)
+    if T < 8:
+        return (np.packbits(nz, axis=2)[:, :, 0] >> (8 - T)).reshape(-1)
+    b = np.packbits(nz, axis=2)  # [nb][T][T/8] bytes, most significant first
+    return b.reshape(-1, T // 8).view(f">u{T // 8}").reshape(-1).astype(f"<u{T // 8}")
+
+
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
+def test_csr2b2sr_vs_rocsparse_bsr(handle, dev, T):
+    """Our device conversion against rocSPARSE csr2bsr (torch's
+    to_sparse_bsr on the GPU) on the same 0/1 matrix, padded to whole
+    blocks: block row pointers, block columns and every tile word equal."""
+    g = graphs.random_csr(700, 900, 0.01, seed=T, long_rows=2, long_len=900)
+    MB, KB = -(-g.M // T), -(-g.K // T)
+    rp = np.concatenate([g.rowptr, np.full(MB * T - g.M, g.rowptr[-1], np.int32)])
+    A = torch.sparse_csr_tensor(torch.from_numpy(rp).long().to(dev),
+                                torch.from_numpy(g.colind).long().to(dev),
+                                torch.ones(g.nnz, device=dev), size=(MB * T, KB * T))
+    try:
+        bsr = A.to_sparse_bsr((T, T))
+        torch.cuda.synchronize()
+    except (RuntimeError, NotImplementedError) as e:
+        pytest.skip(f"torch.sparse CSR -> BSR unavailable on this device: {e}")
+    brp, bci, words = api.csr2b2sr(handle, torch.from_numpy(g.rowptr).to(dev),
+                                   torch.from_numpy(g.colind).to(dev), g.M, g.K, T)
+    assert bsr.crow_indices().cpu().numpy().tolist() == brp.cpu().numpy().tolist()
+    assert bsr.col_indices().cpu().numpy().tolist() == bci.cpu().numpy().tolist()
+    ref = pack_blocks(bsr.values().cpu().numpy(), T)
+    signed = {torch.uint16: torch.int16, torch.uint32: torch.int32, torch.uint64: torch.int64}
+    w = words.view(signed[words.dtype]) if words.dtype in signed else words
+    got = w.cpu().numpy().view(ref.dtype)
+    np.testing.assert_array_equal(got, ref)

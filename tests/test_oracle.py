@@ -412,3 +412,25 @@ def test_planted_group_order_is_caught():
     x = (np.random.default_rng(0).random(g.K) < 0.5).astype(np.uint8)
     cnt, _ = oracle.spmv_b2sr_aligned(T, nrp, nci, bad, _pack_x(x, T))
     assert not np.array_equal(cnt[:g.M], dense(g, binary=True).astype(np.int64) @ x)
+
+
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
+def test_csr2b2sr_matches_torch_bsr(T):
+    """A library routine as a second pin: torch's CPU CSR -> BSR conversion
+    of the same 0/1 matrix (padded to whole blocks) has the same block row
+    pointers and block columns, and its blocks pack to the same words."""
+    import torch
+
+    g = random_csr(300, 500, 0.02, seed=T, long_rows=1, long_len=500)
+    MB, KB = -(-g.M // T), -(-g.K // T)
+    rp = np.concatenate([g.rowptr, np.full(MB * T - g.M, g.rowptr[-1], np.int32)])
+    A = torch.sparse_csr_tensor(torch.from_numpy(rp).long(), torch.from_numpy(g.colind).long(),
+                                torch.ones(g.nnz), size=(MB * T, KB * T))
+    bsr = A.to_sparse_bsr((T, T))
+    brp, bci, words = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    np.testing.assert_array_equal(bsr.crow_indices().numpy(), brp)
+    np.testing.assert_array_equal(bsr.col_indices().numpy(), bci)
+    nz = bsr.values().numpy() != 0  # [nb][T][T]
+    weights = np.array([1 << (T - 1 - c) for c in range(T)], dtype=object)
+    ref = [int((nz[b, r].astype(object) * weights).sum()) for b in range(nz.shape[0]) for r in range(T)]
+    assert [int(x) for x in words] == ref
