@@ -345,114 +345,6 @@ __device__ __forceinline__ void b2sr_range(int t, int t1, int stride, int r, int
   }
 }
 
-// Planned FBF SpMM for T = 16: rows binned by their block row's tile count
-// run in one launch with the CSR classes: short rows one subgroup per row
-// (several rows per subgroup, the next row's tile range requested before
-// this row's gathers), medium rows one wave per row (its subgroups take tile
-// chunks in turn, butterfly reduction), hub rows one workgroup per row (LDS
-// reduction in a fixed order). Each row decodes only its own tile rows.
-template <int T, typename TB, int G>
-__global__ void __launch_bounds__(kB2Threads)
-b2sr_planned_kernel(LaunchTable tab, const int *__restrict__ rows, int M,
-                    const int2 *__restrict__ span, const int *__restrict__ bci,
-                    const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
-                    int ldb, float *__restrict__ C, int ldc) {
-  constexpr int EPL = Vec16<TB>::kElems;
-  constexpr int S = kWave / G;
-  constexpr int CW = G * EPL;
-  constexpr int TC = Chunk<T, G>::kTiles;
-  __shared__ int lds[kB2Waves * S][Chunk<T, G>::kCap];
-  __shared__ float red[kB2Waves - 1][CW];
-  const Seg sg = find_seg(tab, blockIdx.x);
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int lane_sub = lane & (G - 1);
-  const int sub = lane / G;
-  const int gsub = wave * S + sub;
-  const int col0 = blockIdx.y * CW + lane_sub * EPL;
-  const RowLoader<TB, false> ld(B, ldb, col0, 0);
-  int *list = lds[gsub];
-  const int local_wg = blockIdx.x - sg.wg_begin;
-  float acc[EPL];
-
-  if (sg.kind == kSegA) {
-    const int R = sg.rows_per_sub;
-    const int subs_per_wg = kB2Waves * S;
-    const int end = sg.list_off + sg.count;
-    int pos = sg.list_off + local_wg * subs_per_wg * R + gsub;
-    int row = -1, t0 = 0, t1 = 0;
-    if (pos < end) {  // the row and its block row's tile span arrive together
-      row = rows[pos];
-      const int2 sp = span[pos];
-      t0 = sp.x;
-      t1 = sp.y;
-    }
-    for (int k = 0; k < R && row >= 0; ++k) {
-      const int pos_n = pos + subs_per_wg;
-      int row_n = -1, t0n = 0, t1n = 0;
-      if (k + 1 < R && pos_n < end) {
-        row_n = rows[pos_n];
-        const int2 sp = span[pos_n];
-        t0n = sp.x;
-        t1n = sp.y;
-      }
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-      b2sr_range<T, TB, G>(t0, t1, TC, row % T, lane_sub, bci, words, list, ld, acc);
-      float *Crow = C + static_cast<size_t>(row) * ldc + col0;
-#pragma unroll
-      for (int q = 0; q < EPL; q += 4)
-        *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
-      row = row_n;
-      t0 = t0n;
-      t1 = t1n;
-      pos = pos_n;
-    }
-    return;
-  }
-  const bool wave_mode = sg.kind == kSegB;
-  const int pos = wave_mode ? sg.list_off + local_wg * kB2Waves + wave : sg.list_off + local_wg;
-  if (wave_mode && pos >= sg.list_off + sg.count) return;  // whole wave (no barrier in this mode)
-  const int row = rows[pos];
-  const int2 sp = span[pos];
-  const int t0 = sp.x, t1 = sp.y;
-  const int part = wave_mode ? sub : gsub;
-  const int parts = wave_mode ? S : kB2Waves * S;
-#pragma unroll
-  for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-  b2sr_range<T, TB, G>(t0 + part * TC, t1, parts * TC, row % T, lane_sub, bci, words, list, ld,
-                       acc);
-#pragma unroll
-  for (int m = G; m < kWave; m <<= 1) {
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
-  }
-  float *Crow = C + static_cast<size_t>(row) * ldc + col0;
-  if (wave_mode) {
-    if (sub == 0) {
-#pragma unroll
-      for (int q = 0; q < EPL; q += 4)
-        *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
-    }
-    return;
-  }
-  if (wave > 0 && sub == 0) {
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) red[wave - 1][lane_sub * EPL + q] = acc[q];
-  }
-  __syncthreads();
-  if (wave == 0 && sub == 0) {
-#pragma unroll
-    for (int w = 0; w < kB2Waves - 1; ++w) {
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
-    }
-#pragma unroll
-    for (int q = 0; q < EPL; q += 4)
-      *reinterpret_cast<float4 *>(Crow + q) = make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
-  }
-}
-
 // ------------------------------------------- block-row FBF (T = 4 and 8)
 // For the small tiles the planned launch schedules BLOCK rows (binned by
 // tile count), so each tile is decoded once for all T rows. A tile is one
@@ -962,22 +854,22 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
     if (col0 + q < N) Crow[col0 + q] = 2 * cnt[q] - deg;
 }
 
-// ------------------------------ block-row FBF, wide tiles (T = 32 and 64)
-// A tile is T words of T bits (128 or 512 bytes) and power-law graphs
+// ------------------------------------- block-row FBF, wide tiles (T >= 16)
+// A tile is T words of T bits (32, 128 or 512 bytes) and power-law graphs
 // leave about one bit per tile, so the words, not the gathers, are most of
 // the bytes. One workgroup per block row (the plan's block-row order). Lane
 // (slot, j) -- j = tid mod T, SLOTS = 256 / T -- keeps tile row j and loads
-// row j's word of U consecutive tiles (slot s takes tiles s U .. s U + U - 1
-// of the round), so consecutive lanes read consecutive words and one round
-// covers 64 tiles. The round's set bits get positions in a row-segmented
-// LDS list (row 0's B rows, then row 1's, ..., ascending in column within
-// each row) from one strided scan of the lanes' bit counts over the slots
-// and a scan over the T row totals. Subgroup g owns rows [g RPS, (g+1) RPS) and streams that
-// contiguous stretch of the list with 8 gathers in flight into per-row
-// accumulators kept across rounds. Rounds whose bits exceed the list run in
-// windows. Every word is read once, coalesced; rows are summed in ascending
-// column order as the CSR kernel sums them. (At T = 16 the per-row kernel
-// measured faster: 92-107 us against 100-134 us on the 89k-row graph.)
+// row j's word of U = 16 consecutive tiles (slot s takes tiles s U ..
+// s U + U - 1 of the round: 256 / 128 / 64 tiles a round for T = 16 / 32 /
+// 64), so consecutive lanes read consecutive words. The round's set bits get
+// positions in a row-segmented LDS list (row 0's B rows, then row 1's, ...,
+// ascending in column within each row) from one strided scan of the lanes'
+// bit counts over the slots and a scan over the T row totals. Subgroup g
+// owns rows [g RPS, (g+1) RPS) and streams that contiguous stretch of the
+// list with 8 gathers in flight into per-row accumulators kept across
+// rounds. Rounds whose bits exceed the list run in windows. Every word is
+// read once, coalesced; rows are summed in ascending column order as the
+// CSR kernel sums them.
 constexpr int kWideCap = 2048;  // list entries (8 KB)
 
 template <int T, typename TB, int G, bool kBuf>
@@ -989,7 +881,7 @@ b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span, i
   constexpr int S = kWave / G, SW = kB2Waves * S;
   constexpr int RPS = T >= SW ? T / SW : 1;  // rows per subgroup
   constexpr int SLOTS = kB2Threads / T;
-  constexpr int U = 64 / SLOTS;              // consecutive tiles per lane per round
+  constexpr int U = 16;                      // consecutive tiles per lane per round
   __shared__ int wtot[kB2Waves][T];
   __shared__ int seg[T + 1];
   __shared__ int list[kWideCap];
@@ -1309,38 +1201,6 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
   return SPMM_OK;
 }
 
-// Launch table of a B2SR plan for lane geometry G: rows of block rows with
-// fewer than 2 chunks of tiles -> one subgroup per row, at least 2*S chunks
-// -> one workgroup per row, else one wave per row.
-template <int T, typename TB, int G>
-static void launch_planned(const spmm_csr_plan_s *p, int M, const int *brp, const int *bci,
-                           const void *val, const TB *B, int ldb, float *C, int ldc, int tiles,
-                           hipStream_t s) {
-  constexpr int S = kWave / G;
-  constexpr int TC = Chunk<T, G>::kTiles;
-  LaunchTable t;
-  build_table_ex(p->counts, p->offs, S, 2ll * TC, 2ll * S * TC, TC, 16, 1, &t);
-  if (t.total_wg == 0) return;
-  hipLaunchKernelGGL((b2sr_planned_kernel<T, TB, G>), dim3(t.total_wg, tiles), dim3(kB2Threads), 0,
-                     s, t, p->d_rows, M, p->d_span, bci,
-                     static_cast<const typename Word<T>::type *>(val), B, ldb, C, ldc);
-}
-
-template <int T, typename TB>
-static void dispatch_planned(const Geo &g, const spmm_csr_plan_s *p, int M, const int *brp,
-                             const int *bci, const void *val, const TB *B, int ldb, float *C,
-                             int ldc, hipStream_t s) {
-  switch (g.G) {
-    case 1: launch_planned<T, TB, 1>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 2: launch_planned<T, TB, 2>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 4: launch_planned<T, TB, 4>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 8: launch_planned<T, TB, 8>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 16: launch_planned<T, TB, 16>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 32: launch_planned<T, TB, 32>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    default: launch_planned<T, TB, 64>(p, M, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-  }
-}
-
 template <int T, typename TB, int G>
 static void launch_brow(const spmm_csr_plan_s *p, int M, int K, const int *brp, const int *bci,
                         const void *val, const TB *B, int ldb, float *C, int ldc, int tiles,
@@ -1452,7 +1312,7 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     case 2: dispatch_brow<2, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 16: dispatch_planned<16, TB>(g, p, M, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 16: dispatch_wide<16, TB>(g, p, M, K, bci, val, B, ldb, C, ldc, s); break;
     case 32: dispatch_wide<32, TB>(g, p, M, K, bci, val, B, ldb, C, ldc, s); break;
     default: dispatch_wide<64, TB>(g, p, M, K, bci, val, B, ldb, C, ldc, s); break;
   }
@@ -1551,17 +1411,13 @@ int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr, spmm_
   SPMM_CHECK_ARG(h && plan && bsrRowPtr, "null handle, plan or bsrRowPtr");
   SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M > 0, "M must be positive");
-  int st;
-  if (T != 16) {
-    // block rows binned by tile count (the units of b2sr_brow_kernel and
-    // b2sr_wide_kernel), heaviest first
-    st = analyze(h, (M + T - 1) / T, 0, bsrRowPtr, 0, plan, stream);
-    if (st == SPMM_OK) (*plan)->M = M;
-  } else {
-    // rows binned by their block row's tile count (b2sr_planned_kernel's units)
-    st = analyze(h, M, 0, bsrRowPtr, 4, plan, stream);
+  // block rows binned by tile count (the units of b2sr_brow_kernel and
+  // b2sr_wide_kernel), heaviest first
+  const int st = analyze(h, (M + T - 1) / T, 0, bsrRowPtr, 0, plan, stream);
+  if (st == SPMM_OK) {
+    (*plan)->M = M;
+    (*plan)->tile = T;
   }
-  if (st == SPMM_OK) (*plan)->tile = T;
   return st;
 }
 
