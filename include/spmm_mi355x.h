@@ -147,12 +147,13 @@ int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N,
                    const void *bsrVal, const uint16_t *B, int ldb, float *C,
                    int ldc, void *stream);
 
-/* Planned FBF SpMM. spmm_b2sr_analyze bins the block rows (T = 2, 4, 8) or
- * the rows (T = 16, 32, 64) by tile count and keeps each unit's tile span beside
- * it (SYNCHRONISES once); the planned call runs every class in a single
- * launch (no list state, asynchronous) and reads the spans from the plan,
- * so a plan is valid only for the bsrRowPtr it was built from. The plan is
- * a spmm_csr_plan_t; it keeps no pointer to bsrRowPtr. */
+/* Planned FBF SpMM. spmm_b2sr_analyze bins the block rows by tile count and
+ * keeps each block row's tile span beside it (SYNCHRONISES once); the
+ * planned call runs in a single launch (no list state, asynchronous) and
+ * reads the spans from the plan, so a plan is valid only for the bsrRowPtr
+ * it was built from. bsrVal must be aligned to T bytes for T <= 8 and to 16
+ * bytes for T >= 16, and K < 2^27. The plan is a spmm_csr_plan_t; it keeps
+ * no pointer to bsrRowPtr. */
 int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr,
                       spmm_csr_plan_t *plan, void *stream);
 int spmm_b2sr_f32_planned(spmm_csr_plan_t plan, int T, int M, int K, int N,
