@@ -142,6 +142,31 @@ def test_fbf_odd_widths(handle, dev, T, N):
 
 
 @pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("ldb,bf16", [(72, False), (70, False), (136, True), (132, True)])
+def test_fbf_strided_operands(handle, dev, T, ldb, bf16):
+    """B and C as column slices of wider arrays: an aligned leading dimension
+    keeps the vector kernels (ldb 72 FP32, 136 BF16), an unaligned one runs
+    the narrow kernel (70, 132); C's columns past N stay untouched; drop-in
+    and planned forms; +-1 B exact."""
+    N = 64
+    g = graphs.random_csr(300, 500, 0.03, seed=T + ldb, long_rows=2, long_len=500)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    S = np.random.default_rng(T).choice([-1, 1], size=(g.K, ldb)).astype(np.float32)
+    Bfull = torch.from_numpy(S).to(dev)
+    if bf16:
+        Bfull = api.to_bf16_bits(Bfull)
+    B = Bfull[:, :N]
+    exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S[:, :N].astype(np.int32)).astype(np.float32)
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    for p in (None, plan):
+        Cfull = torch.full((g.M, N + 12), 5.0, device=dev)
+        api.spmm_b2sr(handle, T, g.M, brp, bci, words, B, C_out=Cfull[:, :N], plan=p)
+        np.testing.assert_array_equal(Cfull[:, :N].cpu().numpy(), exact)
+        assert (Cfull[:, N:] == 5.0).all()
+
+
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("N", [32, 64, 70, 128])
 def test_sign_packed_popcount(handle, dev, b2sr_graph, T, N):
     g = b2sr_graph
