@@ -233,11 +233,11 @@ b2sr_spmm_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
 
 // Block rows with more than kLongChunks chunks of tiles (the hub rows of a
 // power-law graph) are appended by b2sr_spmm_kernel to a list that this
-// persistent kernel drains, one block row per workgroup: each wave takes
-// one of the block row's rows, its S subgroups take the row's tile chunks
-// in turn (decode + gather), and a butterfly over the subgroups sums the
-// partial rows in a fixed order. The last workgroup to finish resets the
-// list counters, so consecutive (or graph-replayed) calls need no memset.
+// persistent kernel drains one ROW of a listed block row per workgroup: its
+// 4S subgroups take the row's tile chunks in turn (decode + gather), a
+// butterfly sums each wave's subgroups and the waves' sums meet in LDS in a
+// fixed order. The last workgroup to finish resets the list counters, so
+// consecutive (or graph-replayed) calls need no memset.
 template <int T, typename TB, int G, int VEC>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci,
@@ -249,60 +249,71 @@ b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
   constexpr int U = EPL >= 16 ? 4 : 8;
   constexpr int TC = Chunk<T, G>::kTiles;
   __shared__ int lds[kB2Waves * S][Chunk<T, G>::kCap];
+  __shared__ float red[kB2Waves - 1][CW];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int lane_sub = lane & (G - 1);
   const int sub = lane / G;
+  const int gsub = wave * S + sub;
   const int col0 = blockIdx.y * CW + lane_sub * EPL;
   const TB *Bc = B + col0;
-  int *list = lds[wave * S + sub];
-  const int n_long = state[0];
-  for (int i = blockIdx.x; i < n_long; i += gridDim.x) {
-    const int br = state[2 + i];
+  int *list = lds[gsub];
+  const int n_items = state[0] * T;
+  for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
+    const int br = state[2 + i / T];
+    const int r = i % T;
+    const int row = br * T + r;
+    if (row >= M) continue;  // whole workgroup
     const int t0 = brp[br], t1 = brp[br + 1];
-    for (int r0 = 0; r0 < T; r0 += kB2Waves) {
-      const int r = r0 + wave;
-      const int row = br * T + r;
-      if (r >= T || row >= M) continue;  // whole wave (T = 2 leaves waves 2, 3 idle)
-      float acc[EPL];
+    float acc[EPL];
 #pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-      for (int t = t0 + sub * TC; t < t1; t += S * TC) {
-        const int total = decode_chunk<T, G>(t, t1, r, lane_sub, bci, words, list);
-        for (int k = 0; k < total; k += U) {
-          typename Vec16<TB>::Raw raw[U][VEC];
+    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+    for (int t = t0 + gsub * TC; t < t1; t += kB2Waves * S * TC) {
+      const int total = decode_chunk<T, G>(t, t1, r, lane_sub, bci, words, list);
+      for (int k = 0; k < total; k += U) {
+        typename Vec16<TB>::Raw raw[U][VEC];
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const TB *p = Bc + static_cast<size_t>(list[k + u < total ? k + u : 0]) * ldb;
+        for (int u = 0; u < U; ++u) {
+          const TB *p = Bc + static_cast<size_t>(list[k + u < total ? k + u : 0]) * ldb;
 #pragma unroll
-            for (int q = 0; q < VEC; ++q) raw[u][q] = Vec16<TB>::raw(p + q * Vec16<TB>::kElems);
-          }
+          for (int q = 0; q < VEC; ++q) raw[u][q] = Vec16<TB>::raw(p + q * Vec16<TB>::kElems);
+        }
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            if (k + u < total) {
-              float b[EPL];
+        for (int u = 0; u < U; ++u) {
+          if (k + u < total) {
+            float b[EPL];
 #pragma unroll
-              for (int q = 0; q < VEC; ++q) Vec16<TB>::widen(raw[u][q], &b[q * Vec16<TB>::kElems]);
+            for (int q = 0; q < VEC; ++q) Vec16<TB>::widen(raw[u][q], &b[q * Vec16<TB>::kElems]);
 #pragma unroll
-              for (int q = 0; q < EPL; ++q) acc[q] += b[q];
-            }
+            for (int q = 0; q < EPL; ++q) acc[q] += b[q];
           }
         }
-        __builtin_amdgcn_wave_barrier();
       }
-#pragma unroll
-      for (int m = G; m < kWave; m <<= 1) {
-#pragma unroll
-        for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
-      }
-      if (sub == 0) {
-        float *Crow = C + static_cast<size_t>(row) * ldc + col0;
-#pragma unroll
-        for (int q = 0; q < EPL; q += 4)
-          *reinterpret_cast<float4 *>(Crow + q) =
-              make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
-      }
+      __builtin_amdgcn_wave_barrier();
     }
+#pragma unroll
+    for (int m = G; m < kWave; m <<= 1) {
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
+    }
+    if (wave > 0 && sub == 0) {
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) red[wave - 1][lane_sub * EPL + q] = acc[q];
+    }
+    __syncthreads();
+    if (wave == 0 && sub == 0) {
+#pragma unroll
+      for (int w = 0; w < kB2Waves - 1; ++w) {
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
+      }
+      float *Crow = C + static_cast<size_t>(row) * ldc + col0;
+#pragma unroll
+      for (int q = 0; q < EPL; q += 4)
+        *reinterpret_cast<float4 *>(Crow + q) =
+            make_float4(acc[q], acc[q + 1], acc[q + 2], acc[q + 3]);
+    }
+    __syncthreads();  // red[] is reused by the next row
   }
   __syncthreads();
   if (threadIdx.x == 0) {
