@@ -57,11 +57,15 @@ static bool valid_T(int T) {
 }
 
 // ------------------------------------------------------------ conversion
+// Block rows holding more than hub_entries stored entries are appended to
+// hubs (hubs[0] = count, zeroed before the launch) for csr2b2sr_hub_kernel
+// (hub_entries = INT_MAX: none are).
 template <int T, bool kFill>
 __global__ void __launch_bounds__(kB2Threads)
 csr2b2sr_kernel(int M, int K, const int *__restrict__ rowptr, const int *__restrict__ colind,
                 int *__restrict__ bsrRowPtr, int *__restrict__ bsrColInd,
-                typename Word<T>::type *__restrict__ bsrVal, int *__restrict__ err) {
+                typename Word<T>::type *__restrict__ bsrVal, int *__restrict__ err,
+                int hub_entries, int *__restrict__ hubs) {
   using W = typename Word<T>::type;
   constexpr int GPW = kWave / T;  // block rows per wave
   const int lane = threadIdx.x & (kWave - 1);
@@ -75,6 +79,12 @@ csr2b2sr_kernel(int M, int K, const int *__restrict__ rowptr, const int *__restr
   if (live && row < M) {
     pos = rowptr[row];
     end = rowptr[row + 1];
+  }
+  const bool hub =
+      live && rowptr[min(br * T + T, M)] - rowptr[br * T] > hub_entries;  // same in the group
+  if (hub) {
+    pos = end;
+    if (r == 0) hubs[1 + atomicAdd(&hubs[0], 1)] = br;
   }
   int out = 0;
   if (kFill && live) out = bsrRowPtr[br];
@@ -107,9 +117,124 @@ csr2b2sr_kernel(int M, int K, const int *__restrict__ rowptr, const int *__restr
       ++count;
     }
   }
-  if (!kFill && live && r == 0) bsrRowPtr[br] = count;
+  if (!kFill && live && !hub && r == 0) bsrRowPtr[br] = count;
   if (bad) atomicOr(err, 1);
 }
+
+// Block rows with more than kHubEntries stored entries (the hub rows of a
+// power-law graph, whose merge above is one long serial walk) are listed by
+// csr2b2sr_kernel and taken one per workgroup by a persistent grid: the
+// block row's block columns are marked in an LDS bitmap over [0, K/T).
+// Count pass: the bitmap's popcount is the block row's tile count (column
+// indices checked as in the merge). Fill pass: a popcount prefix over the
+// bitmap (per-thread segments, then a workgroup scan) ranks every block
+// column, so bsrColInd comes out ascending; the block row's words are zeroed
+// and each entry ORs its bit in with a 32-bit atomic (bsrVal 4-byte aligned).
+constexpr int kHubEntries = 64;
+constexpr int kHubBitmapWords = 12288;  // 48 KB of LDS: K / T <= 393,216
+
+template <int T, bool kFill>
+__device__ __forceinline__ void hub_block_row(int br, int M, int K, const int *__restrict__ rowptr,
+                                              const int *__restrict__ colind,
+                                              int *__restrict__ bsrRowPtr,
+                                              int *__restrict__ bsrColInd,
+                                              typename Word<T>::type *__restrict__ bsrVal,
+                                              int *__restrict__ err) {
+  __shared__ uint32_t bm[kHubBitmapWords];
+  __shared__ int rp[T + 1];
+  __shared__ int segpre[kB2Threads];
+  __shared__ int wsum[kB2Waves];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int r_end = min(br * T + T, M);
+  const int e0 = rowptr[br * T], e1 = rowptr[r_end];
+  const int KB = (K + T - 1) / T, KBW = (KB + 31) / 32;
+  const int nrows = r_end - br * T;
+  for (int i = tid; i <= nrows; i += kB2Threads) rp[i] = rowptr[br * T + i];
+  for (int w = tid; w < KBW; w += kB2Threads) bm[w] = 0;
+  __syncthreads();
+  int bad = 0;
+  for (int e = e0 + tid; e < e1; e += kB2Threads) {
+    int r = 0;  // this entry's row within the block row
+    while (r + 1 < nrows && rp[r + 1] <= e) ++r;
+    const int c = colind[e];
+    if (c < 0 || c >= K || (e > rp[r] && colind[e - 1] > c)) {  // repeats allowed, as in the merge
+      bad = 1;
+      continue;
+    }
+    const int bc = c / T;
+    atomicOr(&bm[bc >> 5], 1u << (bc & 31));
+  }
+  if (!kFill && bad) atomicOr(err, 1);
+  __syncthreads();
+  // per-thread segments of the bitmap, their popcounts scanned over the block
+  const int seg = (KBW + kB2Threads - 1) / kB2Threads;
+  const int w0 = tid * seg, w1 = min(w0 + seg, KBW);
+  int mine = 0;
+  for (int w = w0; w < w1; ++w) mine += __popc(bm[w]);
+  int inc = mine;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int y = __shfl_up(inc, d, kWave);
+    if (lane >= d) inc += y;
+  }
+  if (lane == kWave - 1) wsum[wave] = inc;
+  __syncthreads();
+  int before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kB2Waves; ++w) {
+    before += w < wave ? wsum[w] : 0;
+    total += wsum[w];
+  }
+  if constexpr (!kFill) {
+    if (tid == 0) bsrRowPtr[br] = total;
+  } else {
+    segpre[tid] = before + inc - mine;
+    const int base = bsrRowPtr[br];
+    // block columns in order
+    int k = base + before + inc - mine;
+    for (int w = w0; w < w1; ++w) {
+      uint32_t x = bm[w];
+      while (x) {
+        const int b = __ffs(x) - 1;
+        x &= x - 1;
+        bsrColInd[k++] = w * 32 + b;
+      }
+    }
+    using W = typename Word<T>::type;
+    for (long long i = tid; i < static_cast<long long>(total) * T; i += kB2Threads)
+      bsrVal[static_cast<size_t>(base) * T + i] = static_cast<W>(0);
+    __syncthreads();
+    uint32_t *val32 = reinterpret_cast<uint32_t *>(bsrVal);
+    for (int e = e0 + tid; e < e1; e += kB2Threads) {
+      int r = 0;
+      while (r + 1 < nrows && rp[r + 1] <= e) ++r;
+      const int c = colind[e];
+      if (c < 0 || c >= K) continue;  // reported by the count pass
+      const int bc = c / T, w = bc >> 5;
+      const int sgi = w / seg;
+      int rank = segpre[sgi];
+      for (int q = sgi * seg; q < w; ++q) rank += __popc(bm[q]);
+      rank += __popc(bm[w] & ((1u << (bc & 31)) - 1u));
+      const size_t word = static_cast<size_t>(base + rank) * T + r;
+      const int p = T - 1 - c % T;  // bit within the tile row's word
+      const size_t bit = word * (sizeof(W) * 8) + p;
+      atomicOr(&val32[bit >> 5], 1u << (bit & 31));
+    }
+  }
+  __syncthreads();  // the LDS arrays are reused by the workgroup's next block row
+}
+
+template <int T, bool kFill>
+__global__ void __launch_bounds__(kB2Threads)
+csr2b2sr_hub_kernel(int M, int K, const int *__restrict__ rowptr, const int *__restrict__ colind,
+                    int *__restrict__ bsrRowPtr, int *__restrict__ bsrColInd,
+                    typename Word<T>::type *__restrict__ bsrVal, int *__restrict__ err,
+                    const int *__restrict__ hubs) {
+  const int n = hubs[0];
+  for (int i = blockIdx.x; i < n; i += gridDim.x)
+    hub_block_row<T, kFill>(hubs[1 + i], M, K, rowptr, colind, bsrRowPtr, bsrColInd, bsrVal, err);
+}
+
 
 // --------------------------------------------------- decode to LDS list
 // LDS list entries per subgroup and tiles decoded per chunk: a chunk of
@@ -1065,24 +1190,35 @@ unpack_cols_kernel(int H, int W, const uint32_t *__restrict__ in, float *__restr
 // ------------------------------------------------------------------ host
 template <int T, bool kFill>
 static void launch_convert(int M, int K, const int *rowptr, const int *colind, int *brp,
-                           int *bci, void *val, int *err, hipStream_t s) {
+                           int *bci, void *val, int *err, int *hubs, hipStream_t s) {
   const int MB = (M + T - 1) / T;
   const int waves = ceil_div(MB, kWave / T);
   const int blocks = ceil_div(waves, kB2Waves);
+  // hub block rows to the bitmap kernel when K / T fits its LDS bitmap (and,
+  // for the fill, bsrVal takes 32-bit atomics); the count and the fill pass
+  // of one conversion make the same choice
+  const bool use_hubs = MB > 0 && (K + T - 1) / T <= kHubBitmapWords * 32 &&
+                        reinterpret_cast<uintptr_t>(val) % 4 == 0;
+  const int hub_entries = use_hubs ? kHubEntries : INT_MAX;
+  auto *w = static_cast<typename Word<T>::type *>(val);
   hipLaunchKernelGGL((csr2b2sr_kernel<T, kFill>), dim3(blocks), dim3(kB2Threads), 0, s, M, K,
-                     rowptr, colind, brp, bci, static_cast<typename Word<T>::type *>(val), err);
+                     rowptr, colind, brp, bci, w, err, hub_entries, hubs);
+  if (use_hubs)  // a persistent grid drains the list the merge kernel filled
+    hipLaunchKernelGGL((csr2b2sr_hub_kernel<T, kFill>), dim3(min(MB, 1024)), dim3(kB2Threads),
+                       0, s, M, K, rowptr, colind, brp, bci, w, err, hubs);
 }
 
+// hubs: 1 + MB ints of device workspace, hubs[0] zeroed on s
 template <bool kFill>
 static void convert(int T, int M, int K, const int *rowptr, const int *colind, int *brp,
-                    int *bci, void *val, int *err, hipStream_t s) {
+                    int *bci, void *val, int *err, int *hubs, hipStream_t s) {
   switch (T) {
-    case 2: launch_convert<2, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
-    case 4: launch_convert<4, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
-    case 8: launch_convert<8, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
-    case 16: launch_convert<16, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
-    case 32: launch_convert<32, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
-    default: launch_convert<64, kFill>(M, K, rowptr, colind, brp, bci, val, err, s); break;
+    case 2: launch_convert<2, kFill>(M, K, rowptr, colind, brp, bci, val, err, hubs, s); break;
+    case 4: launch_convert<4, kFill>(M, K, rowptr, colind, brp, bci, val, err, hubs, s); break;
+    case 8: launch_convert<8, kFill>(M, K, rowptr, colind, brp, bci, val, err, hubs, s); break;
+    case 16: launch_convert<16, kFill>(M, K, rowptr, colind, brp, bci, val, err, hubs, s); break;
+    case 32: launch_convert<32, kFill>(M, K, rowptr, colind, brp, bci, val, err, hubs, s); break;
+    default: launch_convert<64, kFill>(M, K, rowptr, colind, brp, bci, val, err, hubs, s); break;
   }
 }
 
@@ -1366,16 +1502,18 @@ int csr2b2sr_nnzb(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, con
   SPMM_CHECK_ARG(nnz == 0 || colind, "null colind");
   hipStream_t s = as_stream(stream);
   const int MB = (M + T - 1) / T;
-  const size_t part_bytes = scan_workspace_bytes(MB + 1);
-  char *ws = static_cast<char *>(workspace(h, 256 + part_bytes));
+  const size_t part_bytes = (scan_workspace_bytes(MB + 1) + 255) / 256 * 256;
+  char *ws = static_cast<char *>(workspace(h, 256 + part_bytes + sizeof(int) * (MB + 1)));
   int *host = static_cast<int *>(pinned(h, 2 * sizeof(int)));
   if (!ws || !host) return SPMM_ERR_ALLOC;
   int *err = reinterpret_cast<int *>(ws);
   int *part = reinterpret_cast<int *>(ws + 256);
+  int *hubs = reinterpret_cast<int *>(ws + 256 + part_bytes);
   SPMM_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+  SPMM_HIP(hipMemsetAsync(hubs, 0, sizeof(int), s));
   SPMM_HIP(hipMemsetAsync(bsrRowPtr + MB, 0, sizeof(int), s));
   if (MB > 0) {
-    convert<false>(T, M, K, rowptr, colind, bsrRowPtr, nullptr, nullptr, err, s);
+    convert<false>(T, M, K, rowptr, colind, bsrRowPtr, nullptr, nullptr, err, hubs, s);
     SPMM_LAUNCHED();
   }
   SPMM_HIP(exclusive_scan(bsrRowPtr, MB + 1, part, s));
@@ -1398,9 +1536,14 @@ int csr2b2sr(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const in
   SPMM_CHECK_ARG(nnz == 0 || (colind && bsrColInd && bsrVal), "null colind or outputs");
   if (M == 0 || nnz == 0) return SPMM_OK;
   hipStream_t s = as_stream(stream);
-  int *err = static_cast<int *>(workspace(h, 256));
-  if (!err) return SPMM_ERR_ALLOC;
-  convert<true>(T, M, K, rowptr, colind, const_cast<int *>(bsrRowPtr), bsrColInd, bsrVal, err, s);
+  const int MB = (M + T - 1) / T;
+  char *ws = static_cast<char *>(workspace(h, 256 + sizeof(int) * (MB + 1)));
+  if (!ws) return SPMM_ERR_ALLOC;
+  int *err = reinterpret_cast<int *>(ws);
+  int *hubs = reinterpret_cast<int *>(ws + 256);
+  SPMM_HIP(hipMemsetAsync(hubs, 0, sizeof(int), s));
+  convert<true>(T, M, K, rowptr, colind, const_cast<int *>(bsrRowPtr), bsrColInd, bsrVal, err,
+                hubs, s);
   SPMM_LAUNCHED();
   return SPMM_OK;
 }

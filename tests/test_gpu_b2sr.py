@@ -91,6 +91,30 @@ def test_csr2b2sr_rejects_unsorted(handle, dev):
         api.csr2b2sr(handle, rp, ci, 1, 8, 4)
 
 
+@pytest.mark.parametrize("T", [2, 4, 16, 64])
+def test_csr2b2sr_hub_rows_repeats_and_order(handle, dev, T):
+    """Block rows of more than 64 entries take the bitmap path: rows with
+    repeated columns (kept by the reference's reader) convert like the
+    oracle; an unsorted long row is rejected like an unsorted short one."""
+    r = np.random.default_rng(T)
+    rows = []
+    for i in range(3 * T):
+        n = 400 if i % T == 1 else int(r.integers(0, 6))
+        c = np.sort(r.integers(0, 3000, n)).astype(np.int32)  # repeats possible
+        rows.append(c)
+    rp = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int32)
+    ci = np.concatenate(rows).astype(np.int32)
+    brp, bci, words = api.csr2b2sr(handle, torch.from_numpy(rp).to(dev), torch.from_numpy(ci).to(dev),
+                                   len(rows), 3000, T)
+    assert_b2sr_equal((brp, bci, words), oracle.csr2b2sr(rp, ci, T))
+    bad = ci.copy()
+    s0, s1 = rp[1], rp[2]
+    bad[s0:s1] = bad[s0:s1][::-1]  # a long row in descending order
+    with pytest.raises(SpmmError):
+        api.csr2b2sr(handle, torch.from_numpy(rp).to(dev), torch.from_numpy(bad).to(dev),
+                     len(rows), 3000, T)
+
+
 @pytest.fixture(scope="module")
 def b2sr_graph():
     return graphs.workload_b2sr_graph()
