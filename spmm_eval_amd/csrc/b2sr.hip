@@ -24,6 +24,7 @@
 // so decoding a whole block row per round keeps the gathers deep.
 #include "csr_core.hpp"
 
+#include <atomic>
 #include <climits>
 #include <type_traits>
 
@@ -51,6 +52,14 @@ template <typename U> __device__ __forceinline__ int top_bit(U x) {  // x != 0
 constexpr int kB2Threads = 256;
 constexpr int kB2Waves = kB2Threads / kWave;
 constexpr int kLongChunks = 2;  // block rows above this many tile chunks go to b2sr_long_kernel
+// The handle's plan-free state: kStateHead counters (zero between calls;
+// each kernel that counts resets those it used), then the block-row lists.
+constexpr int kStateHead = 8;
+// plan-free wide tiles: block rows of at least this many tiles go first
+// (on a power-law graph: T = 16 at 256 tiles 105 us against 124 at 512 and
+// 173 at 128; T = 32 and 64 at 512 tiles 145 and 264 us against 208 and
+// 380 at 256)
+constexpr int wide_heavy(int T) { return T <= 16 ? 256 : 512; }
 
 static bool valid_T(int T) {
   return T == 2 || T == 4 || T == 8 || T == 16 || T == 32 || T == 64;
@@ -294,6 +303,9 @@ __device__ __forceinline__ int decode_chunk(int t, int t1, int r, int lane_sub,
   return total;
 }
 
+// Plan-free form when bsrVal lacks the planned kernels' alignment: one row
+// per subgroup, Word<T> loads at their natural alignment; block rows of more
+// than kLongChunks chunks are listed for b2sr_long_kernel.
 template <int T, typename TB, int G, int VEC>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_spmm_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci,
@@ -319,7 +331,7 @@ b2sr_spmm_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
   const int t0 = brp[br], t1 = brp[br + 1];
   if (t1 - t0 > kLongChunks * Chunk<T, G>::kTiles) {  // left to b2sr_long_kernel
     // one entry per block row: its row 0, first column tile
-    if (blockIdx.y == 0 && r == 0 && lane_sub == 0) state[2 + atomicAdd(&state[0], 1)] = br;
+    if (blockIdx.y == 0 && r == 0 && lane_sub == 0) state[kStateHead + atomicAdd(&state[0], 1)] = br;
     return;
   }
   float acc[EPL];
@@ -385,7 +397,7 @@ b2sr_long_kernel(int M, const int *__restrict__ brp, const int *__restrict__ bci
   int *list = lds[gsub];
   const int n_items = state[0] * T;
   for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
-    const int br = state[2 + i / T];
+    const int br = state[kStateHead + i / T];
     const int r = i % T;
     const int row = br * T + r;
     if (row >= M) continue;  // whole workgroup
@@ -842,6 +854,143 @@ b2sr_brow_kernel(LaunchTable tab, const int *__restrict__ brows, int M,
                           C, ldc);
 }
 
+// ------------------------------------------- plan-free form, T <= 8
+// The planned form's classes found on the device each call: block rows of
+// at least w_tiles tiles (one wave chunk) are appended to the
+// workgroup-class list (state + kStateHead, count state[0]), the others of
+// at least a_tiles tiles (one subgroup chunk) to the wave-class list
+// (state + kStateHead + MB, count state[2]); one atomic per workgroup and
+// list, positions from the waves' ballots.
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_class_kernel(int MB, const int *__restrict__ brp, int a_tiles, int w_tiles,
+                  int *__restrict__ state) {
+  __shared__ int cnt[2][kB2Waves];
+  __shared__ int base[2];
+  const int br = static_cast<int>(blockIdx.x) * kB2Threads + static_cast<int>(threadIdx.x);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int n = br < MB ? brp[br + 1] - brp[br] : 0;
+  const bool in[2] = {br < MB && n >= w_tiles, br < MB && n < w_tiles && n >= a_tiles};
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long m[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    m[c] = __ballot(in[c]);
+    if (lane == 0) cnt[c][wave] = __popcll(m[c]);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int c = threadIdx.x;
+    int tot = 0;
+#pragma unroll
+    for (int w = 0; w < kB2Waves; ++w) tot += cnt[c][w];
+    base[c] = tot ? atomicAdd(&state[c == 0 ? 0 : 2], tot) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (!in[c]) continue;
+    int pos = base[c] + __popcll(m[c] & below);
+    for (int w = 0; w < wave; ++w) pos += cnt[c][w];
+    state[kStateHead + (c == 0 ? 0 : MB) + pos] = br;
+  }
+}
+
+// One launch over the classes b2sr_class_kernel listed, with the planned
+// form's per-class code and static shares (one counter shared by every
+// workgroup would serialise their atomics across the XCDs). The first E
+// workgroups, as many as the chip holds at once, take the listed work:
+// workgroup x the workgroup-class rows x, x + E, ... (row i = listed block
+// row i / T, row i % T), and the wave-class block rows are dealt to their
+// waves from the last of them backwards, so they land on workgroups with
+// one workgroup-class row fewer. Workgroup E + k then streams block rows
+// k SW .. k SW + SW - 1, one subgroup each, skipping the listed ones. The
+// last workgroup to finish zeroes the list counts, so consecutive or
+// graph-replayed calls need no memset.
+template <int T, typename TB, int G, bool kBuf>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_dropin_kernel(int M, int E, const int *__restrict__ brp, const int *__restrict__ bci,
+                   const typename TileWord<T>::type *__restrict__ tw, const TB *__restrict__ B,
+                   int ldb, int b_bytes, float *__restrict__ C, int ldc,
+                   int *__restrict__ state) {
+  constexpr int EPL = Vec16<TB>::kElems;
+  constexpr int S = kWave / G, SW = kB2Waves * S;
+  constexpr int CW = G * EPL;
+  constexpr int CAP = BrowChunk<G>::kCap;
+  constexpr int RC = Chunk<T, G>::kTiles;
+  constexpr int TA = BrowChunk<G>::kTiles;
+  static_assert(Chunk<T, G>::kCap == CAP, "all paths share the list area");
+  static_assert((kB2Waves - 1) * CW <= SW * CAP, "reduction must fit the list area");
+  __shared__ int lds[SW * CAP];
+  const int MB = (M + T - 1) / T;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1);
+  const int sub = lane / G;
+  const int gsub = wave * S + sub;
+  const int col0 = blockIdx.y * CW + lane_sub * EPL;
+  const RowLoader<TB, kBuf> ld(B, ldb, col0, b_bytes);
+  const int x = blockIdx.x;
+
+  if (x < E) {
+    const int n_rows = state[0] * T, n_brows = state[2];
+    const auto *words = reinterpret_cast<const typename Word<T>::type *>(tw);
+    for (int i = x; i < n_rows; i += E) {  // workgroup class
+      const int br = state[kStateHead + i / T];
+      const int r = i % T;
+      const int row = br * T + r;
+      if (row >= M) continue;  // whole workgroup
+      float acc[EPL];
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+      b2sr_range<T, TB, G>(brp[br] + gsub * RC, brp[br + 1], SW * RC, r, lane_sub, bci, words,
+                           lds + gsub * CAP, ld, acc);
+#pragma unroll
+      for (int m = G; m < kWave; m <<= 1) {
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) acc[q] += __shfl_xor(acc[q], m);
+      }
+      float *red = reinterpret_cast<float *>(lds);
+      __syncthreads();
+      if (wave > 0 && sub == 0) {
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) red[(wave - 1) * CW + lane_sub * EPL + q] = acc[q];
+      }
+      __syncthreads();
+      if (wave == 0 && sub == 0) {
+#pragma unroll
+        for (int w = 0; w < kB2Waves - 1; ++w) {
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) acc[q] += red[w * CW + lane_sub * EPL + q];
+        }
+        store_row<EPL>(C, ldc, row, col0, acc);
+      }
+      __syncthreads();  // red is the next row's list area
+    }
+    for (int j = (E - 1 - x) * kB2Waves + wave; j < n_brows; j += E * kB2Waves) {  // wave class
+      const int br = state[kStateHead + MB + j];
+      brow_wave<T, TB, G>(brp[br], brp[br + 1], lane, bci, tw, lds + wave * S * CAP, ld, C, ldc,
+                          br * T, M, col0);
+    }
+  } else {
+    const int br = (x - E) * SW + gsub;  // subgroup class
+    if (br < MB) {
+      const int t0 = brp[br], t1 = brp[br + 1];
+      if (t1 - t0 < TA)  // else listed
+        brow_stream<T, TB, G>(t0, t1, lane_sub, bci, tw, lds + gsub * CAP, ld, C, ldc, br * T,
+                              M, col0);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // every workgroup has read the counts and lists before it counts itself
+    if (atomicAdd(&state[1], 1) == static_cast<int>(gridDim.x * gridDim.y) - 1) {
+      atomicExch(&state[0], 0);
+      atomicExch(&state[2], 0);
+      atomicExch(&state[1], 0);
+    }
+  }
+}
+
 // Narrow or unaligned N (N not a multiple of one 16-byte vector, or B / C /
 // ld not 16-byte aligned; N = 1 is a binary-A SpMV): L lanes per block row
 // (64, or a workgroup for T >= 16), lane (slot, j) keeps tile row j and takes
@@ -1009,10 +1158,10 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
 constexpr int kWideCap = 2048;  // list entries (8 KB)
 
 template <int T, typename TB, int G, bool kBuf>
-__global__ void __launch_bounds__(kB2Threads)
-b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span, int M,
-                 const int *__restrict__ bci, const typename Word<T>::type *__restrict__ words,
-                 const TB *__restrict__ B, int ldb, int b_bytes, float *__restrict__ C, int ldc) {
+__device__ __forceinline__ void wide_brow(int br, int2 sp, int M, const int *__restrict__ bci,
+                                          const typename Word<T>::type *__restrict__ words,
+                                          const TB *__restrict__ B, int ldb, int b_bytes,
+                                          float *__restrict__ C, int ldc) {
   constexpr int EPL = Vec16<TB>::kElems;
   constexpr int S = kWave / G, SW = kB2Waves * S;
   constexpr int RPS = T >= SW ? T / SW : 1;  // rows per subgroup
@@ -1026,8 +1175,6 @@ b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span, i
   const int lane_sub = lane & (G - 1), sub = lane / G, gsub = wave * S + sub;
   const int col0 = blockIdx.y * G * EPL + lane_sub * EPL;
   const RowLoader<TB, kBuf> ld(B, ldb, col0, b_bytes);
-  const int br = brows[blockIdx.x];
-  const int2 sp = span[blockIdx.x];
   const int r0 = gsub * RPS;  // this subgroup's first row
   const bool owner = r0 < T;
   float acc[RPS][EPL];
@@ -1143,6 +1290,48 @@ b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span, i
   for (int i = 0; i < RPS; ++i) {
     const int row = br * T + r0 + i;
     if (r0 + i < T && row < M) store_row<EPL>(C, ldc, row, col0, acc[i]);
+  }
+}
+
+// Planned (brows != nullptr): workgroup x takes the plan's block row x.
+// Plan-free: the block rows of at least heavy_tiles tiles, listed by
+// b2sr_class_kernel (state + kStateHead + MB, count state[2]), go first to
+// the E workgroups resident from the start (x, x + E, ...); workgroup E + k
+// then takes block row k unless it is listed. The last workgroup to finish
+// zeroes the list count.
+template <int T, typename TB, int G, bool kBuf>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span,
+                 const int *__restrict__ brp, int E, int heavy_tiles,
+                 int *__restrict__ state, int M, const int *__restrict__ bci,
+                 const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
+                 int ldb, int b_bytes, float *__restrict__ C, int ldc) {
+  const int x = blockIdx.x;
+  if (brows) {
+    wide_brow<T, TB, G, kBuf>(brows[x], span[x], M, bci, words, B, ldb, b_bytes, C, ldc);
+    return;
+  }
+  const int MB = (M + T - 1) / T;
+  if (x < E) {
+    const int n = state[2];
+    for (int j = x; j < n; j += E) {
+      const int br = state[kStateHead + MB + j];
+      wide_brow<T, TB, G, kBuf>(br, make_int2(brp[br], brp[br + 1]), M, bci, words, B, ldb,
+                                b_bytes, C, ldc);
+    }
+  } else {
+    const int br = x - E;
+    const int2 sp = make_int2(brp[br], brp[br + 1]);
+    if (sp.y - sp.x < heavy_tiles)
+      wide_brow<T, TB, G, kBuf>(br, sp, M, bci, words, B, ldb, b_bytes, C, ldc);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // every workgroup has read the count and list before it counts itself
+    if (atomicAdd(&state[1], 1) == static_cast<int>(gridDim.x * gridDim.y) - 1) {
+      atomicExch(&state[2], 0);
+      atomicExch(&state[1], 0);
+    }
   }
 }
 
@@ -1293,24 +1482,87 @@ static void dispatch_fbf(const Geo &g, int M, int N, const int *brp, const int *
 // The long-block-row list of the handle, sized for MB block rows. Growing
 // allocates zeroed counters (synchronously); afterwards the kernels keep
 // the counters at zero between calls.
+// The handle's plan-free state for MB block rows: kStateHead counters and
+// two lists of MB block rows; zeroed when (re)allocated.
 static int b2sr_state(spmm_handle_s *h, int MB, int **out) {
-  if (h->b2sr_cap < MB) {
+  const size_t need = kStateHead + 2 * static_cast<size_t>(MB);
+  if (h->b2sr_cap < need) {
     if (h->b2sr_state) {
       if (hipDeviceSynchronize() != hipSuccess) SPMM_FAIL(SPMM_ERR_HIP, "sync before regrowing");
       (void)hipFree(h->b2sr_state);
       h->b2sr_state = nullptr;
       h->b2sr_cap = 0;
     }
-    const int cap = MB + MB / 4 + 64;
-    if (hipMalloc(&h->b2sr_state, sizeof(int) * (static_cast<size_t>(cap) + 2)) != hipSuccess)
-      SPMM_FAIL(SPMM_ERR_ALLOC, "long-row list allocation failed");
-    if (hipMemset(h->b2sr_state, 0, sizeof(int) * 2) != hipSuccess)
-      SPMM_FAIL(SPMM_ERR_HIP, "long-row list init");
+    const size_t cap = need + need / 4 + 64;
+    if (hipMalloc(&h->b2sr_state, sizeof(int) * cap) != hipSuccess)
+      SPMM_FAIL(SPMM_ERR_ALLOC, "plan-free state allocation failed");
+    if (hipMemset(h->b2sr_state, 0, sizeof(int) * cap) != hipSuccess)
+      SPMM_FAIL(SPMM_ERR_HIP, "plan-free state init");
     h->b2sr_cap = cap;
   }
   *out = h->b2sr_state;
   return SPMM_OK;
 }
+
+// Workgroups of `kernel` the chip holds at once (per-CU occupancy, cached
+// per kernel, times the CU count).
+static int resident_wgs(const void *kernel, int cus, std::atomic<int> *cache) {
+  int per_cu = cache->load(std::memory_order_relaxed);
+  if (per_cu == 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kB2Threads, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+      per_cu = 4;
+    cache->store(per_cu, std::memory_order_relaxed);
+  }
+  return per_cu * cus;
+}
+
+template <int T, typename TB, int G>
+static void launch_dropin(int M, int K, const int *brp, const int *bci, const void *val,
+                          const TB *B, int ldb, float *C, int ldc, int tiles, int *state,
+                          int cus, hipStream_t s) {
+  constexpr int SW = kB2Waves * (kWave / G);
+  const int MB = (M + T - 1) / T;
+  hipLaunchKernelGGL(b2sr_class_kernel, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0, s,
+                     MB, brp, BrowChunk<G>::kTiles, BrowChunk<G>::kWaveTiles, state);
+  const auto *tw = static_cast<const typename TileWord<T>::type *>(val);
+  const int b_bytes = buffer_span(K, ldb, sizeof(TB));
+  auto *const kern = b_bytes ? &b2sr_dropin_kernel<T, TB, G, true>
+                             : &b2sr_dropin_kernel<T, TB, G, false>;
+  static std::atomic<int> per_cu[2];
+  // listed work on the workgroups resident from the start (no more than
+  // there are rows to list), then one workgroup per SW block rows
+  const int E = std::min(resident_wgs(reinterpret_cast<const void *>(kern), cus,
+                                      &per_cu[b_bytes ? 1 : 0]),
+                         M);
+  const dim3 grid(E + ceil_div(MB, SW), tiles);
+  hipLaunchKernelGGL(kern, grid, dim3(kB2Threads), 0, s, M, E, brp, bci, tw, B, ldb, b_bytes, C,
+                     ldc, state);
+}
+
+template <int T, typename TB>
+static void dispatch_dropin(const Geo &g, int M, int K, const int *brp, const int *bci,
+                            const void *val, const TB *B, int ldb, float *C, int ldc,
+                            int *state, int cus, hipStream_t s) {
+#define SPMM_DI(gg) \
+  launch_dropin<T, TB, gg>(M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, state, cus, s)
+  switch (g.G) {
+    case 1: SPMM_DI(1); break;
+    case 2: SPMM_DI(2); break;
+    case 4: SPMM_DI(4); break;
+    case 8: SPMM_DI(8); break;
+    case 16: SPMM_DI(16); break;
+    case 32: SPMM_DI(32); break;
+    default: SPMM_DI(64); break;
+  }
+#undef SPMM_DI
+}
+
+template <int T, typename TB>
+static void dispatch_wide(const Geo &g, const spmm_csr_plan_s *p, const int *brp, int *state,
+                          int cus, int M, int K, const int *bci, const void *val, const TB *B,
+                          int ldb, float *C, int ldc, hipStream_t s);
 
 template <typename TB>
 static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp, const int *bci,
@@ -1324,15 +1576,40 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
   SPMM_CHECK_ARG(h != nullptr, "null handle");
   hipStream_t s = as_stream(stream);
   Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
+  // the planned form's kernels when bsrVal has their alignment (T bytes for
+  // T <= 8, 16 bytes above)
+  const uintptr_t va = reinterpret_cast<uintptr_t>(val);
+  const bool classed = g.G && (T > 8 ? va % 16 == 0 : va % T == 0);
   int *state = nullptr;
-  int long_grid = 0;
+  int long_grid = 0, cus = 0;
   if (g.G) {
     const int st = b2sr_state(h, (M + T - 1) / T, &state);
     if (st) return st;
-    int dev = 0, cus = 0;
+    int dev = 0;
     SPMM_HIP(hipGetDevice(&dev));
     SPMM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     long_grid = 8 * cus;
+  }
+  if (classed && T > 8) {
+#define SPMM_WP(t) \
+  dispatch_wide<t, TB>(g, nullptr, brp, state, cus, M, K, bci, val, B, ldb, C, ldc, s)
+    switch (T) {
+      case 16: SPMM_WP(16); break;
+      case 32: SPMM_WP(32); break;
+      default: SPMM_WP(64); break;
+    }
+#undef SPMM_WP
+    SPMM_LAUNCHED();
+    return SPMM_OK;
+  }
+  if (classed) {
+    switch (T) {
+      case 2: dispatch_dropin<2, TB>(g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
+      case 4: dispatch_dropin<4, TB>(g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
+      default: dispatch_dropin<8, TB>(g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
+    }
+    SPMM_LAUNCHED();
+    return SPMM_OK;
   }
 #define SPMM_FBF(t) dispatch_fbf<t, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, state, long_grid, s)
   switch (T) {
@@ -1388,35 +1665,48 @@ static void dispatch_brow(const Geo &g, const spmm_csr_plan_s *p, int M, int K, 
   }
 }
 
+// p == nullptr: the plan-free form (block rows of at least wide_heavy(T)
+// tiles listed first, on `state`).
 template <int T, typename TB, int G>
-static void launch_wide(const spmm_csr_plan_s *p, int K, const int *bci, const void *val,
-                        const TB *B, int ldb, float *C, int ldc, int tiles, int M,
-                        hipStream_t s) {
+static void launch_wide(const spmm_csr_plan_s *p, const int *brp, int *state, int cus, int K,
+                        const int *bci, const void *val, const TB *B, int ldb, float *C,
+                        int ldc, int tiles, int M, hipStream_t s) {
   const int MB = (M + T - 1) / T;
   const auto *w = static_cast<const typename Word<T>::type *>(val);
   const int b_bytes = buffer_span(K, ldb, sizeof(TB));
-  if (b_bytes)
-    hipLaunchKernelGGL((b2sr_wide_kernel<T, TB, G, true>), dim3(MB, tiles), dim3(kB2Threads), 0,
-                       s, p->d_rows, p->d_span, M, bci, w, B, ldb, b_bytes, C, ldc);
-  else
-    hipLaunchKernelGGL((b2sr_wide_kernel<T, TB, G, false>), dim3(MB, tiles), dim3(kB2Threads),
-                       0, s, p->d_rows, p->d_span, M, bci, w, B, ldb, 0, C, ldc);
+  auto *const kern =
+      b_bytes ? &b2sr_wide_kernel<T, TB, G, true> : &b2sr_wide_kernel<T, TB, G, false>;
+  int E = 0;
+  if (!p) {
+    hipLaunchKernelGGL(b2sr_class_kernel, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0,
+                       s, MB, brp, wide_heavy(T), INT_MAX, state);
+    static std::atomic<int> per_cu[2];
+    E = std::min(resident_wgs(reinterpret_cast<const void *>(kern), cus,
+                              &per_cu[b_bytes ? 1 : 0]),
+                 MB);
+  }
+  hipLaunchKernelGGL(kern, dim3(E + MB, tiles), dim3(kB2Threads), 0, s, p ? p->d_rows : nullptr,
+                     p ? p->d_span : nullptr, brp, E, wide_heavy(T), state, M, bci, w, B, ldb,
+                     b_bytes, C, ldc);
 }
 
 // G = 64 runs as G = 32 over twice the column tiles (eight rows per subgroup
 // at most).
 template <int T, typename TB>
-static void dispatch_wide(const Geo &g, const spmm_csr_plan_s *p, int M, int K, const int *bci,
-                          const void *val, const TB *B, int ldb, float *C, int ldc,
-                          hipStream_t s) {
+static void dispatch_wide(const Geo &g, const spmm_csr_plan_s *p, const int *brp, int *state,
+                          int cus, int M, int K, const int *bci, const void *val, const TB *B,
+                          int ldb, float *C, int ldc, hipStream_t s) {
+#define SPMM_WD(gg, tl) \
+  launch_wide<T, TB, gg>(p, brp, state, cus, K, bci, val, B, ldb, C, ldc, tl, M, s)
   switch (g.G) {
-    case 1: launch_wide<T, TB, 1>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
-    case 2: launch_wide<T, TB, 2>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
-    case 4: launch_wide<T, TB, 4>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
-    case 8: launch_wide<T, TB, 8>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
-    case 16: launch_wide<T, TB, 16>(p, K, bci, val, B, ldb, C, ldc, g.tiles, M, s); break;
-    default: launch_wide<T, TB, 32>(p, K, bci, val, B, ldb, C, ldc, g.tiles * (g.G / 32), M, s); break;
+    case 1: SPMM_WD(1, g.tiles); break;
+    case 2: SPMM_WD(2, g.tiles); break;
+    case 4: SPMM_WD(4, g.tiles); break;
+    case 8: SPMM_WD(8, g.tiles); break;
+    case 16: SPMM_WD(16, g.tiles); break;
+    default: SPMM_WD(32, g.tiles * (g.G / 32)); break;
   }
+#undef SPMM_WD
 }
 
 template <typename TB>
@@ -1459,9 +1749,9 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     case 2: dispatch_brow<2, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 16: dispatch_wide<16, TB>(g, p, M, K, bci, val, B, ldb, C, ldc, s); break;
-    case 32: dispatch_wide<32, TB>(g, p, M, K, bci, val, B, ldb, C, ldc, s); break;
-    default: dispatch_wide<64, TB>(g, p, M, K, bci, val, B, ldb, C, ldc, s); break;
+    case 16: dispatch_wide<16, TB>(g, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s); break;
+    case 32: dispatch_wide<32, TB>(g, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s); break;
+    default: dispatch_wide<64, TB>(g, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s); break;
   }
   SPMM_LAUNCHED();
   return SPMM_OK;

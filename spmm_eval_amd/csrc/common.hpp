@@ -59,10 +59,11 @@ struct spmm_handle_s {
   size_t ws_bytes = 0;
   void *pinned = nullptr;  // small pinned host buffer for count read-backs
   size_t pinned_bytes = 0;
-  // B2SR long-block-row list: [count, done, list...]; counters start at 0
-  // and are reset by the kernel that drains the list.
+  // plan-free B2SR state: counters, block-row lists and per-column-tile
+  // counters; counters start at 0 and are reset by the kernel that drains
+  // the lists.
   int *b2sr_state = nullptr;
-  int b2sr_cap = 0;
+  size_t b2sr_cap = 0;
 };
 
 namespace spmm {

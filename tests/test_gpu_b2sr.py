@@ -224,7 +224,7 @@ def test_dense_packers(dev, H, W):
     assert np_u32(api.pack_bin_rows(Xh)).reshape(-1).tolist() == sign["bin"]
 
 
-@pytest.mark.parametrize("T", [2, 4, 32, 64])
+@pytest.mark.parametrize("T", [2, 4, 8, 32, 64])
 @pytest.mark.parametrize("N", [64, 512])
 def test_fbf_long_block_rows_repeated(handle, dev, T, N):
     """Block rows long enough to go through the long-row list (N=512 adds a
@@ -245,6 +245,69 @@ def test_fbf_long_block_rows_repeated(handle, dev, T, N):
             if mat is g:
                 outs.append(C)
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
+def test_fbf_dropin_unaligned_words(handle, dev, T):
+    """bsrVal one word off the planned kernels' alignment (T bytes for T <= 8,
+    16 above): the drop-in takes its row-per-subgroup kernels instead; +-1 B
+    exact against integer CSR SpMM and equal to the aligned call."""
+    N = 64
+    g = graphs.random_csr(400, 3000, 0.01, seed=T, long_rows=3, long_len=2500)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    esz = words.element_size()
+    raw = torch.zeros(words.numel() * esz + esz, dtype=torch.uint8, device=dev)
+    raw[esz:] = words.view(torch.uint8)
+    off = raw[esz:].view(words.dtype)  # one word further on
+    assert off.data_ptr() % (T if T <= 8 else 16) != 0
+    S = np.random.default_rng(T).choice([-1, 1], size=(g.K, N)).astype(np.float32)
+    Bd = torch.from_numpy(S).to(dev)
+    exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S.astype(np.int32)).astype(np.float32)
+    C_off = api.spmm_b2sr(handle, T, g.M, brp, bci, off, Bd)
+    np.testing.assert_array_equal(C_off.cpu().numpy(), exact)
+    C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd)
+    assert torch.equal(C, C_off)
+
+
+@pytest.mark.parametrize("T", [2, 4, 8, 16])
+def test_fbf_dropin_graph_replays(handle, dev, b2sr_graph, T):
+    """The plan-free form captured in a hipGraph (two calls) and replayed
+    three times, with an eager call on another matrix between replays: its
+    device counters and class lists must be back to their idle state after
+    every call, so every replay equals the eager result."""
+    g = b2sr_graph
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    other = graphs.random_csr(300, 8192, 0.01, seed=7, long_rows=2, long_len=4000)
+    orp, oci = dev_csr(other, dev)
+    obrp, obci, owords = api.csr2b2sr(handle, orp, oci, other.M, other.K, T)
+    Bo = torch.from_numpy(graphs.uniform_pm1((other.K, 64), seed=3)).to(dev)
+    Bd = torch.from_numpy(graphs.uniform_pm1((g.K, 64), seed=T)).to(dev)
+    C = torch.empty(g.M, 64, device=dev)
+    api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, C_out=C)
+    torch.cuda.synchronize()
+    ref = C.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, C_out=C)
+        s.synchronize()
+        with torch.cuda.graph(graph, stream=s):
+            api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, C_out=C)
+            api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, C_out=C)
+    torch.cuda.synchronize()
+    o_ref = api.spmm_b2sr(handle, T, other.M, obrp, obci, owords, Bo)
+    for rep in range(3):
+        C.fill_(0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(C, ref), rep
+        o = api.spmm_b2sr(handle, T, other.M, obrp, obci, owords, Bo)
+        torch.cuda.synchronize()
+        assert torch.equal(o, o_ref), rep
 
 
 @pytest.mark.parametrize("T", [2, 4, 16, 64])
