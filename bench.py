@@ -419,6 +419,20 @@ def extra_fwd_bwd(args, h, dev):
         bwd_x(GG)
         bwd_v(XX, GG)
 
+    # the backward as one pass over A's transposed plan (dX and dval together)
+    pt = api.CsrTPlan(h, rp, ci, g.K)
+    dX2 = torch.empty((g.K, N), device=dev)
+    dval2 = torch.empty(g.nnz, device=dev)
+
+    def bwd_fused(XX, GG, tune):
+        api.spmm_csr_t_sddmm(pt, rp, ci, v, GG, XX, dX_out=dX2, dval_out=dval2, tune=tune)
+
+    def tune_fused(XX, GG):
+        lanes = N // (16 // XX.element_size())
+        cands = [(lanes // vec, vec, rs) for vec in (1, 2) for rs in (1, 2, 4)
+                 if lanes % vec == 0 and lanes // vec <= 64]
+        return min(cands, key=lambda t: time_loop(lambda: bwd_fused(XX, GG, t), 30, 3))
+
     res = {}
     for name, XX, GG, s in (("f32", Xd, dYd, 4), ("bf16", Xh, dYh, 2)):
         geo = (pf.autotune(rp, ci, v, XX, Y), pb.autotune(trp, tci, tval, GG, dX, K=g.M))
@@ -426,6 +440,15 @@ def extra_fwd_bwd(args, h, dev):
         parts = {"fwd_ms": round(time_loop(lambda: fwd(XX), 100, 10), 4),
                  "dX_ms": round(time_loop(lambda: bwd_x(GG), 100, 10), 4),
                  "sddmm_ms": round(time_loop(lambda: bwd_v(XX, GG), 100, 10), 4)}
+        tf = tune_fused(XX, GG)
+        fused_ms = time_loop(lambda: (fwd(XX), bwd_fused(XX, GG, tf)), 100, 10)
+        torch.cuda.synchronize()
+        parts["fused"] = {"step_ms": round(fused_ms, 4),
+                          "bwd_ms": round(time_loop(lambda: bwd_fused(XX, GG, tf), 100, 10), 4),
+                          "geometry": list(tf),
+                          "max_abs_diff_dX": float((dX2 - dX).abs().max()),
+                          "max_rel_diff_dval": float(((dval2 - dval).abs().max()
+                                                      / dval.abs().max().clamp_min(1e-30)))}
         nbytes = 2 * csr_bytes(g.M, g.nnz, N, s_b=s) + sddmm_bytes(g.M, g.nnz, N, s)
         rows = np.random.default_rng(7).choice(g.M, 1500, replace=False)
         Yc, dvc = Y.cpu().numpy(), dval.cpu().numpy()
@@ -439,6 +462,11 @@ def extra_fwd_bwd(args, h, dev):
         res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1), "geometry": geo, **parts,
                      "rel_err_Y_fp64": e_y, "rel_err_dval_fp64": (num / den) ** 0.5}
     res["bf16_speedup_vs_f32"] = round(res["f32"]["ms"] / res["bf16"]["ms"], 3)
+    # BF16 step with the fused backward against the plain FP32 step (three
+    # separate FP32 launches) and against the FP32 step with the fused backward
+    res["bf16_fused_vs_f32_plain"] = round(res["f32"]["ms"] / res["bf16"]["fused"]["step_ms"], 3)
+    res["bf16_fused_vs_f32_fused"] = round(res["f32"]["fused"]["step_ms"]
+                                           / res["bf16"]["fused"]["step_ms"], 3)
     return {"gcn_fwd_bwd_89k_n128": res}
 
 

@@ -286,6 +286,28 @@ int spmm_csr_t_bf16(spmm_handle_t h, int M, int K, int N, int nnz,
                     const uint16_t *B, int ldb, float *C, int ldc,
                     void *stream);
 
+/* A training step's backward in one pass over A's transposed plan (no
+ * counterpart in the reference, which has no backward): dX[K][N] = A^T dY
+ * and dval[e] = sum_k dY[i][k] * X[j][k] for stored (i, j) = entry e. The
+ * dY row gathered for dX is dotted with X[j], held in registers, so the
+ * separate SDDMM's gather of X rows is not needed. Fused when one subgroup
+ * covers N (tune: lanes x vec x 16 bytes = N, or null for the default) with
+ * 16-byte aligned dY, X, dX and leading dimensions; otherwise the transposed
+ * SpMM and the plan-free SDDMM run in turn (rowptr / colind: A's CSR, read
+ * only then). dY [M][lddy], X [K][ldx], dX [K][lddx]. Asynchronous. */
+int spmm_csr_t_sddmm_f32_planned(spmm_csrt_plan_t plan, int M, int K, int N,
+                                 int nnz, const int *rowptr, const int *colind,
+                                 const float *val, const float *dY, int lddy,
+                                 const float *X, int ldx, float *dX, int lddx,
+                                 float *dval, const spmm_tune_t *tune,
+                                 void *stream);
+int spmm_csr_t_sddmm_bf16_planned(spmm_csrt_plan_t plan, int M, int K, int N,
+                                  int nnz, const int *rowptr, const int *colind,
+                                  const float *val, const uint16_t *dY, int lddy,
+                                  const uint16_t *X, int ldx, float *dX, int lddx,
+                                  float *dval, const spmm_tune_t *tune,
+                                  void *stream);
+
 /* SDDMM at the stored positions: dval[e] = sum_k dY[i][k] * X[j][k] for
  * stored (i, j) = entry e, FP32 accumulation. dY [M][lddy], X [K][ldx]. */
 int sddmm_csr_f32(spmm_handle_t h, int M, int K, int N, int nnz,

@@ -77,6 +77,10 @@ _SIGS = {
     "spmm_csr_t_autotune": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp, _i, C.POINTER(Tune), _vp],
     "spmm_csr_t_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_csr_t_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp],
+    "spmm_csr_t_sddmm_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp,
+                                     _i, _vp, _vp, _vp],
+    "spmm_csr_t_sddmm_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp,
+                                      _i, _vp, _vp, _vp],
     "spmm_csr_t_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_csr_t_bf16": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_mtx_read_info": [C.c_char_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],

@@ -321,6 +321,24 @@ def csr_transpose(h: Handle, rowptr, colind, val, K: int, stream=None):
     return trp, tci[:nnz], (tv[:nnz] if tv is not None else None), perm[:nnz]
 
 
+def spmm_csr_t_sddmm(plan: CsrTPlan, rowptr, colind, val, dY, X, dX_out=None, dval_out=None,
+                     tune: tuple | None = None, stream=None):
+    """A training step's backward in one pass over A's transposed plan:
+    dX = A^T dY [K][N] and dval[e] = dY[row(e)] . X[col(e)] (FP32 outputs;
+    dY and X both FP32 or both BF16). Returns (dX, dval)."""
+    M, K, nnz = plan.M, plan.K, plan.nnz
+    N = dY.shape[1]
+    if dX_out is None:
+        dX_out = torch.empty((K, N), dtype=torch.float32, device=dY.device)
+    if dval_out is None:
+        dval_out = torch.empty(max(nnz, 1), dtype=torch.float32, device=dY.device)[:nnz]
+    fn = "spmm_csr_t_sddmm_bf16_planned" if _is_bf16(dY) else "spmm_csr_t_sddmm_f32_planned"
+    t = C.byref(Tune(*tune)) if tune else None
+    call(fn, plan.p, M, K, N, nnz, _p(rowptr), _p(colind), _p(val), _p(dY), dY.stride(0),
+         _p(X), X.stride(0), _p(dX_out), dX_out.stride(0), _p(dval_out), t, _stream(stream))
+    return dX_out, dval_out
+
+
 def sddmm_csr(h: Handle, rowptr, colind, dY, X, out=None, plan: CsrPlan | None = None,
               stream=None):
     """dval[e] = dY[row(e)] . X[col(e)] at the stored positions (FP32 out).

@@ -215,7 +215,7 @@ static void launch_sddmm_rows(const spmm_csr_plan_s *p, const int *rowptr, const
 
 // plan == nullptr: entry-chunked (no schedule needed); else rows of the plan.
 template <typename TB>
-static int run_sddmm(const spmm_csr_plan_s *plan, int M, int K, int N, int nnz,
+int run_sddmm(const spmm_csr_plan_s *plan, int M, int K, int N, int nnz,
                      const int *rowptr, const int *colind, const TB *dY, int lddy, const TB *X,
                      int ldx, float *out, void *stream) {
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0 && nnz >= 0, "negative size");
@@ -270,6 +270,13 @@ static int run_sddmm(const spmm_csr_plan_s *plan, int M, int K, int N, int nnz,
   SPMM_LAUNCHED();
   return SPMM_OK;
 }
+
+template int run_sddmm<float>(const spmm_csr_plan_s *, int, int, int, int, const int *,
+                              const int *, const float *, int, const float *, int, float *,
+                              void *);
+template int run_sddmm<uint16_t>(const spmm_csr_plan_s *, int, int, int, int, const int *,
+                                 const int *, const uint16_t *, int, const uint16_t *, int,
+                                 float *, void *);
 
 }  // namespace spmm
 

@@ -495,6 +495,12 @@ void build_table(const int *counts, const int *offs, int G, int rows_scale, Laun
 
 struct spmm_csr_plan_s;
 namespace spmm {
+// SDDMM dval[e] = dY[i] . X[colind[e]] (plan == nullptr: entry-chunked;
+// else on the rows of a CSR plan); defined for float and uint16_t.
+template <typename TB>
+int run_sddmm(const spmm_csr_plan_s *plan, int M, int K, int N, int nnz, const int *rowptr,
+              const int *colind, const TB *dY, int lddy, const TB *X, int ldx, float *out,
+              void *stream);
 // Bins rows by degree rowptr[(r >> shift) + 1] - rowptr[r >> shift].
 int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
             spmm_csr_plan_s **out, void *stream);

@@ -11,7 +11,7 @@ import torch
 
 import oracle
 from spmm_eval_amd import api, graphs
-from spmm_eval_amd._lib import SpmmError
+from spmm_eval_amd._lib import SpmmError, call
 
 pytestmark = pytest.mark.gpu
 HAND = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "b2sr_hand.json")))
@@ -272,4 +272,80 @@ def test_spmm_csr_t_plan_mismatch(handle, dev):
     plan = api.CsrTPlan(handle, rp, ci, 80)
     with pytest.raises(SpmmError) as ei:
         api.spmm_csr_t(handle, rp, ci, v, torch.ones(100, 8, device=dev), 81, plan=plan)
+    assert ei.value.status == 5
+
+
+@pytest.mark.parametrize("M,K,dens,nl,ll,N,bf16", [
+    (2000, 1500, 0.004, 3, 1500, 128, False),   # fused, 32 lanes
+    (2000, 1500, 0.004, 3, 1500, 128, True),    # fused, 16 lanes
+    (777, 3000, 0.003, 2, 3000, 64, False),
+    (500, 400, 0.02, 2, 400, 512, False),       # fused, 64 lanes x 2 vectors
+    (500, 400, 0.02, 2, 400, 1024, False),      # one subgroup cannot cover N: two passes
+    (300, 200, 0.05, 2, 200, 7, False),         # narrow N: two passes
+    (1, 5, 1.0, 0, 0, 4, False),
+])
+def test_t_sddmm_fused(handle, dev, M, K, dens, nl, ll, N, bf16):
+    """dX = A^T dY and dval = SDDMM(dY, X) in one pass: dX bit for bit the
+    transposed SpMM on the same geometry (same entry order and sums), dval
+    within 1e-5 of FP64 (of the BF16-rounded operands for BF16); the
+    two-pass fallback equals the separate calls bit for bit."""
+    g = graphs.random_csr(M, K, dens, seed=M + K + N, long_rows=nl, long_len=ll)
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    dY = torch.from_numpy(graphs.normal((M, N), 3, N)).to(dev)
+    X = torch.from_numpy(graphs.normal((K, N), 4, N)).to(dev)
+    if bf16:
+        dY, X = api.to_bf16_bits(dY), api.to_bf16_bits(X)
+    plan = api.CsrTPlan(handle, rp, ci, K)
+    dX, dval = api.spmm_csr_t_sddmm(plan, rp, ci, v, dY, X)
+    e16 = 8 if bf16 else 4
+    lanes = N // e16 if N % e16 == 0 else 0
+    fused = lanes > 0 and (lanes & (lanes - 1)) == 0 and lanes <= 128
+    if fused:
+        vec = 2 if lanes > 64 else 1
+        dX_ref = api.spmm_csr_t(handle, rp, ci, v, dY, K, plan=plan, tune=(lanes // vec, vec, 1))
+    else:
+        dX_ref = api.spmm_csr_t(handle, rp, ci, v, dY, K, plan=plan)
+        assert torch.equal(dval, api.sddmm_csr(handle, rp, ci, dY, X))
+    torch.cuda.synchronize()
+    assert torch.equal(dX, dX_ref)
+    to32 = (lambda t: oracle.bf16_to_f32(t.view(torch.int16).cpu().numpy().view(np.uint16))) \
+        if bf16 else (lambda t: t.cpu().numpy())
+    ref = oracle.sddmm_f64(g.rowptr, g.colind, to32(dY), to32(X))
+    n = np.linalg.norm(ref)
+    assert np.linalg.norm(dval.cpu().numpy() - ref) / (n if n > 0 else 1.0) <= 1e-5
+    cp, ri, cv, _ = oracle.csr2csc(g.rowptr, g.colind, g.val, K)
+    ref_dX = oracle.csr_spmm_f64(cp, ri, cv, to32(dY))
+    nx = np.linalg.norm(ref_dX)
+    assert np.linalg.norm(dX.cpu().numpy() - ref_dX) / (nx if nx > 0 else 1.0) <= 1e-5
+
+
+def test_t_sddmm_fused_gcn_and_tune(handle, dev, gcn):
+    """The GCN workload's backward (N = 128) fused, FP32 and BF16, against
+    FP64, on the default geometry and on an explicit one; a tune that does
+    not cover N in one subgroup takes the two-pass path; a plan of another
+    shape is refused."""
+    g, X, dY = gcn
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    plan = api.CsrTPlan(handle, rp, ci, g.K)
+    cp, ri, cv, _ = oracle.csr2csc(g.rowptr, g.colind, g.val, g.K)
+    Xd, dYd = torch.from_numpy(X).to(dev), torch.from_numpy(dY).to(dev)
+    for bf16 in (False, True):
+        Xo, dYo = (api.to_bf16_bits(Xd), api.to_bf16_bits(dYd)) if bf16 else (Xd, dYd)
+        if bf16:
+            Xq = oracle.bf16_to_f32(Xo.view(torch.int16).cpu().numpy().view(np.uint16))
+            dYq = oracle.bf16_to_f32(dYo.view(torch.int16).cpu().numpy().view(np.uint16))
+        else:
+            Xq, dYq = X, dY
+        ref_dX = oracle.csr_spmm_f64(cp, ri, cv, dYq)
+        ref_dv = oracle.sddmm_f64(g.rowptr, g.colind, dYq, Xq)
+        for tune in (None, (16, 2, 1) if not bf16 else (8, 2, 1), (8, 1, 1)):
+            dX, dval = api.spmm_csr_t_sddmm(plan, rp, ci, v, dYo, Xo, tune=tune)
+            assert np.linalg.norm(dX.cpu().numpy() - ref_dX) / np.linalg.norm(ref_dX) <= 1e-5
+            assert np.linalg.norm(dval.cpu().numpy() - ref_dv) / np.linalg.norm(ref_dv) <= 1e-5
+    dX = torch.empty(g.K, 128, device=dev)
+    dval = torch.empty(g.nnz, device=dev)
+    with pytest.raises(SpmmError) as ei:  # the plan was built for g.M rows
+        call("spmm_csr_t_sddmm_f32_planned", plan.p, g.M + 1, g.K, 128, g.nnz, rp.data_ptr(),
+             ci.data_ptr(), v.data_ptr(), dYd.data_ptr(), 128, Xd.data_ptr(), 128, dX.data_ptr(),
+             128, dval.data_ptr(), None, None)
     assert ei.value.status == 5
