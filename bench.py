@@ -504,8 +504,16 @@ def extra_batched(args, h, dev):
 
     ms_sep = time_loop(separate, 5, 1)
     info = plan.info()
+    # each SDDMM form at the tuned geometry (form 2 only on one column tile)
+    forms = {}
+    for form in (0, 1, 2):
+        if form == 2 and info["lanes_per_row"] * info["vec"] * 8 != N:
+            continue
+        plan.set_geometry(info["lanes_per_row"], info["vec"], info["rows_scale"],
+                          info["persistent"], form)
+        forms[str(form)] = round(time_loop(lambda: plan.step(Xh, dYh, use_graph=True), 20, 3), 4)
     plan.close()
-    return {"batched_512_fwd_bwd_n64": {"fused_launch": info,
+    return {"batched_512_fwd_bwd_n64": {"fused_launch": info, "sddmm_form_ms": forms,
         "graphs": len(gs), "rows": M, "nnz": plan.nnz,
         "bf16_one_graph_ms": round(ms_graph, 4), "f32_separate_launches_ms": round(ms_sep, 3),
         "speedup": round(ms_sep / ms_graph, 2)}}

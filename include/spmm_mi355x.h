@@ -364,12 +364,19 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off,
                            int bf16, spmm_batch_plan_t *plan, void *stream);
 int spmm_batch_plan_destroy(spmm_batch_plan_t plan);
 /* The geometry the plan's autotuner chose (lanes per row, vector width,
- * rows per subgroup), whether the fused launch is persistent, whether its
- * SDDMM runs on the rows of the forward plan (1) or in fixed entry chunks
- * (0; both give the same bits), and the tuned time of one step in ms. Any
+ * rows per subgroup), whether the fused launch is persistent, its SDDMM form
+ * (0: fixed entry chunks; 1: the rows of the forward plan, the same bits as
+ * 0; 2: no SDDMM items, the backward items form dval beside dX in one pass
+ * over the transposed rows, which needs lanes x vec x 16 bytes = N; dval
+ * then equals spmm_csr_t_sddmm_*_planned per graph bit for bit and the
+ * other forms to FP32 rounding), and the tuned time of one step in ms. Any
  * output pointer may be NULL. */
 int spmm_batch_plan_info(spmm_batch_plan_t plan, spmm_tune_t *tune,
                          int *persistent, int *sddmm_rows, float *tuned_ms);
+/* Overrides the autotuner's choice (same meanings and limits as above;
+ * SPMM_ERR_INVALID_ARG for a combination the launch cannot run). */
+int spmm_batch_plan_set(spmm_batch_plan_t plan, const spmm_tune_t *tune,
+                        int persistent, int sddmm_form);
 /* One fwd+bwd step; features are uint16 BF16 when the plan is bf16, else
  * float. Y, dX are FP32 [total_rows][N]; dval FP32 [total_nnz]. */
 int spmm_batch_fwd_bwd(spmm_batch_plan_t plan, const void *X, const void *dY,

@@ -89,6 +89,7 @@ _SIGS = {
     "sddmm_csr_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_batch_plan_create": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, C.POINTER(_vp), _vp],
     "spmm_batch_plan_destroy": [_vp],
+    "spmm_batch_plan_set": [_vp, _vp, _i, _i],
     "spmm_batch_plan_info": [_vp, C.POINTER(Tune), C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_float)],
     "spmm_batch_fwd_bwd": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
 }

@@ -48,8 +48,15 @@ class BatchPlan:
         t, pers, srows, ms = Tune(), C.c_int(), C.c_int(), C.c_float()
         call("spmm_batch_plan_info", self.p, C.byref(t), C.byref(pers), C.byref(srows), C.byref(ms))
         return {"lanes_per_row": t.lanes_per_row, "vec": t.vec, "rows_scale": t.rows_scale,
-                "persistent": bool(pers.value), "sddmm_rows": bool(srows.value),
+                "persistent": bool(pers.value), "sddmm_form": int(srows.value),
                 "tuned_ms": round(ms.value, 4)}
+
+    def set_geometry(self, lanes_per_row: int, vec: int, rows_scale: int, persistent: bool,
+                     sddmm_form: int):
+        """Pin the fused launch's geometry and SDDMM form (0 entry chunks,
+        1 forward rows, 2 fused into the backward items)."""
+        call("spmm_batch_plan_set", self.p, C.byref(Tune(lanes_per_row, vec, rows_scale)),
+             int(persistent), int(sddmm_form))
 
     def step(self, X: torch.Tensor, dY: torch.Tensor, use_graph: bool = True, stream=None):
         call("spmm_batch_fwd_bwd", self.p, _p(X), _p(dY), _p(self.Y), _p(self.dX), _p(self.dval),

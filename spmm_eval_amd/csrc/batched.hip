@@ -25,6 +25,7 @@ struct spmm_batch_plan_s {
   int *gcol = nullptr;          // global column indices (owned)
   int *trowptr = nullptr, *tcol = nullptr;  // transposed union (owned)
   float *tval = nullptr;
+  int *perm = nullptr;          // transposed entry -> source entry (owned)
   int *counters = nullptr;      // [queue, done] of the persistent mode
   spmm_csr_plan_t fwd = nullptr, bwd = nullptr;
   // chosen geometry
@@ -49,24 +50,31 @@ __global__ void globalise_cols_kernel(int ngraphs, const int *__restrict__ row_o
     gcol[e] = colind[e] + off;
 }
 
+// SDDMM forms of the fused launch (all give dval within FP32 rounding of
+// each other; each is bitwise reproducible)
+constexpr int kSddChunks = 0;  // entry-chunked SDDMM items
+constexpr int kSddRows = 1;    // SDDMM items on the forward plan's rows
+constexpr int kSddFused = 2;   // no SDDMM items: the backward items also form dval
+
 struct FusedArgs {
   LaunchTable fwd, bwd, sdd;  // sdd: the forward plan's rows at the SDDMM lane count
   int ctiles;                 // CSR column tiles
   int n_fwd, n_bwd, n_sddmm;  // items of each part (CSR parts include column tiles)
-  int sddmm_rows;             // SDDMM on the forward plan's rows (1) or entry-chunked (0)
+  int sddmm_rows;             // the SDDMM form (kSddChunks, kSddRows or kSddFused)
   int M, N, nnz, sddmm_ntile;
   const int *rows_fwd, *rows_bwd;
   const int2 *span_fwd, *span_bwd;
   const int *rowptr, *colind, *trowptr, *tcol;
   const float *val, *tval;
+  const int *perm;
   const void *X, *dY;
   float *Y, *dX, *dval;
   int *queue, *done;          // persistent mode only
 };
 
-// kSddRows: SDDMM on the forward plan's rows (one form per kernel, so the
-// other form's registers do not lower the occupancy)
-template <typename TB, int G, int VEC, int GS, bool kSddRows>
+// kSdd: the SDDMM form, one per kernel, so another form's registers do not
+// lower the occupancy. kSddFused needs one column tile (ctiles == 1).
+template <typename TB, int G, int VEC, int GS, int kSdd>
 __device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
   const TB *X = static_cast<const TB *>(a.X);
   const TB *dY = static_cast<const TB *>(a.dY);
@@ -76,13 +84,19 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
     return;
   }
   item -= a.n_fwd;
+  if (kSdd == kSddFused) {
+    if (item < a.n_bwd)
+      csr_t_sddmm_wg<TB, G, VEC, true>(a.bwd, item, a.rows_bwd, a.span_bwd, a.tcol, a.tval,
+                                       a.perm, dY, a.N, X, a.N, a.dX, a.N, a.dval);
+    return;
+  }
   if (item < a.n_bwd) {
     csr_wg<TB, G, VEC>(a.bwd, item / a.ctiles, item % a.ctiles, a.rows_bwd, a.span_bwd,
                        a.tcol, a.tval, dY, a.N, a.dX, a.N);
     return;
   }
   item -= a.n_bwd;
-  if (kSddRows)
+  if (kSdd == kSddRows)
     sddmm_row_wg<TB, GS>(a.sdd, item, a.rows_fwd, a.span_fwd, a.colind, dY, a.N, X, a.N,
                          a.dval, a.sddmm_ntile);
   else
@@ -90,11 +104,11 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
                      a.sddmm_ntile);
 }
 
-template <typename TB, int G, int VEC, int GS, bool kPersistent, bool kSddRows>
+template <typename TB, int G, int VEC, int GS, bool kPersistent, int kSdd>
 __global__ void __launch_bounds__(kThreads) fused_step_kernel(FusedArgs a) {
   const int total = a.n_fwd + a.n_bwd + a.n_sddmm;
   if (!kPersistent) {
-    fused_item<TB, G, VEC, GS, kSddRows>(a, blockIdx.x);
+    fused_item<TB, G, VEC, GS, kSdd>(a, blockIdx.x);
     return;
   }
   __shared__ int next;
@@ -104,7 +118,7 @@ __global__ void __launch_bounds__(kThreads) fused_step_kernel(FusedArgs a) {
     const int item = next;
     __syncthreads();
     if (item >= total) break;
-    fused_item<TB, G, VEC, GS, kSddRows>(a, item);
+    fused_item<TB, G, VEC, GS, kSdd>(a, item);
   }
   if (threadIdx.x == 0) {
     // Every workgroup stops fetching before it counts itself done, so the
@@ -123,20 +137,24 @@ static int sddmm_G(int N, int e16) {
   return 0;
 }
 
-template <typename TB, int G, int VEC, int GS, bool kSddRows>
+template <typename TB, int G, int VEC, int GS, int kSdd>
 static void launch_fused_k(const FusedArgs &a, bool persistent, int grid, hipStream_t s) {
   if (persistent)
-    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, true, kSddRows>), dim3(grid),
+    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, true, kSdd>), dim3(grid),
                        dim3(kThreads), 0, s, a);
   else
-    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, false, kSddRows>),
+    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, false, kSdd>),
                        dim3(a.n_fwd + a.n_bwd + a.n_sddmm), dim3(kThreads), 0, s, a);
 }
 
 template <typename TB, int G, int VEC, int GS>
 static int launch_fused_g(const FusedArgs &a, bool persistent, int grid, hipStream_t s) {
-  if (a.sddmm_rows) launch_fused_k<TB, G, VEC, GS, true>(a, persistent, grid, s);
-  else launch_fused_k<TB, G, VEC, GS, false>(a, persistent, grid, s);
+  if (a.sddmm_rows == kSddFused)
+    launch_fused_k<TB, G, VEC, GS, kSddFused>(a, persistent, grid, s);
+  else if (a.sddmm_rows == kSddRows)
+    launch_fused_k<TB, G, VEC, GS, kSddRows>(a, persistent, grid, s);
+  else
+    launch_fused_k<TB, G, VEC, GS, kSddChunks>(a, persistent, grid, s);
   return hipGetLastError() == hipSuccess ? SPMM_OK : SPMM_ERR_HIP;
 }
 
@@ -184,10 +202,14 @@ static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, in
   a.n_bwd = a.bwd.total_wg * a.ctiles;
   const int GS = sddmm_G(p->N, e16);
   build_table(p->fwd->counts, p->fwd->offs, GS, 1, &a.sdd);
+  if (sddmm_rows == kSddFused && a.ctiles != 1)
+    return SPMM_ERR_UNSUPPORTED;  // the fused backward needs one column tile
   a.sddmm_rows = sddmm_rows;
-  a.n_sddmm = sddmm_rows ? a.sdd.total_wg
-                         : static_cast<int>(ceil_div_ll(
-                               p->nnz, static_cast<long long>(kThreads / GS) * kSddmmChunk));
+  a.n_sddmm = sddmm_rows == kSddFused ? 0
+              : sddmm_rows == kSddRows
+                  ? a.sdd.total_wg
+                  : static_cast<int>(ceil_div_ll(
+                        p->nnz, static_cast<long long>(kThreads / GS) * kSddmmChunk));
   a.sddmm_ntile = p->N / (GS * e16);
   a.M = p->M;
   a.N = p->N;
@@ -202,6 +224,7 @@ static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, in
   a.tcol = p->tcol;
   a.val = p->val;
   a.tval = p->tval;
+  a.perm = p->perm;
   a.X = X;
   a.dY = dY;
   a.Y = Y;
@@ -252,13 +275,15 @@ static int tune_plan(spmm_batch_plan_s *p) {
     (void)hipEventElapsedTime(ms, e0, e1);
     return static_cast<int>(SPMM_OK);
   };
-  // SDDMM form: row-scheduled (1) or entry-chunked (0); bitwise equal
-  for (int srows = 1; srows >= 0 && !st; --srows) {
+  // SDDMM form: fused into the backward items (2, one column tile only),
+  // row-scheduled (1) or entry-chunked (0)
+  for (int srows = kSddFused; srows >= kSddChunks && !st; --srows) {
     for (int vec = 1; vec <= 2 && !st; ++vec) {
       if (p->N % (e16 * vec)) continue;
       const int lanes = p->N / (e16 * vec);
       for (int G = 64; G >= 4 && !st; G >>= 1) {
         if (lanes % G || G > lanes || G * 4 < std::min(lanes, 64)) continue;
+        if (srows == kSddFused && G != lanes) continue;
         for (int rs = 1; rs <= 8 && !st; rs <<= 1) {
           for (int pers = 0; pers <= 1 && !st; ++pers) {
             float ms = 0.f;
@@ -306,6 +331,7 @@ int spmm_batch_plan_destroy(spmm_batch_plan_t p) {
   if (p->trowptr) (void)hipFree(p->trowptr);
   if (p->tcol) (void)hipFree(p->tcol);
   if (p->tval) (void)hipFree(p->tval);
+  if (p->perm) (void)hipFree(p->perm);
   if (p->counters) (void)hipFree(p->counters);
   delete p;
   return SPMM_OK;
@@ -347,6 +373,7 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off, con
       hipMalloc(&p->trowptr, sizeof(int) * (static_cast<size_t>(M) + 1)) != hipSuccess ||
       hipMalloc(&p->tcol, sizeof(int) * nn) != hipSuccess ||
       hipMalloc(&p->tval, sizeof(float) * nn) != hipSuccess ||
+      hipMalloc(&p->perm, sizeof(int) * nn) != hipSuccess ||
       hipMalloc(&p->counters, sizeof(int) * 2) != hipSuccess)
     return fail(SPMM_ERR_ALLOC, "device allocation failed");
   if (hipMemsetAsync(p->counters, 0, sizeof(int) * 2, s) != hipSuccess)
@@ -363,8 +390,8 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off, con
     if (hipGetLastError() != hipSuccess) return fail(SPMM_ERR_HIP, "globalise launch");
   }
   if (hipStreamSynchronize(s) != hipSuccess) return fail(SPMM_ERR_HIP, "sync");
-  int st = csr_transpose(h, M, M, nnz, rowptr, p->gcol, val, p->trowptr, p->tcol, p->tval, nullptr,
-                         stream);
+  int st = csr_transpose(h, M, M, nnz, rowptr, p->gcol, val, p->trowptr, p->tcol, p->tval,
+                         p->perm, stream);
   if (st) { spmm_batch_plan_destroy(p); return st; }
   st = spmm_csr_analyze(h, M, nnz, rowptr, &p->fwd, stream);
   if (st) { spmm_batch_plan_destroy(p); return st; }
@@ -391,6 +418,30 @@ int spmm_batch_plan_info(spmm_batch_plan_t p, spmm_tune_t *tune, int *persistent
   if (persistent) *persistent = p->persistent;
   if (sddmm_rows) *sddmm_rows = p->sddmm_rows;
   if (tuned_ms) *tuned_ms = p->tuned_ms;
+  return SPMM_OK;
+}
+
+int spmm_batch_plan_set(spmm_batch_plan_t p, const spmm_tune_t *tune, int persistent,
+                        int sddmm_form) {
+  SPMM_CHECK_ARG(p && tune, "null plan or tune");
+  const int e16 = p->bf16 ? 8 : 4;
+  const int G = tune->lanes_per_row, vec = tune->vec, rs = tune->rows_scale;
+  SPMM_CHECK_ARG(vec == 1 || vec == 2, "vec must be 1 or 2");
+  SPMM_CHECK_ARG(rs == 1 || rs == 2 || rs == 4 || rs == 8, "rows_scale must be 1, 2, 4 or 8");
+  SPMM_CHECK_ARG(p->N % (e16 * vec) == 0, "N is not a whole number of vectors");
+  const int lanes = p->N / (e16 * vec);
+  SPMM_CHECK_ARG((G == 4 || G == 8 || G == 16 || G == 32 || G == 64) && lanes % G == 0,
+                 "lanes_per_row must be 4..64 and divide N / (elements per vector x vec)");
+  SPMM_CHECK_ARG(persistent == 0 || persistent == 1, "persistent must be 0 or 1");
+  SPMM_CHECK_ARG(sddmm_form >= kSddChunks && sddmm_form <= kSddFused, "sddmm_form must be 0, 1 or 2");
+  SPMM_CHECK_ARG(sddmm_form != kSddFused || G == lanes,
+                 "the fused backward needs one column tile (lanes_per_row x vec x 16 B = N)");
+  destroy_graph(p);  // the next graph step captures the new launch
+  p->G = G;
+  p->VEC = vec;
+  p->rows_scale = rs;
+  p->persistent = persistent;
+  p->sddmm_rows = sddmm_form;
   return SPMM_OK;
 }
 

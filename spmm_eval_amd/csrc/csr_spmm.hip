@@ -98,102 +98,6 @@ csr_spmm_perm_kernel(LaunchTable tab, const int *__restrict__ rows,
                            ldc, perm);
 }
 
-// ------------------------------------------------ fused dX + SDDMM
-// One pass over the transposed structure (row c of A^T = column c of A; its
-// entries e' carry A's row i = tcol[e'] and source entry perm[e']) gives
-// both halves of a training step's backward: dX[c] = sum val[perm[e']] dY[i]
-// and dval[perm[e']] = dY[i] . X[c]. The dY row gathered for dX is dotted
-// with X[c], which the subgroup holds in registers for the row, so the
-// separate SDDMM's gather of X rows disappears. Each entry's dot product
-// stays in the subgroup that gathers it (lane partials over its columns in
-// column order, then a reduce-scatter and a butterfly over the G lanes), so
-// no cross-subgroup reduction exists for dval; lane k + u of a chunk stores
-// entry k + u.
-template <typename TB, int G, int EPL>
-__device__ __forceinline__ void accumulate_range_sddmm(
-    int e_begin, int e_end, int first_chunk_off, int chunk_stride, int lane_sub,
-    const int *__restrict__ tcol, const float *__restrict__ val, const int *__restrict__ perm,
-    const TB *__restrict__ dYc, int lddy, const float (&x)[EPL], float (&acc)[EPL],
-    float *__restrict__ dval) {
-  constexpr int U0 = EPL >= 16 ? 4 : 8;
-  constexpr int U = G < U0 ? G : U0;
-  constexpr int V = EPL / Vec16<TB>::kElems;
-  int e = e_begin + first_chunk_off;
-  int cidx = 0, pidx = 0;
-  float v = 0.f;
-  if (e + lane_sub < e_end) {
-    cidx = tcol[e + lane_sub];
-    pidx = perm[e + lane_sub];
-    v = val[pidx];
-  }
-  for (; e < e_end; e += chunk_stride) {
-    const int n = min(G, e_end - e);
-    const int my_n = e + chunk_stride + lane_sub;
-    int cidx_n = 0, pidx_n = 0;
-    float v_n = 0.f;
-    if (my_n < e_end) {
-      cidx_n = tcol[my_n];
-      pidx_n = perm[my_n];
-      v_n = val[pidx_n];
-    }
-    for (int k = 0; k < n; k += U) {
-      int cc[U];
-      float vv[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        cc[u] = __shfl(cidx, k + u, G);
-        vv[u] = __shfl(v, k + u, G);
-      }
-      // all U loads unconditional (slots past n read the padding lanes' row
-      // 0) and waited for once; the arithmetic is predicated
-      typename Vec16<TB>::Raw raw[U][V];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const TB *p = dYc + static_cast<size_t>(cc[u]) * lddy;
-#pragma unroll
-        for (int q = 0; q < V; ++q) raw[u][q] = Vec16<TB>::raw(p + q * G * Vec16<TB>::kElems);
-      }
-      float d[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        d[u] = 0.f;
-        if (k + u < n) {
-          float b[EPL];
-#pragma unroll
-          for (int q = 0; q < V; ++q) Vec16<TB>::widen(raw[u][q], &b[q * Vec16<TB>::kElems]);
-#pragma unroll
-          for (int q = 0; q < EPL; ++q) {
-            acc[q] = fmaf(vv[u], b[q], acc[q]);
-            d[u] = fmaf(b[q], x[q], d[u]);
-          }
-        }
-      }
-      // reduce-scatter over the low log2(U) lane bits (U - 1 shuffles)
-      // leaves entry k + (l mod U)'s partial in lane l; a butterfly over the
-      // other bits completes it, and lane l (which holds that entry's source
-      // index, as l - k = l mod U) stores it
-#pragma unroll
-      for (int m = U / 2; m >= 1; m >>= 1) {
-        const bool hi_half = (lane_sub & m) != 0;
-#pragma unroll
-        for (int i = 0; i < m; ++i) {
-          const float send = hi_half ? d[i] : d[i + m];
-          const float keep = hi_half ? d[i + m] : d[i];
-          d[i] = keep + __shfl_xor(send, m, G);
-        }
-      }
-#pragma unroll
-      for (int m = U; m < G; m <<= 1) d[0] += __shfl_xor(d[0], m, G);
-      if (lane_sub >= k && lane_sub < k + U && lane_sub < n) dval[pidx] = d[0];
-    }
-    cidx = cidx_n;
-    pidx = pidx_n;
-    v = v_n;
-  }
-}
-
-// The row classes of csr_wg (one column tile: the subgroup's G lanes cover
-// all N columns), each row's X[c] loaded beside its span.
 template <typename TB, int G, int VEC>
 __global__ void __launch_bounds__(kThreads)
 csr_t_sddmm_kernel(LaunchTable tab, const int *__restrict__ rows,
@@ -201,79 +105,8 @@ csr_t_sddmm_kernel(LaunchTable tab, const int *__restrict__ rows,
                    const float *__restrict__ val, const int *__restrict__ perm,
                    const TB *__restrict__ dY, int lddy, const TB *__restrict__ X, int ldx,
                    float *__restrict__ dX, int lddx, float *__restrict__ dval) {
-  constexpr int E = Vec16<TB>::kElems;
-  constexpr int EPL = VEC * E;
-  constexpr int S = kWave / G;
-  constexpr int CW = G * EPL;
-  __shared__ float red[kWavesPerWG - 1][CW];
-  const Seg sg = find_seg(tab, blockIdx.x);
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int lane_sub = lane & (G - 1);
-  const int sub = lane / G;
-  const int col0 = lane_sub * E;
-  const TB *dYc = dY + col0;
-  const int local_wg = blockIdx.x - sg.wg_begin;
-  auto load_x = [&](int c, float (&x)[EPL]) {
-    const TB *xr = X + static_cast<size_t>(c) * ldx + col0;
-#pragma unroll
-    for (int q = 0; q < VEC; ++q) Vec16<TB>::load(xr + q * G * E, &x[q * E]);
-  };
-
-  if (sg.kind == kSegA) {
-    const int R = sg.rows_per_sub;
-    const int subs_per_wg = kWavesPerWG * S;
-    const int end = sg.list_off + sg.count;
-    int pos = sg.list_off + local_wg * subs_per_wg * R + wave * S + sub;
-    for (int k = 0; k < R && pos < end; ++k, pos += subs_per_wg) {
-      const int c = rows[pos];
-      const int2 sp = span[pos];
-      float x[EPL], acc[EPL];
-      load_x(c, x);
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-      accumulate_range_sddmm<TB, G, EPL>(sp.x, sp.y, 0, G, lane_sub, tcol, val, perm, dYc, lddy,
-                                         x, acc, dval);
-      store_row<G, E, EPL>(dX + static_cast<size_t>(c) * lddx + col0, acc);
-    }
-  } else if (sg.kind == kSegB) {
-    const int pos = sg.list_off + local_wg * kWavesPerWG + wave;
-    if (pos >= sg.list_off + sg.count) return;  // whole wave
-    const int c = rows[pos];
-    const int2 sp = span[pos];
-    float x[EPL], acc[EPL];
-    load_x(c, x);
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    accumulate_range_sddmm<TB, G, EPL>(sp.x, sp.y, sub * G, S * G, lane_sub, tcol, val, perm,
-                                       dYc, lddy, x, acc, dval);
-    wave_reduce_subgroups<G, EPL>(acc);
-    if (sub == 0) store_row<G, E, EPL>(dX + static_cast<size_t>(c) * lddx + col0, acc);
-  } else {
-    const int c = rows[sg.list_off + local_wg];
-    const int2 sp = span[sg.list_off + local_wg];
-    float x[EPL], acc[EPL];
-    load_x(c, x);
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    const int gsub = wave * S + sub;
-    accumulate_range_sddmm<TB, G, EPL>(sp.x, sp.y, gsub * G, kWavesPerWG * S * G, lane_sub,
-                                       tcol, val, perm, dYc, lddy, x, acc, dval);
-    wave_reduce_subgroups<G, EPL>(acc);
-    if (wave > 0 && sub == 0) {
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) red[wave - 1][lane_sub * EPL + q] = acc[q];
-    }
-    __syncthreads();
-    if (wave == 0 && sub == 0) {
-#pragma unroll
-      for (int w = 0; w < kWavesPerWG - 1; ++w) {
-#pragma unroll
-        for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
-      }
-      store_row<G, E, EPL>(dX + static_cast<size_t>(c) * lddx + col0, acc);
-    }
-  }
+  csr_t_sddmm_wg<TB, G, VEC, false>(tab, blockIdx.x, rows, span, tcol, val, perm, dY, lddy, X,
+                                    ldx, dX, lddx, dval);
 }
 
 // Narrow or unaligned N (N not a multiple of one 16-byte vector, or B / C /

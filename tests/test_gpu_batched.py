@@ -10,6 +10,7 @@ import torch
 import oracle
 from spmm_eval_amd import api, graphs
 from spmm_eval_amd.batched import BatchPlan
+from spmm_eval_amd._lib import SpmmError
 
 pytestmark = pytest.mark.gpu
 
@@ -65,4 +66,51 @@ def test_batched_matches_per_graph_and_oracle(handle, dev, batch, bf16):
     assert np.linalg.norm(dX.cpu().numpy() - ref_dx) / np.linalg.norm(ref_dx) <= 1e-5
     ref_dv = oracle.sddmm_f64(U.rowptr, U.colind, dYf, Xf)
     assert np.linalg.norm(dval.cpu().numpy() - ref_dv) / np.linalg.norm(ref_dv) <= 1e-5
+    plan.close()
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_batched_sddmm_forms(handle, dev, batch, bf16):
+    """Every SDDMM form of the fused launch pinned on one plan: entry chunks
+    (0) and forward rows (1) give the same bits; the backward items forming
+    dval beside dX (2) equal the per-graph fused backward bit for bit and the
+    other forms to 1e-6; each form's graph replay equals its eager step; a
+    form the geometry cannot run is refused."""
+    gs, (U, row_off, nnz_off) = batch
+    N = 64
+    Xd = torch.from_numpy(graphs.normal((U.M, N), 11, 1)).to(dev)
+    dYd = torch.from_numpy(graphs.normal((U.M, N), 11, 2)).to(dev)
+    if bf16:
+        Xd, dYd = api.to_bf16_bits(Xd), api.to_bf16_bits(dYd)
+    plan = BatchPlan(handle, gs, N=N, bf16=bf16, device=dev)
+    lanes = 8 if bf16 else 16  # one column tile at one vector per lane
+    out = {}
+    for form in (0, 1, 2):
+        for pers in (False, True):
+            plan.set_geometry(lanes, 1, 1, pers, form)
+            Y, dX, dval = (t.clone() for t in plan.step(Xd, dYd, use_graph=False))
+            Yg, dXg, dvalg = plan.step(Xd, dYd, use_graph=True)
+            torch.cuda.synchronize()
+            assert torch.equal(Y, Yg) and torch.equal(dX, dXg) and torch.equal(dval, dvalg)
+            out[(form, pers)] = (Y, dX, dval)
+    for key, (Y, dX, dval) in out.items():
+        assert torch.equal(Y, out[(0, False)][0]), key
+    assert torch.equal(out[(1, False)][2], out[(0, False)][2])
+    Y2, dX2, dv2 = out[(2, False)]
+    assert torch.equal(dX2, out[(0, False)][1])  # same transposed rows, same sums
+    dv0 = out[(0, False)][2]
+    assert float((dv2 - dv0).abs().max()) <= 1e-6 * float(dv0.abs().max())
+    for gi in (0, 5, len(gs) - 1):
+        g = gs[gi]
+        r0, r1 = int(row_off[gi]), int(row_off[gi + 1])
+        e0, e1 = int(nnz_off[gi]), int(nnz_off[gi + 1])
+        rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+        tp = api.CsrTPlan(handle, rp, ci, g.K)
+        dXi, dvi = api.spmm_csr_t_sddmm(tp, rp, ci, v, dYd[r0:r1], Xd[r0:r1], tune=(lanes, 1, 1))
+        torch.cuda.synchronize()
+        assert torch.equal(dXi, dX2[r0:r1]) and torch.equal(dvi, dv2[e0:e1]), gi
+    with pytest.raises(SpmmError):  # two column tiles cannot form dval in the backward items
+        plan.set_geometry(lanes // 2, 1, 1, False, 2)
+    with pytest.raises(SpmmError):
+        plan.set_geometry(lanes, 1, 3, False, 0)
     plan.close()
