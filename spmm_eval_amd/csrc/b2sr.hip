@@ -1293,13 +1293,13 @@ __device__ __forceinline__ void wide_brow(int br, int2 sp, int M, const int *__r
   }
 }
 
-// Planned (brows != nullptr): workgroup x takes the plan's block row x.
+// Planned (kPlanFree false): workgroup x takes the plan's block row x.
 // Plan-free: the block rows of at least heavy_tiles tiles, listed by
 // b2sr_class_kernel (state + kStateHead + MB, count state[2]), go first to
 // the E workgroups resident from the start (x, x + E, ...); workgroup E + k
 // then takes block row k unless it is listed. The last workgroup to finish
 // zeroes the list count.
-template <int T, typename TB, int G, bool kBuf>
+template <int T, typename TB, int G, bool kBuf, bool kPlanFree>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span,
                  const int *__restrict__ brp, int E, int heavy_tiles,
@@ -1307,7 +1307,7 @@ b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span,
                  const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
                  int ldb, int b_bytes, float *__restrict__ C, int ldc) {
   const int x = blockIdx.x;
-  if (brows) {
+  if constexpr (!kPlanFree) {  // its own instantiation: no plan-free registers
     wide_brow<T, TB, G, kBuf>(brows[x], span[x], M, bci, words, B, ldb, b_bytes, C, ldc);
     return;
   }
@@ -1675,7 +1675,10 @@ static void launch_wide(const spmm_csr_plan_s *p, const int *brp, int *state, in
   const auto *w = static_cast<const typename Word<T>::type *>(val);
   const int b_bytes = buffer_span(K, ldb, sizeof(TB));
   auto *const kern =
-      b_bytes ? &b2sr_wide_kernel<T, TB, G, true> : &b2sr_wide_kernel<T, TB, G, false>;
+      p ? (b_bytes ? &b2sr_wide_kernel<T, TB, G, true, false>
+                   : &b2sr_wide_kernel<T, TB, G, false, false>)
+        : (b_bytes ? &b2sr_wide_kernel<T, TB, G, true, true>
+                   : &b2sr_wide_kernel<T, TB, G, false, true>);
   int E = 0;
   if (!p) {
     hipLaunchKernelGGL(b2sr_class_kernel, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0,
