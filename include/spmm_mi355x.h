@@ -151,6 +151,20 @@ int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N,
                    const void *bsrVal, const uint16_t *B, int ldb, float *C,
                    int ldc, void *stream);
 
+/* Allocation-free FBF SpMM (FP32 B). No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same operands and
+ * results as spmm_b2sr_f32, without a handle: the block-row lists live in a
+ * static device array of this library (one per GPU, about 2 MiB, held until
+ * the process ends), so the call makes no hipMalloc/hipFree. At most 262144
+ * block rows (ceil(M/T)); larger M returns SPMM_ERR_UNSUPPORTED. The array
+ * is shared by every call on one GPU: calls that overlap on one GPU (other
+ * streams, other threads) race and must be ordered by the caller. */
+int spmm_b2sr_f32_noalloc(int T, int M, int K, int N, const int *bsrRowPtr,
+                          const int *bsrColInd, const void *bsrVal,
+                          const float *B, int ldb, float *C, int ldc,
+                          void *stream);
+
 /* Planned FBF SpMM. spmm_b2sr_analyze bins the block rows by tile count and
  * keeps each block row's tile span beside it (SYNCHRONISES once); the
  * planned call runs in a single launch (no list state, asynchronous) and

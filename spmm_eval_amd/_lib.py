@@ -55,6 +55,7 @@ _SIGS = {
     "csr2b2sr": [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "spmm_b2sr_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_b2sr_bf16": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
+    "spmm_b2sr_f32_noalloc": [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_b2sr_analyze": [_vp, _i, _i, _vp, C.POINTER(_vp), _vp],
     "spmm_b2sr_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_b2sr_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],

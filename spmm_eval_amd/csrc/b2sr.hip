@@ -26,6 +26,7 @@
 
 #include <atomic>
 #include <climits>
+#include <mutex>
 #include <type_traits>
 
 namespace spmm {
@@ -1566,14 +1567,15 @@ static void dispatch_wide(const Geo &g, const spmm_csr_plan_s *p, const int *brp
 
 template <typename TB>
 static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp, const int *bci,
-                   const void *val, const TB *B, int ldb, float *C, int ldc, void *stream) {
+                   const void *val, const TB *B, int ldb, float *C, int ldc, void *stream,
+                   int *ext_state = nullptr) {
   SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
   if (M == 0 || N == 0) return SPMM_OK;
   SPMM_CHECK_ARG(brp && C, "null bsrRowPtr or C");
   SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldb >= N), "ldb/ldc smaller than N");
   SPMM_CHECK_ARG(K == 0 || B, "null B");
-  SPMM_CHECK_ARG(h != nullptr, "null handle");
+  SPMM_CHECK_ARG(h != nullptr || ext_state != nullptr, "null handle");
   hipStream_t s = as_stream(stream);
   Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
   // the planned form's kernels when bsrVal has their alignment (T bytes for
@@ -1583,8 +1585,12 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
   int *state = nullptr;
   int long_grid = 0, cus = 0;
   if (g.G) {
-    const int st = b2sr_state(h, (M + T - 1) / T, &state);
-    if (st) return st;
+    if (ext_state) {
+      state = ext_state;
+    } else {
+      const int st = b2sr_state(h, (M + T - 1) / T, &state);
+      if (st) return st;
+    }
     int dev = 0;
     SPMM_HIP(hipGetDevice(&dev));
     SPMM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1784,6 +1790,38 @@ static void dispatch_sign(int M, int N, const int *brp, const int *bci, const vo
 
 using namespace spmm;
 
+// Allocation-free form of the plan-free FBF SpMM: the block-row lists live
+// in `scratch`, one copy per GPU, held until the process ends. In host code
+// `scratch` names a host-side stand-in, so the device address of each GPU's
+// copy is looked up once per device (hipGetSymbolAddress, with that device
+// current) and only that address is handed to kernels.
+constexpr int kNoAllocMaxBlockRows = 1 << 18;  // 2 MiB of lists + head
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) int scratch[kStateHead + 2 * kNoAllocMaxBlockRows];
+
+static int scratch_ptr(int **out) {
+  static std::mutex mu;
+  static int *ptrs[kMaxDevices] = {};
+  int dev = 0;
+  SPMM_HIP(hipGetDevice(&dev));
+  SPMM_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr)
+      SPMM_FAIL(SPMM_ERR_HIP, "hipGetSymbolAddress of the B2SR scratch failed");
+    ptrs[dev] = static_cast<int *>(p);
+  }
+  *out = ptrs[dev];
+  return SPMM_OK;
+}
+
+// Counters the list kernels expect at zero (a kernel, not a memset node, so
+// the call stays correct under graph capture).
+__global__ void b2sr_zero_head_kernel(int *state) {
+  if (threadIdx.x < kStateHead) state[threadIdx.x] = 0;
+}
+
 extern "C" {
 
 int csr2b2sr_nnzb(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const int *colind,
@@ -1851,6 +1889,25 @@ int spmm_b2sr_bf16(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRo
                    const int *bsrColInd, const void *bsrVal, const uint16_t *B, int ldb,
                    float *C, int ldc, void *stream) {
   return run_fbf<uint16_t>(h, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc, stream);
+}
+
+int spmm_b2sr_f32_noalloc(int T, int M, int K, int N, const int *bsrRowPtr,
+                          const int *bsrColInd, const void *bsrVal, const float *B, int ldb,
+                          float *C, int ldc, void *stream) {
+  SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
+  SPMM_CHECK_ARG(M >= 0, "negative size");
+  if ((M + T - 1) / T > kNoAllocMaxBlockRows)
+    SPMM_FAIL(SPMM_ERR_UNSUPPORTED,
+              "spmm_b2sr_f32_noalloc: more than 262144 block rows (the static scratch limit)");
+  int *state = nullptr;
+  const int st = scratch_ptr(&state);
+  if (st) return st;
+  if (M > 0 && N > 0) {
+    hipLaunchKernelGGL(b2sr_zero_head_kernel, dim3(1), dim3(kWave), 0, as_stream(stream), state);
+    SPMM_LAUNCHED();
+  }
+  return run_fbf<float>(nullptr, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc,
+                        stream, state);
 }
 
 int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr, spmm_csr_plan_t *plan,
