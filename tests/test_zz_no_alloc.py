@@ -45,3 +45,41 @@ def test_noalloc_rejects_too_many_block_rows(dev):
     with pytest.raises(SpmmError) as e:
         call("spmm_b2sr_f32_noalloc", 4, M, 4, 4, None, None, None, None, 4, None, 4, None)
     assert "262144" in str(e.value)
+
+
+@pytest.mark.parametrize("T", [4, 16])
+def test_noalloc_graph_replay(handle, dev, T):
+    """Captured in a hipGraph (the caller this form is for) and replayed
+    three times, with an eager allocation-free call on another graph between
+    replays sharing the same scratch: every replay equals the handle form."""
+    g = graphs.chung_lu(12000, 120000, seed=21)
+    other = graphs.random_csr(300, 8192, 0.01, seed=7, long_rows=2, long_len=4000)
+    ops = []
+    for h in (g, other):
+        rp = torch.from_numpy(h.rowptr).to(dev)
+        ci = torch.from_numpy(h.colind).to(dev)
+        brp, bci, words = api.csr2b2sr(handle, rp, ci, h.M, h.K, T)
+        B = torch.from_numpy(graphs.uniform_pm1((h.K, 64), seed=T)).to(dev)
+        ops.append((h.M, brp, bci, words, B, api.spmm_b2sr(handle, T, h.M, brp, bci, words, B)))
+    (M, brp, bci, words, B, ref), (Mo, obrp, obci, owords, Bo, o_ref) = ops
+    C = torch.empty(M, 64, device=dev)
+    Co = torch.empty(Mo, 64, device=dev)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        _noalloc(T, M, brp, bci, words, B, C)  # first call on the device outside capture
+        s.synchronize()
+        with torch.cuda.graph(graph, stream=s):
+            _noalloc(T, M, brp, bci, words, B, C)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        C.fill_(0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(C, ref), rep
+        _noalloc(T, Mo, obrp, obci, owords, Bo, Co)
+        torch.cuda.synchronize()
+        assert torch.equal(Co, o_ref), rep
