@@ -20,7 +20,7 @@ def _noalloc(T, M, brp, bci, words, B, C):
          api._p(words), api._p(B), B.stride(0), api._p(C), C.stride(0), api._stream())
 
 
-@pytest.mark.parametrize("T,N", [(4, 64), (8, 32), (16, 64), (32, 32), (4, 3)])
+@pytest.mark.parametrize("T,N", [(2, 64), (4, 64), (8, 32), (16, 64), (32, 32), (64, 64), (4, 3)])
 def test_noalloc_matches_handle_form(handle, dev, T, N):
     for n, nnz, seed in ((20000, 200000, 11), (3000, 24000, 12)):
         print(f"noalloc T={T} N={N} n={n} seed={seed}", flush=True)
