@@ -899,7 +899,7 @@ b2sr_class_kernel(int MB, const int *__restrict__ brp, int a_tiles, int w_tiles,
 // One launch over the classes b2sr_class_kernel listed, with the planned
 // form's per-class code and static shares (one counter shared by every
 // workgroup would serialise their atomics across the XCDs). The first E
-// workgroups (half as many as the chip holds at once) take the listed work:
+// workgroups (as many as the chip holds at once, or half) take the listed work:
 // workgroup x the workgroup-class rows x, x + E, ... (row i = listed block
 // row i / T, row i % T), and the wave-class block rows are dealt to their
 // waves from the last of them backwards, so they land on workgroups with
@@ -1532,12 +1532,16 @@ static void launch_dropin(int M, int K, const int *brp, const int *bci, const vo
   auto *const kern = b_bytes ? &b2sr_dropin_kernel<T, TB, G, true>
                              : &b2sr_dropin_kernel<T, TB, G, false>;
   static std::atomic<int> per_cu[2];
-  // listed work on half as many workgroups as the chip holds at once (no
-  // more than there are rows to list), then one workgroup per SW block rows;
-  // on the 89k-row power-law graph (T = 4, N = 64) half measured 47.1 us,
-  // the full resident count 49.8, a quarter 63.5 and twice 65.1
+  // listed work on as many workgroups as the chip holds at once, or half
+  // as many for FP32 B at T <= 4 and BF16 B at T = 2 (no more than there
+  // are rows to list), then one workgroup per SW block rows. On the 89k-row
+  // power-law graph at N = 64 (us per call, full / half): FP32 T = 2 63.6 /
+  // 57.0, T = 4 50.0 / 47.0, T = 8 47.3 / 61.9; BF16 T = 2 45.4 / 44.2,
+  // T = 4 42.6 / 47.8, T = 8 58.2 / 79.9 (a quarter or twice the count at
+  // FP32 T = 4: 63.5 and 65.1)
+  constexpr int kDiv = (T == 2 || (T == 4 && sizeof(TB) == 4)) ? 2 : 1;
   const int E = std::min(std::max(1, resident_wgs(reinterpret_cast<const void *>(kern), cus,
-                                                  &per_cu[b_bytes ? 1 : 0]) / 2),
+                                                  &per_cu[b_bytes ? 1 : 0]) / kDiv),
                          M);
   const dim3 grid(E + ceil_div(MB, SW), tiles);
   hipLaunchKernelGGL(kern, grid, dim3(kB2Threads), 0, s, M, E, brp, bci, tw, B, ldb, b_bytes, C,
