@@ -1,23 +1,40 @@
 """SpMM benchmark on one MI355X.
 
-Headline step (`value`): the FBF SpMM the reference harness launches --
-B2SR-4 bit tiles x FP32 B -> FP32 C (spmm4_full_full_new2,
-backend/spmm.cu:877-956, test_spmm_full.cu:344-369) -- on the seeded
-89,250-row power-law 0/1 graph (~900k nnz) with hidden N = 64, inputs
-resident in HBM. Metric: algorithmic GB/s (gather model: every stored bit
-reads one full B row), with GFLOP/s = 2 nnz N / t.
+Headline step (`value`): CSR FP32 SpMM C = A B -- the cuSPARSE call of
+evalCSRSpmmFloatCuSPARSE (test_spmm_full.cu:212-342) that the library
+replaces -- on the largest workload that fits one GPU: the seeded
+232,965-row power-law graph (~114.5M nnz, heaviest row ~17k entries,
+values U(0,1] row-normalised), B 232,965 x 128 FP32 U[-1,1], inputs
+resident in HBM, row schedule built and autotuned once outside the timed
+region (as the reference uploads its operands before timing).
 
-`extra` holds the other workloads (skipped with --quick): CSR FP32 on the
-16,384-row graph (N=32) and on the 232,965-row ~114.6M-nnz graph (N=64 and
-128, FP32 and BF16 B); GCN-style fwd+bwd (Y = A X, dX = A^T dY,
-dval = SDDMM) in FP32 and BF16 features; 512 uneven graphs batched in one
-captured hipGraph vs 512 separate FP32 launches; aligned B2SR-8/16 binary
-SpMV (binary and FP32 x) on the headline graph beside CSR at N = 1; the B2SR
-grid (tiles 2..64 x N = 1/32/64/128) beside CSR. Errors there are measured
-against an FP64 numpy recomputation on sampled rows.
+Byte counts. `value` and `roofline` use the COMPULSORY bytes of a launch:
+every array read or written once (rowptr, colind, val, B, C). That is a
+lower bound on HBM traffic, so `roofline.frac` <= 1 holds by construction.
+The gather-model count (every stored entry reads one full B row; the
+per-entry figure of the measurement plan) is reported as
+`effective_GBps_gather_model`: B rows served by L2 or the MALL count there
+each time, so it exceeds the HBM peak and is not an HBM figure.
 
-`cpu_baseline`: the OpenMP FP32 CSR SpMM of the oracle library on the same
-graph and B (3 warm-ups, median of 15), same byte count.
+Timing. The timed region is `--steps` launches after `--warmup` between two
+events on the launch stream (ms_per_step). Beside it every GPU figure is also
+the median of 5 batches of >= 100 back-to-back launches after 10 warm-ups
+(`median_of_5`), and kernels too short for host launch pacing are timed as
+captured hipGraph replays (median of 5 replays of 100-200 launches).
+
+`cpu_baseline`: the oracle library's OpenMP FP32 CSR SpMM on the same graph
+and B on all host threads (3 warm-ups, median of 10). `extra.cpu` holds the
+same kind of baseline for every other workload (forward, CPU dX and CPU
+SDDMM for the fwd+bwd comparisons).
+
+`extra` (skipped with --quick): the same graph at N = 64 and in BF16; the
+16,384-row N=32 workload; B2SR-4 FBF SpMM on the 89,250-row 0/1 graph
+(planned and plan-free) with the tile x N grid, the sign-packed popcount
+SpMM and the dense packers; BMM32 (XNOR/popcount matmul); GCN-style
+fwd+bwd (Y = A X, dX = A^T dY, dval = SDDMM) in FP32 and BF16; 512 uneven
+graphs batched in one captured launch vs 512 separate FP32 launches; aligned
+B2SR-8/16 binary SpMV. GPU errors there are against an FP64 numpy
+recomputation on sampled rows (the heaviest rows always included).
 """
 from __future__ import annotations
 
@@ -40,90 +57,73 @@ def log(msg):
 
 
 # ------------------------------------------------------------------ bytes
-def csr_bytes(M, nnz, N, s_val=4, s_b=4):
+def csr_gather_bytes(M, nnz, N, s_val=4, s_b=4):
+    """Gather model: every stored entry reads one full B row."""
     return 4 * (M + 1) + nnz * (4 + s_val) + nnz * N * s_b + M * N * 4
 
 
-def b2sr_bytes(M, T, nb, nnz, N, s_b=4):
+def csr_compulsory_bytes(M, K, nnz, N, s_b=4):
+    """Each array once: rowptr, colind, val, B, C."""
+    return 4 * (M + 1) + 8 * nnz + K * N * s_b + M * N * 4
+
+
+TILE_BYTES = {2: 2, 4: 4, 8: 8, 16: 32, 32: 128, 64: 512}
+
+
+def b2sr_gather_bytes(M, T, nb, nnz, N, s_b=4):
     MB = (M + T - 1) // T
-    tile = {2: 2, 4: 4, 8: 8, 16: 32, 32: 128, 64: 512}[T]
-    return 4 * (MB + 1) + nb * (4 + tile) + nnz * N * s_b + M * N * 4
+    return 4 * (MB + 1) + nb * (4 + TILE_BYTES[T]) + nnz * N * s_b + M * N * 4
 
 
-def sddmm_bytes(M, nnz, N, s=4):
+def b2sr_compulsory_bytes(M, K, T, nb, N, s_b=4):
+    MB = (M + T - 1) // T
+    return 4 * (MB + 1) + nb * (4 + TILE_BYTES[T]) + K * N * s_b + M * N * 4
+
+
+def sddmm_gather_bytes(M, nnz, N, s=4):
     return 4 * (M + 1) + 8 * nnz + nnz * N * s + M * N * s
 
 
-def b2sr_lanes(N, elem_bytes=4):
-    """Lanes per row the FBF kernels use for an aligned N (widest power of
-    two dividing N / (16 / elem_bytes), at most 64)."""
-    lanes = N // (16 // elem_bytes)
-    t = 64
-    while t > 1 and (t > lanes or lanes % t):
-        t //= 2
-    return t
+def rate(nbytes, ms):
+    return round(nbytes / (ms * 1e-3) / 1e9, 1)
 
 
 def profiled_traffic(profile: str, kernel: str):
     """HBM bytes per dispatch of `kernel` from a committed counter summary
-    (FETCH_SIZE and WRITE_SIZE passes, each scaled by what it reads for a
-    device copy of known size). None when the summary has no row for it."""
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by its calibration factor);
+    None when the summary has no row for that kernel."""
     path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
-    key = f"`void spmm::{kernel}`"
+    key = f"`{kernel}`"
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == key:
+        if len(cells) >= 5 and cells[0] == key:
             try:
-                return int(cells[4])
+                return int(cells[-1])
             except ValueError:
                 return None
     return None
 
 
 # ----------------------------------------------------------------- timing
-def time_launches(fn, steps, warmup):
-    """Runs fn() warmup times, then `steps` times back to back between two
-    events on the current stream (the timed region: ms per step); then a
-    second pass of `steps` launches, each between its own event pair on the
-    same stream (mean kernel duration, kept out of the timed region because
-    the extra event records between launches cost a few microseconds each).
-    Returns (ms per step, mean ms per launch)."""
+def _events():
+    import torch
+
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def time_region(fn, steps, warmup):
+    """The contract's timed region: `warmup` untimed launches, a sync, then
+    exactly `steps` launches between two events on the launch stream (the
+    library launches on torch's current stream) and a sync. ms per step."""
     import torch
 
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record(s)
-    for _ in range(steps):
-        fn()
-    t1.record(s)
-    torch.cuda.synchronize()
-    total = t0.elapsed_time(t1) / steps
-    pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-    for a, b in pairs:
-        a.record(s)
-        fn()
-        b.record(s)
-    torch.cuda.synchronize()
-    per = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
-    return total, per
-
-
-def time_loop(fn, steps, warmup):
-    """Back-to-back launches between two events (no per-launch events)."""
-    import torch
-
-    for _ in range(warmup):
-        fn()
-    torch.cuda.synchronize()
-    s = torch.cuda.current_stream()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
+    t0, t1 = _events()
     t0.record(s)
     for _ in range(steps):
         fn()
@@ -132,10 +132,33 @@ def time_loop(fn, steps, warmup):
     return t0.elapsed_time(t1) / steps
 
 
+def median_batches(fn, launches=100, batches=5, warmup=10):
+    """Median over `batches` batches of `launches` back-to-back launches
+    (events on the launch stream around each batch), after `warmup`
+    launches. Returns (median ms per launch, [ms per launch of each batch])."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    per = []
+    for _ in range(batches):
+        t0, t1 = _events()
+        t0.record(s)
+        for _ in range(launches):
+            fn()
+        t1.record(s)
+        torch.cuda.synchronize()
+        per.append(t0.elapsed_time(t1) / launches)
+    return float(np.median(per)), [round(x, 5) for x in per]
+
+
 def time_graph(fn, out, steps=100, replays=5):
     """Kernels too short for host launch pacing: `steps` back-to-back calls
-    captured in one hipGraph, replayed `replays` times between two events
-    (ms per call). Every replay's `out` must equal the eager result."""
+    captured in one hipGraph, replayed `replays` times, each replay between
+    two events; median ms per call. Every replay's `out` must equal the
+    eager result."""
     import torch
 
     fn()
@@ -151,20 +174,39 @@ def time_graph(fn, out, steps=100, replays=5):
             for _ in range(steps):
                 fn()
     torch.cuda.synchronize()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    total = 0.0
+    per = []
     for _ in range(replays):
         out.fill_(0)
         torch.cuda.synchronize()
+        t0, t1 = _events()
         t0.record()
         g.replay()
         t1.record()
         torch.cuda.synchronize()
-        total += t0.elapsed_time(t1)
+        per.append(t0.elapsed_time(t1) / steps)
         if not torch.equal(out, ref):
             raise RuntimeError("graph replay differs from the eager result")
-    return total / (replays * steps)
+    return float(np.median(per))
+
+
+def cpu_time(fn, warmup=3, reps=10):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)) * 1e3
+
+
+def check_rows(g, rng_seed, n=1500, heavy=16):
+    """Rows for the sampled FP64 check: the `heavy` heaviest rows plus `n`
+    random ones."""
+    deg = g.degrees()
+    top = np.argsort(deg)[-heavy:]
+    rnd = np.random.default_rng(rng_seed).choice(g.M, n, replace=False)
+    return np.unique(np.concatenate([top, rnd]))
 
 
 def sampled_rel_err(g, B64, C, rows):
@@ -178,58 +220,50 @@ def sampled_rel_err(g, B64, C, rows):
     return (num / den) ** 0.5 if den > 0 else 0.0
 
 
+def bf16_round(x):
+    """FP32 -> BF16 (round to nearest even) -> FP32, for the error reference
+    of BF16 runs."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
 # ------------------------------------------------------------------ parts
 def headline(args, h, dev):
     import torch
 
     from spmm_eval_amd import api, graphs
 
-    T, N = args.tile, args.hidden
-    g = graphs.workload_b2sr_graph()
-    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    N = args.hidden
     t = time.time()
-    brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
-    torch.cuda.synchronize()
-    conv_ms = (time.time() - t) * 1e3
-    nb = bci.numel()
-    B = graphs.uniform_pm1((g.K, N), seed=graphs.SEED_BASE + 2)
+    g = graphs.workload_large_graph()
+    log(f"large graph: {g.M} rows, {g.nnz} nnz, max degree {int(g.degrees().max())} "
+        f"({time.time() - t:.1f}s)")
+    B = graphs.uniform_pm1((g.K, N), seed=graphs.SEED_BASE + 3 + N)
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
     Bd = torch.from_numpy(B).to(dev)
     C = torch.empty((g.M, N), device=dev)
-
-    # row schedule built once per structure, outside the timed region (the
-    # reference converts and uploads its B2SR arrays before timing too)
-    bplan = api.B2srPlan(h, T, g.M, brp)
+    t = time.time()
+    plan = api.CsrPlan(h, rp, g.nnz)
     torch.cuda.synchronize()
+    analyze_ms = (time.time() - t) * 1e3
+    geo = plan.autotune(rp, ci, v, Bd, C)
 
     def step():
-        api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=C, plan=bplan)
+        api.spmm_csr(h, rp, ci, v, Bd, C_out=C, plan=plan)
 
-    ms_step, ms_launch = time_launches(step, args.steps, args.warmup)
-    # the plan-free drop-in form (bulk kernel + hub-row list kernel)
-    Cd = torch.empty((g.M, N), device=dev)
-    dropin_ms = time_loop(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=Cd),
-                          args.steps, args.warmup)
-    dropin_diff = float((C - Cd).abs().max())
-    nbytes = b2sr_bytes(g.M, T, nb, g.nnz, N)
-    # the same product through the CSR kernel (the reference prints its
-    # cuSPARSE CSR time beside the B2SR one, test_spmm_full.cu:505)
-    v = torch.ones(g.nnz, device=dev)
-    plan = api.CsrPlan(h, rp, g.nnz)
-    Ccsr = torch.empty((g.M, N), device=dev)
-    csr_ms = time_loop(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=Ccsr, plan=plan), args.steps, args.warmup)
-    del plan
-    flops = 2.0 * g.nnz * N
-    # exactness spot check: +-1 B against an integer recompute of 2000 rows
-    S = np.random.default_rng(3).choice([-1, 1], size=(g.K, N)).astype(np.float32)
-    Cs = api.spmm_b2sr(h, T, g.M, brp, bci, words, torch.from_numpy(S).to(dev), plan=bplan).cpu().numpy()
-    rows = np.random.default_rng(4).choice(g.M, 2000, replace=False)
-    exact = all(np.array_equal(Cs[i], S[g.colind[g.rowptr[i]:g.rowptr[i + 1]]].sum(axis=0)) for i in rows)
+    ms_step = time_region(step, args.steps, args.warmup)
+    ms_med, batches = median_batches(step, launches=args.batch_launches)
+    rows = check_rows(g, 6)
     err = sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows)
-    same = float((C - Ccsr).abs().max())
-    return dict(g=g, B=B, T=T, N=N, nb=nb, nbytes=nbytes, flops=flops, ms_step=ms_step,
-                ms_launch=ms_launch, conv_ms=conv_ms, exact=exact, err=err, csr_ms=csr_ms,
-                csr_bytes=csr_bytes(g.M, g.nnz, N), csr_maxdiff=same, dropin_ms=dropin_ms,
-                dropin_diff=dropin_diff)
+    # the plan-free drop-in (spmm_csr_f32: schedule built inside the call)
+    Cd = torch.empty((g.M, N), device=dev)
+    dropin_ms, _ = median_batches(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=Cd),
+                                  launches=20, warmup=3)
+    dropin_err = sampled_rel_err(g, B.astype(np.float64), Cd.cpu().numpy(), rows)
+    return dict(g=g, B=B, N=N, geo=geo, plan=plan, dev_arrays=(rp, ci, v), ms_step=ms_step,
+                ms_med=ms_med, batches=batches, err=err, rows=rows, analyze_ms=analyze_ms,
+                dropin_ms=dropin_ms, dropin_err=dropin_err)
 
 
 def cpu_baseline(hd):
@@ -237,129 +271,187 @@ def cpu_baseline(hd):
 
     g, B = hd["g"], hd["B"]
     C = np.empty((g.M, B.shape[1]), np.float32)
-    for _ in range(3):
-        oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B, out=C)
-    ts = []
-    for _ in range(15):
-        t = time.perf_counter()
-        oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B, out=C)
-        ts.append(time.perf_counter() - t)
-    ms = float(np.median(ts)) * 1e3
+    ms = cpu_time(lambda: oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B, out=C))
+    nbytes = csr_compulsory_bytes(g.M, g.K, g.nnz, B.shape[1])
     return {
-        "value": round(hd["nbytes"] / (ms * 1e-3) / 1e9, 2),
-        "unit": "GB/s (algorithmic, same byte count)",
+        "value": rate(nbytes, ms),
+        "unit": "GB/s (compulsory bytes, same count as value)",
         "ms_per_step": round(ms, 3),
         "cores": oracle.omp_threads(),
         "kind": "port",
-        "sample": f"OpenMP FP32 CSR SpMM (oracle/spmm_oracle.c orc_csr_spmm_f32_omp) on the full "
-                  f"89,250-row graph, N={B.shape[1]}, values 1; 3 warm-ups, median of 15 runs",
+        "sample": f"the whole workload: OpenMP FP32 CSR SpMM (oracle/spmm_oracle.c "
+                  f"orc_csr_spmm_f32_omp, dynamic chunks of 64 rows) on the {g.M}-row "
+                  f"{g.nnz}-nnz graph, N={B.shape[1]}, same B; 3 warm-ups, median of 10",
+        "gpu_speedup": round(ms / hd["ms_step"], 1),
     }
 
 
-def extra_csr(args, h, dev):
+def extra_large_variants(args, h, dev, hd):
+    """The headline graph at the other widths and precision: FP32 N=64, BF16
+    N=64 and N=128 (FP32 accumulate), each autotuned; with CPU baselines."""
     import torch
 
+    import oracle
     from spmm_eval_amd import api, graphs
 
-    out = {}
-    # workload 1: 16,384 rows, N=32
+    g = hd["g"]
+    rp, ci, v = hd["dev_arrays"]
+    plan = hd["plan"]
+    rows = hd["rows"]
+    out, cpu = {}, {}
+    for N in (64, 128):
+        B = hd["B"] if N == hd["N"] else graphs.uniform_pm1((g.K, N), seed=graphs.SEED_BASE + 3 + N)
+        Bd = torch.from_numpy(B).to(dev)
+        Bh = api.to_bf16_bits(Bd)
+        C = torch.empty((g.M, N), device=dev)
+        for name, BB, sb in (("f32", Bd, 4), ("bf16", Bh, 2)):
+            if name == "f32" and N == hd["N"]:
+                continue  # the headline itself
+            geo = plan.autotune(rp, ci, v, BB, C)
+            ms, _ = median_batches(lambda: api.spmm_csr(h, rp, ci, v, BB, C_out=C, plan=plan),
+                                   launches=100)
+            Bref = (bf16_round(B) if name == "bf16" else B).astype(np.float64)
+            out[f"csr_{name}_233k_n{N}"] = {
+                "geometry": geo, "ms_median_of_5": round(ms, 4),
+                "compulsory_GBps": rate(csr_compulsory_bytes(g.M, g.K, g.nnz, N, sb), ms),
+                "effective_GBps_gather_model": rate(csr_gather_bytes(g.M, g.nnz, N, s_b=sb), ms),
+                "GFLOPs": round(2 * g.nnz * N / ms / 1e6, 1),
+                "rel_err_fp64_sampled": sampled_rel_err(g, Bref, C.cpu().numpy(), rows),
+                "rel_err_vs_fp32_source_sampled": sampled_rel_err(g, B.astype(np.float64),
+                                                                  C.cpu().numpy(), rows)}
+            if name == "bf16":
+                Bhn = Bh.cpu().numpy().view(np.uint16)
+                Cc = np.empty((g.M, N), np.float32)
+                cms = cpu_time(lambda: oracle.csr_spmm_bf16_omp(g.rowptr, g.colind, g.val, Bhn, out=Cc),
+                               reps=5)
+            else:
+                Cc = np.empty((g.M, N), np.float32)
+                cms = cpu_time(lambda: oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B, out=Cc),
+                               reps=5)
+            cpu[f"csr_{name}_233k_n{N}"] = {"ms": round(cms, 2), "gpu_speedup": round(cms / ms, 1)}
+        del Bd, Bh, C
+    torch.cuda.empty_cache()
+    out["cpu"] = cpu
+    return out
+
+
+def extra_csr_small(args, h, dev):
+    """16,384 rows, ~260k nnz, values 1, B 16,384 x 32 (config 1)."""
+    import torch
+
+    import oracle
+    from spmm_eval_amd import api, graphs
+
     g, B = graphs.workload_csr_small()
     rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
     plan = api.CsrPlan(h, rp, g.nnz)
     Bd = torch.from_numpy(B).to(dev)
     C = torch.empty((g.M, 32), device=dev)
     geo = plan.autotune(rp, ci, v, Bd, C)
-    # ~8 us a call: timed as back-to-back launches in a captured graph
+    # ~7 us a call: back-to-back launches in a captured graph
     ms = time_graph(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=C, plan=plan), C, steps=200)
-    nbytes = csr_bytes(g.M, g.nnz, 32)
-    rows = np.random.default_rng(5).choice(g.M, 2000, replace=False)
-    out["csr_f32_16k_n32"] = {"geometry": geo,
-        "nnz": g.nnz, "ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1),
-        "pct_hbm_peak": round(100 * nbytes / ms / 1e6 / HBM_PEAK_GBS, 1),
+    Cd = torch.empty((g.M, 32), device=dev)
+    ms_d, _ = median_batches(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=Cd), launches=100)
+    dropin = {"ms_eager_median_of_5": round(ms_d, 5)}
+    rows = check_rows(g, 5, n=2000)
+    nb = csr_gather_bytes(g.M, g.nnz, 32)
+    Cc = np.empty((g.M, 32), np.float32)
+    cms = cpu_time(lambda: oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B, out=Cc))
+    return {"csr_f32_16k_n32": {
+        "geometry": geo, "nnz": g.nnz, "ms_graph_replay_median_of_5": round(ms, 5),
+        "effective_GBps_gather_model": rate(nb, ms),
+        "pct_of_8TBps_gather_model": round(100 * nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
+        "compulsory_GBps": rate(csr_compulsory_bytes(g.M, g.K, g.nnz, 32), ms),
         "GFLOPs": round(2 * g.nnz * 32 / ms / 1e6, 1),
-        "rel_err_fp64_sampled": sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows)}
-    del plan
-    # workload 3: 232,965 rows, ~114.6M nnz
-    t = time.time()
-    g = graphs.workload_large_graph()
-    log(f"large graph: {g.M} rows, {g.nnz} nnz, max degree {g.degrees().max()} ({time.time() - t:.1f}s)")
-    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
-    t = time.time()
-    plan = api.CsrPlan(h, rp, g.nnz)
-    analyze_ms = (time.time() - t) * 1e3
-    rows = np.random.default_rng(6).choice(g.M, 1500, replace=False)
-    for N in (64, 128):
-        B = graphs.uniform_pm1((g.K, N), seed=graphs.SEED_BASE + 3 + N)
-        B64 = B.astype(np.float64)
-        Bd = torch.from_numpy(B).to(dev)
-        Bh = api.to_bf16_bits(Bd)
-        C = torch.empty((g.M, N), device=dev)
-        for name, BB, sb in (("f32", Bd, 4), ("bf16", Bh, 2)):
-            geo = plan.autotune(rp, ci, v, BB, C)
-            ms = time_loop(lambda: api.spmm_csr(h, rp, ci, v, BB, C_out=C, plan=plan), args.big_steps, 3)
-            nbytes = csr_bytes(g.M, g.nnz, N, s_b=sb)
-            out[f"csr_{name}_233k_n{N}"] = {"geometry": geo,
-                "nnz": g.nnz, "ms": round(ms, 3), "GBps": round(nbytes / ms / 1e6, 1),
-                "pct_hbm_peak": round(100 * nbytes / ms / 1e6 / HBM_PEAK_GBS, 1),
-                "GFLOPs": round(2 * g.nnz * N / ms / 1e6, 1),
-                "rel_err_fp64_sampled": sampled_rel_err(g, B64, C.cpu().numpy(), rows)}
-        del Bd, Bh, C
-    out["csr_233k_analyze_ms"] = round(analyze_ms, 2)
-    del plan, rp, ci, v
-    torch.cuda.empty_cache()
-    return out
+        "rel_err_fp64_sampled": sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows),
+        "dropin": dropin,
+        "cpu": {"ms": round(cms, 4), "gpu_speedup": round(cms / ms, 1)}}}
 
 
-def extra_spmv_aligned(args, h, dev):
-    """Aligned B2SR-8/16 repack and the binary SpMV over it on the headline
-    graph, beside CSR SpMM at N = 1 (the csrmv the reference checks against)."""
+def extra_b2sr(args, h, dev):
+    """The reference harness's own step -- B2SR-4 FBF SpMM
+    (spmm4_full_full_new2, test_spmm_full.cu:344-369) -- on the 89,250-row
+    0/1 graph, planned and plan-free, beside CSR on the same graph; the tile x
+    N grid; the sign-packed popcount SpMM (int32 counts); the packers."""
     import torch
 
+    import oracle
     from spmm_eval_amd import api, graphs
 
-    out = {}
     g = graphs.workload_b2sr_graph()
-    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
-    rng = np.random.default_rng(11)
-    xf = torch.from_numpy(rng.uniform(-1, 1, g.K).astype(np.float32)).to(dev)
-    plan = api.CsrPlan(h, rp, g.nnz)
-    y = torch.empty((g.M, 1), device=dev)
-    ms_csr = time_graph(lambda: api.spmm_csr(h, rp, ci, v, xf[:, None], C_out=y, plan=plan), y)
-    out["spmv_csr_f32_89k"] = {"ms": round(ms_csr, 5),
-                               "GBps": round(csr_bytes(g.M, g.nnz, 1) / ms_csr / 1e6, 1)}
-    del plan
-    for T in (8, 16):
-        t0 = time.time()
-        brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
-        nrp, nci, nval = api.b2sr_aligned(h, T, brp, bci, words)
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    v = torch.ones(g.nnz, device=dev)
+    out = {}
+    # csr2b2sr conversion (both passes + the count read-back), after a first
+    # call that sizes the handle's workspace; median of 5 calls
+    brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, 4)
+    conv = []
+    for _ in range(5):
         torch.cuda.synchronize()
-        conv_ms = (time.time() - t0) * 1e3
-        MB, KB, nt = (g.M + T - 1) // T, (g.K + T - 1) // T, nci.numel()
-        xb = torch.from_numpy(rng.integers(0, 1 << T, KB).astype(np.int32)).to(dev)
-        xb = xb.to(torch.uint8) if T == 8 else xb.to(torch.int16).view(torch.uint16)
-        cnt = torch.empty(g.M, dtype=torch.int32, device=dev)
-        yb = torch.empty(MB, dtype=xb.dtype, device=dev)
-        yf = torch.empty(g.M, device=dev)
-        ms_bin = time_graph(lambda: api.spmv_b2sr_aligned(h, T, g.M, g.K, nrp, nci, nval, xb,
-                                                          y_count=cnt, y_bin=yb), cnt)
-        ms_f32 = time_graph(lambda: api.spmv_b2sr_aligned(h, T, g.M, g.K, nrp, nci, nval, xf,
-                                                          y=yf), yf)
-        struct = (MB + 1) * 4 + nt * 4 + nval.numel() * 4
-        b_bin = struct + KB * T // 8 + g.M * 4 + MB * T // 8
-        b_f32 = struct + g.K * 4 + g.M * 4
-        out[f"spmv_b2sr{T}_aligned_89k"] = {
-            "tiles_padded": nt, "tiles": int(bci.numel()), "convert_ms_host_clock": round(conv_ms, 2),
-            "bin_ms": round(ms_bin, 5), "bin_GBps": round(b_bin / ms_bin / 1e6, 1),
-            "f32_ms": round(ms_f32, 5), "f32_GBps": round(b_f32 / ms_f32 / 1e6, 1)}
-        del brp, bci, words, nrp, nci, nval
-    torch.cuda.empty_cache()
+        t0, t1 = _events()
+        t0.record()
+        brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, 4)
+        t1.record()
+        torch.cuda.synchronize()
+        conv.append(t0.elapsed_time(t1))
+    conv_ms = float(np.median(conv))
+    N, T = 64, 4
+    nb = bci.numel()
+    B = graphs.uniform_pm1((g.K, N), seed=graphs.SEED_BASE + 2)
+    Bd = torch.from_numpy(B).to(dev)
+    C = torch.empty((g.M, N), device=dev)
+    bplan = api.B2srPlan(h, T, g.M, brp)
+    ms = time_graph(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=C, plan=bplan), C)
+    Cd = torch.empty((g.M, N), device=dev)
+    ms_d = time_graph(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=Cd), Cd)
+    cplan = api.CsrPlan(h, rp, g.nnz)
+    Cc = torch.empty((g.M, N), device=dev)
+    ms_c = time_graph(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=Cc, plan=cplan), Cc)
+    S = np.random.default_rng(3).choice([-1, 1], size=(g.K, N)).astype(np.float32)
+    Cs = api.spmm_b2sr(h, T, g.M, brp, bci, words, torch.from_numpy(S).to(dev), plan=bplan).cpu().numpy()
+    rows = check_rows(g, 4, n=2000)
+    exact = all(np.array_equal(Cs[i], S[g.colind[g.rowptr[i]:g.rowptr[i + 1]]].sum(axis=0)) for i in rows)
+    cb = b2sr_compulsory_bytes(g.M, g.K, T, nb, N)
+    out["b2sr4_fbf_89k_n64"] = {
+        "nnzb": nb, "planned_ms_graph_replay": round(ms, 5),
+        "compulsory_GBps": rate(cb, ms), "effective_GBps_gather_model":
+            rate(b2sr_gather_bytes(g.M, T, nb, g.nnz, N), ms),
+        "dropin_ms_graph_replay": round(ms_d, 5),
+        "dropin_max_abs_diff_vs_planned": float((C - Cd).abs().max()),
+        "csr_same_graph_ms": round(ms_c, 5),
+        "csr2b2sr_ms_device": round(conv_ms, 3),
+        "pm1_exact_sampled_rows": bool(exact),
+        "rel_err_fp64_sampled": sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows)}
+    Cn = np.empty((g.M, N), np.float32)
+    out["b2sr4_fbf_89k_n64"]["cpu_csr_f32_same_graph_ms"] = round(
+        cpu_time(lambda: oracle.csr_spmm_f32_omp(g.rowptr, g.colind, g.val, B, out=Cn)), 3)
+    del bplan, cplan
+    # sign-packed popcount SpMM (B signs packed 32 to a word, int32 counts)
+    sign = {}
+    for NN in (32, 64, 128):
+        Bs = torch.from_numpy(graphs.uniform_pm1((g.K, NN), seed=NN + 31)).to(dev)
+        Bw = api.pack_sign_rows(Bs)
+        for TT in (4, 8, 16, 32):
+            brp2, bci2, w2 = (brp, bci, words) if TT == 4 else api.csr2b2sr(h, rp, ci, g.M, g.K, TT)
+            Ci = torch.empty((g.M, NN), dtype=torch.int32, device=dev)
+            ms_s = time_graph(lambda: api.spmm_b2sr_sign(h, TT, g.M, g.K, NN, brp2, bci2, w2, Bw,
+                                                         C_out=Ci), Ci)
+            nbb = bci2.numel()
+            MB = (g.M + TT - 1) // TT
+            comp = 4 * (MB + 1) + nbb * (4 + TILE_BYTES[TT]) + g.K * NN // 8 + g.M * NN * 4
+            sign[f"T{TT}_N{NN}"] = {"ms": round(ms_s, 5), "compulsory_GBps": rate(comp, ms_s)}
+    out["b2sr_signpacked_89k"] = sign
+    # dense packers on the 89,250 x 128 B
+    Bs = torch.from_numpy(graphs.uniform_pm1((g.K, 128), seed=77)).to(dev)
+    W = torch.empty((g.K, 4), dtype=torch.uint32, device=dev)
+    ms_p = time_graph(lambda: W.copy_(api.pack_sign_rows(Bs)), W)
+    out["pack_sign_rows_89k_x128"] = {"ms_incl_copy": round(ms_p, 5)}
     return out
 
 
 def extra_b2sr_grid(args, h, dev):
-    """The B2SR workload's grid on the headline graph: planned FBF SpMM for
-    tiles 4/8/16/32 (and 2/64) x hidden 32/64/128, plus N = 1 (the
-    narrow kernel), beside CSR at the same N; hipGraph-replay timing."""
+    """Planned FBF SpMM for tiles 2..64 x N = 1/32/64/128 on the 89k graph
+    (N = 1 runs the narrow kernel), beside CSR at the same N."""
     import torch
 
     from spmm_eval_amd import api, graphs
@@ -374,14 +466,14 @@ def extra_b2sr_grid(args, h, dev):
     out = {"csr_us": {}, "b2sr_us": {}}
     for N in widths:
         ms = time_graph(lambda: api.spmm_csr(h, rp, ci, v, Bs[N], C_out=Cs[N], plan=cp), Cs[N],
-                        steps=50, replays=3)
+                        steps=50, replays=5)
         out["csr_us"][f"N{N}"] = round(ms * 1e3, 2)
     for T in (2, 4, 8, 16, 32, 64):
         brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
         plan = api.B2srPlan(h, T, g.M, brp)
         for N in widths:
             ms = time_graph(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bs[N], C_out=Cs[N],
-                                                  plan=plan), Cs[N], steps=50, replays=3)
+                                                  plan=plan), Cs[N], steps=50, replays=5)
             out["b2sr_us"][f"T{T}_N{N}"] = round(ms * 1e3, 2)
         plan.close()
         del brp, bci, words
@@ -389,9 +481,34 @@ def extra_b2sr_grid(args, h, dev):
     return {"b2sr_grid_89k": out}
 
 
+def extra_bmm(args, h, dev):
+    """XNOR/popcount BMM32 (backend/mm.cu:17-125) on square sizes, float and
+    sign-binarised outputs."""
+    import torch
+
+    from spmm_eval_amd import api
+
+    out = {}
+    rng = np.random.default_rng(12)
+    for n in (1024, 4096):
+        Kw = n // 32
+        A = torch.from_numpy(rng.integers(0, 2 ** 32, (Kw, n), dtype=np.uint64).astype(np.uint32)).to(dev)
+        Bw = torch.from_numpy(rng.integers(0, 2 ** 32, (Kw, n), dtype=np.uint64).astype(np.uint32)).to(dev)
+        Cf = api.bmm32_xnor_f32(A, Bw, n, n, Kw)
+        ms_f = time_graph(lambda: Cf.copy_(api.bmm32_xnor_f32(A, Bw, n, n, Kw)), Cf, steps=50)
+        Cb = api.bmm32_xnor_bin(A, Bw, n, n, n)
+        ms_b = time_graph(lambda: Cb.copy_(api.bmm32_xnor_bin(A, Bw, n, n, n)), Cb, steps=50)
+        ops = 2.0 * n * n * n  # one XNOR + one popcount-add per bit pair
+        out[f"bmm32_{n}"] = {"f32_out_ms_incl_copy": round(ms_f, 5), "bin_out_ms_incl_copy": round(ms_b, 5),
+                             "f32_out_TOPs_binary": round(ops / ms_f / 1e9, 1),
+                             "bin_out_TOPs_binary": round(ops / ms_b / 1e9, 1)}
+    return out
+
+
 def extra_fwd_bwd(args, h, dev):
     import torch
 
+    import oracle
     from spmm_eval_amd import api, graphs
 
     g, X, dY = graphs.workload_gcn(N=128)
@@ -431,27 +548,30 @@ def extra_fwd_bwd(args, h, dev):
         lanes = N // (16 // XX.element_size())
         cands = [(lanes // vec, vec, rs) for vec in (1, 2) for rs in (1, 2, 4)
                  if lanes % vec == 0 and lanes // vec <= 64]
-        return min(cands, key=lambda t: time_loop(lambda: bwd_fused(XX, GG, t), 30, 3))
+        return min(cands, key=lambda t: median_batches(lambda: bwd_fused(XX, GG, t), 30, 3, 3)[0])
+
+    def med(fn):
+        return round(median_batches(fn, launches=100)[0], 4)
 
     res = {}
     for name, XX, GG, s in (("f32", Xd, dYd, 4), ("bf16", Xh, dYh, 2)):
         geo = (pf.autotune(rp, ci, v, XX, Y), pb.autotune(trp, tci, tval, GG, dX, K=g.M))
-        ms = time_loop(lambda: step(XX, GG), 100, 10)
-        parts = {"fwd_ms": round(time_loop(lambda: fwd(XX), 100, 10), 4),
-                 "dX_ms": round(time_loop(lambda: bwd_x(GG), 100, 10), 4),
-                 "sddmm_ms": round(time_loop(lambda: bwd_v(XX, GG), 100, 10), 4)}
+        ms = med(lambda: step(XX, GG))
+        parts = {"fwd_ms": med(lambda: fwd(XX)), "dX_ms": med(lambda: bwd_x(GG)),
+                 "sddmm_ms": med(lambda: bwd_v(XX, GG))}
         tf = tune_fused(XX, GG)
-        fused_ms = time_loop(lambda: (fwd(XX), bwd_fused(XX, GG, tf)), 100, 10)
+        fused_ms = med(lambda: (fwd(XX), bwd_fused(XX, GG, tf)))
         torch.cuda.synchronize()
-        parts["fused"] = {"step_ms": round(fused_ms, 4),
-                          "bwd_ms": round(time_loop(lambda: bwd_fused(XX, GG, tf), 100, 10), 4),
+        parts["fused"] = {"step_ms": fused_ms, "bwd_ms": med(lambda: bwd_fused(XX, GG, tf)),
                           "geometry": list(tf),
                           "max_abs_diff_dX": float((dX2 - dX).abs().max()),
                           "max_rel_diff_dval": float(((dval2 - dval).abs().max()
                                                       / dval.abs().max().clamp_min(1e-30)))}
-        nbytes = 2 * csr_bytes(g.M, g.nnz, N, s_b=s) + sddmm_bytes(g.M, g.nnz, N, s)
-        rows = np.random.default_rng(7).choice(g.M, 1500, replace=False)
+        nbytes = 2 * csr_gather_bytes(g.M, g.nnz, N, s_b=s) + sddmm_gather_bytes(g.M, g.nnz, N, s)
+        rows = check_rows(g, 7)
         Yc, dvc = Y.cpu().numpy(), dval.cpu().numpy()
+        Xr = (bf16_round(X) if name == "bf16" else X).astype(np.float64)
+        dYr = (bf16_round(dY) if name == "bf16" else dY).astype(np.float64)
         e_y = sampled_rel_err(g, X.astype(np.float64), Yc, rows)
         num = den = 0.0
         for i in rows:
@@ -459,20 +579,39 @@ def extra_fwd_bwd(args, h, dev):
             ref = X[g.colind[s0:e0]].astype(np.float64) @ dY[i].astype(np.float64)
             num += float(((dvc[s0:e0] - ref) ** 2).sum())
             den += float((ref ** 2).sum())
-        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / ms / 1e6, 1), "geometry": geo, **parts,
-                     "rel_err_Y_fp64": e_y, "rel_err_dval_fp64": (num / den) ** 0.5}
-    res["bf16_speedup_vs_f32"] = round(res["f32"]["ms"] / res["bf16"]["ms"], 3)
-    # BF16 step with the fused backward against the plain FP32 step (three
-    # separate FP32 launches) and against the FP32 step with the fused backward
-    res["bf16_fused_vs_f32_plain"] = round(res["f32"]["ms"] / res["bf16"]["fused"]["step_ms"], 3)
-    res["bf16_fused_vs_f32_fused"] = round(res["f32"]["fused"]["step_ms"]
-                                           / res["bf16"]["fused"]["step_ms"], 3)
+        e_yr = sampled_rel_err(g, Xr, Yc, rows)
+        res[name] = {"ms": ms, "effective_GBps_gather_model": rate(nbytes, ms), "geometry": geo,
+                     **parts, "rel_err_Y_fp64_vs_fp32_source": e_y,
+                     "rel_err_Y_fp64_of_rounded_operands": e_yr,
+                     "rel_err_dval_fp64_vs_fp32_source": (num / den) ** 0.5}
+    res["bf16_vs_f32_three_launches_each"] = round(res["f32"]["ms"] / res["bf16"]["ms"], 3)
+    res["bf16_vs_f32_fused_backward_each"] = round(res["f32"]["fused"]["step_ms"]
+                                                   / res["bf16"]["fused"]["step_ms"], 3)
+    res["bf16_fused_vs_f32_three_launches"] = round(res["f32"]["ms"] / res["bf16"]["fused"]["step_ms"], 3)
+    # CPU fwd + dX + SDDMM (OpenMP, all host threads), FP32 and BF16 operands
+    tcp, tri, tcv, _ = oracle.csr2csc(g.rowptr, g.colind, g.val, g.K)
+    Yn = np.empty((g.M, N), np.float32)
+    dXn = np.empty((g.K, N), np.float32)
+    dvn = np.empty(g.nnz, np.float32)
+    Xhn, dYhn = Xh.cpu().numpy().view(np.uint16), dYh.cpu().numpy().view(np.uint16)
+    cpu = {}
+    for name, XX, GG in (("f32", X, dY), ("bf16", Xhn, dYhn)):
+        spmm = oracle.csr_spmm_bf16_omp if name == "bf16" else oracle.csr_spmm_f32_omp
+        f = cpu_time(lambda: spmm(g.rowptr, g.colind, g.val, XX, out=Yn))
+        b = cpu_time(lambda: spmm(tcp, tri, tcv, GG, out=dXn))
+        sd = cpu_time(lambda: oracle.sddmm_omp(g.rowptr, g.colind, GG, XX, out=dvn))
+        cpu[name] = {"fwd_ms": round(f, 3), "dX_ms": round(b, 3), "sddmm_ms": round(sd, 3),
+                     "step_ms": round(f + b + sd, 3),
+                     "gpu_speedup_three_launches": round((f + b + sd) / res[name]["ms"], 1)}
+    res["cpu"] = cpu
+    res["cpu_threads"] = oracle.omp_threads()
     return {"gcn_fwd_bwd_89k_n128": res}
 
 
 def extra_batched(args, h, dev):
     import torch
 
+    import oracle
     from spmm_eval_amd import api, graphs
     from spmm_eval_amd.batched import BatchPlan
 
@@ -483,7 +622,7 @@ def extra_batched(args, h, dev):
     X = torch.from_numpy(graphs.normal((M, N), 21, 1)).to(dev)
     dY = torch.from_numpy(graphs.normal((M, N), 21, 2)).to(dev)
     Xh, dYh = api.to_bf16_bits(X), api.to_bf16_bits(dY)
-    ms_graph = time_loop(lambda: plan.step(Xh, dYh, use_graph=True), 50, 5)
+    ms_graph, _ = median_batches(lambda: plan.step(Xh, dYh, use_graph=True), launches=100)
     # 512 separate plain FP32 launches per operation
     per = []
     ro = plan.row_off
@@ -502,34 +641,156 @@ def extra_batched(args, h, dev):
             api.spmm_csr(h, trp, tci, tval, dY[r0:r1], K=g.M, C_out=dX[r0:r1], plan=pb)
             api.sddmm_csr(h, rp, ci, dY[r0:r1], X[r0:r1], out=dval[e0:e1])
 
-    ms_sep = time_loop(separate, 5, 1)
+    ms_sep, _ = median_batches(separate, launches=3, batches=5, warmup=1)
     info = plan.info()
-    # each SDDMM form at the tuned geometry (form 2 only on one column tile)
     forms = {}
     for form in (0, 1, 2):
         if form == 2 and info["lanes_per_row"] * info["vec"] * 8 != N:
             continue
         plan.set_geometry(info["lanes_per_row"], info["vec"], info["rows_scale"],
                           info["persistent"], form)
-        forms[str(form)] = round(time_loop(lambda: plan.step(Xh, dYh, use_graph=True), 20, 3), 4)
+        forms[str(form)] = round(median_batches(lambda: plan.step(Xh, dYh, use_graph=True), 30, 5, 3)[0], 4)
+    # CPU: the same fwd + dX + SDDMM over the block-diagonal union, FP32
+    u, _, _ = graphs.concat_block_diagonal(gs)
+    tcp, tri, tcv, _ = oracle.csr2csc(u.rowptr, u.colind, u.val, u.K)
+    Xn, dYn = X.cpu().numpy(), dY.cpu().numpy()
+    Yn, dXn, dvn = np.empty((M, N), np.float32), np.empty((M, N), np.float32), np.empty(u.nnz, np.float32)
+
+    def cpu_step():
+        oracle.csr_spmm_f32_omp(u.rowptr, u.colind, u.val, Xn, out=Yn)
+        oracle.csr_spmm_f32_omp(tcp, tri, tcv, dYn, out=dXn)
+        oracle.sddmm_omp(u.rowptr, u.colind, dYn, Xn, out=dvn)
+
+    cms = cpu_time(cpu_step, warmup=2, reps=5)
     plan.close()
     return {"batched_512_fwd_bwd_n64": {"fused_launch": info, "sddmm_form_ms": forms,
         "graphs": len(gs), "rows": M, "nnz": plan.nnz,
-        "bf16_one_graph_ms": round(ms_graph, 4), "f32_separate_launches_ms": round(ms_sep, 3),
-        "speedup": round(ms_sep / ms_graph, 2)}}
+        "bf16_one_graph_ms_median_of_5": round(ms_graph, 4),
+        "f32_separate_launches_ms_median_of_5": round(ms_sep, 3),
+        "speedup": round(ms_sep / ms_graph, 2),
+        "cpu_f32_union_step_ms": round(cms, 2), "cpu_threads": oracle.omp_threads()}}
+
+
+def extra_spmv_aligned(args, h, dev):
+    """Aligned B2SR-8/16 repack and the binary SpMV over it on the 89k graph,
+    beside CSR SpMM at N = 1 (the csrmv the reference checks against)."""
+    import torch
+
+    from spmm_eval_amd import api, graphs
+
+    out = {}
+    g = graphs.workload_b2sr_graph()
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    rng = np.random.default_rng(11)
+    xf = torch.from_numpy(rng.uniform(-1, 1, g.K).astype(np.float32)).to(dev)
+    plan = api.CsrPlan(h, rp, g.nnz)
+    y = torch.empty((g.M, 1), device=dev)
+    ms_csr = time_graph(lambda: api.spmm_csr(h, rp, ci, v, xf[:, None], C_out=y, plan=plan), y)
+    out["spmv_csr_f32_89k"] = {"ms": round(ms_csr, 5),
+                               "effective_GBps": rate(csr_gather_bytes(g.M, g.nnz, 1), ms_csr)}
+    del plan
+    for T in (8, 16):
+        brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
+        nrp, nci, nval = api.b2sr_aligned(h, T, brp, bci, words)
+        torch.cuda.synchronize()
+        MB, KB, nt = (g.M + T - 1) // T, (g.K + T - 1) // T, nci.numel()
+        xb = torch.from_numpy(rng.integers(0, 1 << T, KB).astype(np.int32)).to(dev)
+        xb = xb.to(torch.uint8) if T == 8 else xb.to(torch.int16).view(torch.uint16)
+        cnt = torch.empty(g.M, dtype=torch.int32, device=dev)
+        yb = torch.empty(MB, dtype=xb.dtype, device=dev)
+        yf = torch.empty(g.M, device=dev)
+        ms_bin = time_graph(lambda: api.spmv_b2sr_aligned(h, T, g.M, g.K, nrp, nci, nval, xb,
+                                                          y_count=cnt, y_bin=yb), cnt)
+        ms_f32 = time_graph(lambda: api.spmv_b2sr_aligned(h, T, g.M, g.K, nrp, nci, nval, xf,
+                                                          y=yf), yf)
+        struct = (MB + 1) * 4 + nt * 4 + nval.numel() * 4
+        b_bin = struct + KB * T // 8 + g.M * 4 + MB * T // 8
+        b_f32 = struct + g.K * 4 + g.M * 4
+        out[f"spmv_b2sr{T}_aligned_89k"] = {
+            "tiles_padded": nt, "tiles": int(bci.numel()),
+            "bin_ms": round(ms_bin, 5), "bin_GBps": rate(b_bin, ms_bin),
+            "f32_ms": round(ms_f32, 5), "f32_GBps": rate(b_f32, ms_f32)}
+        del brp, bci, words, nrp, nci, nval
+    torch.cuda.empty_cache()
+    return out
+
+
+def gather_calib(args, h, dev):
+    """Gather-shaped calibration of the L2-miss counters and measured gather
+    ceilings, with the product CSR kernel on synthetic CSR matrices:
+    (a) every row of a 1 GiB B read exactly once in random order (a random
+        permutation of the columns, 8 entries per row): the bytes the kernel
+        must fetch past L2 are known (B + colind + val + rowptr + the plan's
+        row list and spans), so FETCH_SIZE of this dispatch calibrates the
+        counter for random 256-byte (N=64) and 512-byte (N=128) rows;
+    (b) uniform random columns, 232,965 rows x 128 entries, from B footprints
+        of 2 MiB (L2-resident), 64 MiB (MALL-resident) and 1 GiB (HBM): the
+        gather rate each level of the memory system sustains."""
+    import torch
+
+    from spmm_eval_amd import api
+
+    rng = np.random.default_rng(2024)
+    res = {"permutation": {}, "ceilings": {}}
+    for N in (64, 128):
+        K = (1 << 30) // (N * 4)
+        d = 8
+        M = K // d
+        perm = rng.permutation(K).astype(np.int32).reshape(M, d)
+        ci = np.sort(perm, axis=1).reshape(-1)
+        rp = np.arange(0, K + 1, d, dtype=np.int32)
+        rpd, cid = torch.from_numpy(rp).to(dev), torch.from_numpy(ci).to(dev)
+        v = torch.ones(K, device=dev)
+        B = torch.rand((K, N), device=dev)
+        C = torch.empty((M, N), device=dev)
+        plan = api.CsrPlan(h, rpd, K)
+        tune = (16 if N == 64 else 32, 1, 1)
+        api.spmm_csr(h, rpd, cid, v, B, C_out=C, plan=plan, tune=tune)
+        ms, _ = median_batches(lambda: api.spmm_csr(h, rpd, cid, v, B, C_out=C, plan=plan, tune=tune),
+                               launches=10, batches=3, warmup=2)
+        fetch = K * N * 4 + 8 * K + 4 * (M + 1) + 12 * M  # B, colind+val, rowptr, rows+spans
+        res["permutation"][f"N{N}"] = {"rows_of_B": K, "B_bytes": K * N * 4, "fetch_bytes": fetch,
+                                       "write_bytes": M * N * 4, "grid_rows": M, "lanes": tune[0],
+                                       "ms": round(ms, 4), "GBps": rate(fetch + M * N * 4, ms)}
+        log(f"permutation gather N={N}: {ms:.3f} ms")
+        del rpd, cid, v, B, C, plan
+        torch.cuda.empty_cache()
+    M, d = 232_965, 128
+    for N in (64, 128):
+        for label, nbytes in (("L2_2MiB", 2 << 20), ("MALL_64MiB", 64 << 20), ("HBM_1GiB", 1 << 30)):
+            K = nbytes // (N * 4)
+            ci = np.sort(rng.integers(0, K, size=(M, d), dtype=np.int32), axis=1).reshape(-1)
+            rp = np.arange(0, M * d + 1, d, dtype=np.int32)
+            rpd, cid = torch.from_numpy(rp).to(dev), torch.from_numpy(ci).to(dev)
+            v = torch.ones(M * d, device=dev)
+            B = torch.rand((K, N), device=dev)
+            C = torch.empty((M, N), device=dev)
+            plan = api.CsrPlan(h, rpd, M * d)
+            geo = plan.autotune(rpd, cid, v, B, C)
+            ms, _ = median_batches(lambda: api.spmm_csr(h, rpd, cid, v, B, C_out=C, plan=plan),
+                                   launches=20, batches=5, warmup=3)
+            res["ceilings"][f"N{N}_{label}"] = {
+                "geometry": geo, "ms": round(ms, 4), "gathers": M * d,
+                "G_gathers_per_s": round(M * d / ms / 1e6, 2),
+                "gather_GBps": rate(M * d * N * 4, ms)}
+            log(f"ceiling N={N} {label}: {ms:.3f} ms")
+            del rpd, cid, v, B, C, plan
+            torch.cuda.empty_cache()
+    return {"gather_calibration": res}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--tile", type=int, default=4)
-    ap.add_argument("--hidden", type=int, default=64)
-    ap.add_argument("--big-steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--hidden", type=int, default=128)
+    ap.add_argument("--batch-launches", type=int, default=100,
+                    help="launches per batch of the median-of-5 timing")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["csr", "fwd_bwd", "batched", "spmv_aligned", "b2sr_grid"], default=None,
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--only", choices=["csr_small", "fwd_bwd", "batched", "spmv_aligned", "b2sr",
+                                       "b2sr_grid", "bmm", "gather_calib"], default=None,
                     help="run one extra workload alone and print its result (for profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
@@ -556,24 +817,24 @@ def main():
 
     h = api.Handle()
     if args.only:
-        fn = {"csr": extra_csr, "fwd_bwd": extra_fwd_bwd, "batched": extra_batched,
-              "spmv_aligned": extra_spmv_aligned, "b2sr_grid": extra_b2sr_grid}[args.only]
+        fn = {"csr_small": extra_csr_small, "fwd_bwd": extra_fwd_bwd, "batched": extra_batched,
+              "spmv_aligned": extra_spmv_aligned, "b2sr": extra_b2sr, "b2sr_grid": extra_b2sr_grid,
+              "bmm": extra_bmm, "gather_calib": gather_calib}[args.only]
         print(json.dumps(fn(args, h, dev)))
         h.close()
         return
-    log(f"headline: B2SR-{args.tile} FBF SpMM, N={args.hidden}")
+    log(f"headline: CSR FP32 SpMM, 233k rows, N={args.hidden}")
     hd = headline(args, h, dev)
-    gbs = hd["nbytes"] / (hd["ms_step"] * 1e-3) / 1e9
-    ach = hd["nbytes"] / (hd["ms_launch"] * 1e-3) / 1e9
-    kbuf = "true" if hd["g"].K * hd["N"] * 4 < 2 ** 31 else "false"  # B through a buffer resource
-    if hd["T"] <= 8:  # block-row kernels: narrow tiles, wide tiles
-        kname = f"b2sr_brow_kernel<{hd['T']}, float, {b2sr_lanes(hd['N'])}, {kbuf}>"
-    else:
-        kname = f"b2sr_wide_kernel<{hd['T']}, float, {min(32, b2sr_lanes(hd['N']))}, {kbuf}>"
-    traffic = profiled_traffic(f"b2sr_fbf_t{hd['T']}_n{hd['N']}_89k.md", kname)
+    g, N = hd["g"], hd["N"]
+    comp = csr_compulsory_bytes(g.M, g.K, g.nnz, N)
+    gath = csr_gather_bytes(g.M, g.nnz, N)
+    G, VEC, _ = hd["geo"]
+    kname = f"void spmm::csr_spmm_kernel<float, {G}, {VEC}>"
+    traffic = profiled_traffic(f"csr_f32_233k_n{N}.md", kname)
+    ach = comp / (hd["ms_med"] * 1e-3) / 1e9
     result = {
-        "metric": "SpMM HBM GB/s (algorithmic) -- B2SR-4 FBF SpMM, FP32 B",
-        "value": round(gbs, 2),
+        "metric": f"SpMM HBM GB/s (compulsory bytes) -- CSR FP32 SpMM, 232,965-row power-law graph, N={N}",
+        "value": rate(comp, hd["ms_step"]),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -583,32 +844,48 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Chung-Lu power-law graph, uniform[-1,1] B)",
-        "config": {"workload": "b2sr_fbf_spmm", "rows": hd["g"].M, "nnz": hd["g"].nnz,
-                   "tile": hd["T"], "nnzb": hd["nb"], "hidden": hd["N"], "parallelism": "single GPU"},
-        "gflops": round(hd["flops"] / (hd["ms_step"] * 1e-3) / 1e9, 1),
-        "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
+        "data": "synthetic (seeded Chung-Lu power-law graph, values U(0,1] row-normalised, B U[-1,1])",
+        "config": {"workload": "csr_spmm_f32_powerlaw_233k", "rows": g.M, "nnz": g.nnz,
+                   "max_row_degree": int(g.degrees().max()), "hidden": N,
+                   "plan": "row schedule analysed once, geometry autotuned",
+                   "geometry_lanes_vec_rows": list(hd["geo"]), "parallelism": "single GPU"},
+        "gflops": round(2.0 * g.nnz * N / (hd["ms_step"] * 1e-3) / 1e9, 1),
+        "pct_hbm_peak": round(100 * comp / (hd["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+        "effective_GBps_gather_model": rate(gath, hd["ms_step"]),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": "profiles/ counter summary (FETCH_SIZE x2 + WRITE_SIZE x1, "
-                                       "factors from a 256 MiB copy)" if traffic else None,
-                     "kernel": kname,
-                     "bytes_per_launch": hd["nbytes"], "mean_launch_ms": round(hd["ms_launch"], 5)},
-        "checks": {"pm1_exact_2000_rows": bool(hd["exact"]), "rel_err_fp64_sampled": hd["err"]},
-        "csr2b2sr_ms": round(hd["conv_ms"], 2),
-        "dropin_unplanned": {"ms": round(hd["dropin_ms"], 5),
-                             "GBps": round(hd["nbytes"] / hd["dropin_ms"] / 1e6, 1),
-                             "max_abs_diff_vs_planned": hd["dropin_diff"]},
-        "csr_same_graph": {"ms": round(hd["csr_ms"], 5),
-                           "GBps": round(hd["csr_bytes"] / hd["csr_ms"] / 1e6, 1),
-                           "max_abs_diff_vs_b2sr": hd["csr_maxdiff"]},
+                     "traffic_source": f"profiles/csr_f32_233k_n{N}.md (FETCH_SIZE and WRITE_SIZE passes "
+                                       "scaled by the factors measured there)" if traffic else None,
+                     "kernel": kname, "bytes_per_launch": comp,
+                     "bytes_model": "compulsory: rowptr + colind + val + B + C, each once",
+                     "mean_launch_ms": round(hd["ms_med"], 5),
+                     "launch_ms_source": "median of 5 batches of back-to-back launches, events on the "
+                                         "launch stream"},
+        "timing": {"timed_region_ms_per_step": round(hd["ms_step"], 5),
+                   "median_of_5_batches_ms": round(hd["ms_med"], 5), "batches_ms": hd["batches"],
+                   "launches_per_batch": args.batch_launches},
+        "checks": {"rel_err_fp64_sampled_rows_incl_heaviest": hd["err"], "bar": 1e-5},
+        "analyze_ms_host_clock": round(hd["analyze_ms"], 2),
+        "dropin_plan_free": {"ms_median_of_5": round(hd["dropin_ms"], 4),
+                             "rel_err_fp64_sampled": hd["dropin_err"]},
     }
     if not args.no_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(hd)
     if not args.quick:
         extra = {}
-        for fn in (extra_csr, extra_fwd_bwd, extra_batched, extra_spmv_aligned, extra_b2sr_grid):
+        try:
+            log("extra_large_variants")
+            extra.update(extra_large_variants(args, h, dev, hd))
+        except Exception as e:
+            extra["extra_large_variants"] = {"error": repr(e)}
+        hd["plan"].close()
+        del hd
+        import torch as _t
+
+        _t.cuda.empty_cache()
+        for fn in (extra_csr_small, extra_b2sr, extra_b2sr_grid, extra_bmm, extra_fwd_bwd,
+                   extra_batched, extra_spmv_aligned):
             log(fn.__name__)
             try:
                 extra.update(fn(args, h, dev))

@@ -67,6 +67,8 @@ def lib():
             "orc_bmm32_xnor_bin": ([_i, _i, _i, _u32p, _u32p, _u32p], None),
             "orc_csr2csc": ([_i, _i, _i32p, _i32p, _vp, _i32p, _i32p, _vp, _vp], None),
             "orc_sddmm_f64": ([_i, _i, _i32p, _i32p, _f32p, _i, _f32p, _i, _f64p], None),
+            "orc_sddmm_f32_omp": ([_i, _i, _i32p, _i32p, _f32p, _i, _f32p, _i, _f32p], None),
+            "orc_sddmm_bf16_omp": ([_i, _i, _i32p, _i32p, _u16p, _i, _u16p, _i, _f32p], None),
             "orc_b2sr_aligned_size": ([_i, _i, _i32p], _i),
             "orc_b2sr_aligned_pack": ([_i, _i, _i32p, _i32p, _vp, _i32p, _i32p, _u32p], None),
             "orc_spmv_b2sr_aligned": ([_i, _i, _i32p, _i32p, _u32p, _vp, _i32p, _vp], None),
@@ -261,6 +263,18 @@ def sddmm_f64(rowptr, colind, dY, X) -> np.ndarray:
     M, N = len(rowptr) - 1, dY.shape[1]
     out = np.empty(max(int(rowptr[-1]), 1), np.float64)
     lib().orc_sddmm_f64(M, N, rowptr, colind if len(colind) else np.zeros(1, np.int32), dY, N, X, X.shape[1], out)
+    return out[: int(rowptr[-1])]
+
+
+def sddmm_omp(rowptr, colind, dY, X, out=None) -> np.ndarray:
+    """FP32 SDDMM at the stored positions (the timed CPU baseline); BF16
+    operands when dY and X are uint16 bit patterns."""
+    M, N = len(rowptr) - 1, dY.shape[1]
+    if out is None:
+        out = np.empty(max(int(rowptr[-1]), 1), np.float32)
+    fn = lib().orc_sddmm_bf16_omp if dY.dtype == np.uint16 else lib().orc_sddmm_f32_omp
+    fn(M, N, rowptr, colind if len(colind) else np.zeros(1, np.int32), dY, dY.shape[1], X,
+       X.shape[1], out)
     return out[: int(rowptr[-1])]
 
 

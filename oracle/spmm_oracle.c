@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY. Only the tests, the smoke check and the CPU
  * baseline leg of the benchmark may load this library, and only as the
- * checker (or, for orc_csr_spmm_f32_omp, as the timed CPU baseline). The
+ * checker (or, for the *_omp functions, as the timed CPU baseline). The
  * product library never calls it.
  *
  * Each function cites the reference file:line whose behaviour it restates
@@ -430,6 +430,39 @@ void orc_sddmm_f64(int M, int N, const int *rowptr, const int *colind,
       for (int k = 0; k < N; ++k) s += (double)y[k] * (double)x[k];
       out[e] = s;
     }
+}
+
+/* The timed CPU baseline of the backward's SDDMM: out[e] = dY[i] . X[j] in
+ * FP32 (sum in k order), rows in dynamic chunks of 64. Same operation as
+ * orc_sddmm_f64 in FP32; its BF16 form reads both operands as bf16 bits. */
+void orc_sddmm_f32_omp(int M, int N, const int *rowptr, const int *colind,
+                       const float *dY, int lddy, const float *X, int ldx,
+                       float *out) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < M; ++i) {
+    const float *restrict y = dY + (int64_t)i * lddy;
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+      const float *restrict x = X + (int64_t)colind[e] * ldx;
+      float s = 0.f;
+      for (int k = 0; k < N; ++k) s += y[k] * x[k];
+      out[e] = s;
+    }
+  }
+}
+
+void orc_sddmm_bf16_omp(int M, int N, const int *rowptr, const int *colind,
+                        const uint16_t *dY, int lddy, const uint16_t *X, int ldx,
+                        float *out) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < M; ++i) {
+    const uint16_t *restrict y = dY + (int64_t)i * lddy;
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+      const uint16_t *restrict x = X + (int64_t)colind[e] * ldx;
+      float s = 0.f;
+      for (int k = 0; k < N; ++k) s += bf16_to_f32(y[k]) * bf16_to_f32(x[k]);
+      out[e] = s;
+    }
+  }
 }
 
 /* ------------------------------------------------- aligned B2SR repack --

@@ -258,6 +258,14 @@ def test_sddmm_matches_dense():
     rows = np.repeat(np.arange(60), g.degrees())
     full = dY.astype(np.float64) @ X.astype(np.float64).T
     np.testing.assert_allclose(out, full[rows, g.colind], rtol=1e-12, atol=1e-12)
+    # the timed CPU baselines: FP32 within FP32 rounding of the dense
+    # product, BF16 equal to the dense product of the rounded operands
+    o32 = oracle.sddmm_omp(g.rowptr, g.colind, dY, X)
+    np.testing.assert_allclose(o32, full[rows, g.colind], rtol=1e-5, atol=1e-5)
+    dYh, Xh = oracle.f32_to_bf16(dY), oracle.f32_to_bf16(X)
+    o16 = oracle.sddmm_omp(g.rowptr, g.colind, dYh, Xh)
+    fullh = oracle.bf16_to_f32(dYh).astype(np.float64) @ oracle.bf16_to_f32(Xh).astype(np.float64).T
+    np.testing.assert_allclose(o16, fullh[rows, g.colind], rtol=1e-5, atol=1e-5)
 
 
 def test_sddmm_is_value_gradient():
