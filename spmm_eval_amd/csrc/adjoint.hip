@@ -116,6 +116,18 @@ gather_kernel(int nnz, const int *__restrict__ pos, const int *__restrict__ rowi
   }
 }
 
+// Set / copy int arrays with kernels rather than memset / memcpy nodes, so
+// the plan-free transposed SpMM can be captured into a graph.
+__global__ void __launch_bounds__(kAdjThreads)
+fill_ints_kernel(int *__restrict__ d, int n, int v) {
+  for (int i = blockIdx.x * kAdjThreads + threadIdx.x; i < n; i += gridDim.x * kAdjThreads) d[i] = v;
+}
+
+__global__ void __launch_bounds__(kAdjThreads)
+copy_ints_kernel(const int *__restrict__ a, int *__restrict__ b, int n) {
+  for (int i = blockIdx.x * kAdjThreads + threadIdx.x; i < n; i += gridDim.x * kAdjThreads) b[i] = a[i];
+}
+
 template <typename TB, int G>
 __global__ void __launch_bounds__(kAdjThreads)
 sddmm_kernel(int M, int nnz, const int *__restrict__ rowptr, const int *__restrict__ colind,
@@ -278,35 +290,35 @@ template int run_sddmm<uint16_t>(const spmm_csr_plan_s *, int, int, int, int, co
                                  const int *, const uint16_t *, int, const uint16_t *, int,
                                  float *, void *);
 
-}  // namespace spmm
-
-using namespace spmm;
-
-extern "C" {
-
-int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const int *colind,
-                  const float *val, int *tRowPtr, int *tColInd, float *tVal, int *perm,
-                  void *stream) {
+// The stable transpose (csr2csc order, readMtx.hpp:319-345) without the
+// final synchronisation: atomic column counts -> scan -> scatter of source
+// positions -> per-column sort of those positions -> gather. Work is queued
+// on `s` only (kernels, no memset or memcpy nodes), so it can be captured
+// once the handle's workspace has grown to this size.
+int transpose_async(spmm_handle_s *h, int M, int K, int nnz, const int *rowptr, const int *colind,
+                    const float *val, int *tRowPtr, int *tColInd, float *tVal, int *perm,
+                    hipStream_t s) {
   SPMM_CHECK_ARG(h && tRowPtr, "null handle or tRowPtr");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && nnz >= 0, "negative size");
   SPMM_CHECK_ARG(M == 0 || rowptr, "null rowptr");
   SPMM_CHECK_ARG(nnz == 0 || (colind && tColInd), "null colind or tColInd");
   SPMM_CHECK_ARG(!(tVal && !val && nnz), "tVal requested without val");
-  hipStream_t s = as_stream(stream);
   // Workspace: cursor[K+1] | scan partials | pos[nnz] | rowidx[nnz]
   const size_t a0 = 0;
   const size_t a1 = (a0 + sizeof(int) * (static_cast<size_t>(K) + 1) + 255) & ~size_t(255);
   const size_t a2 = (a1 + scan_workspace_bytes(K + 1) + 255) & ~size_t(255);
   const size_t a3 = (a2 + sizeof(int) * static_cast<size_t>(nnz) + 255) & ~size_t(255);
   const size_t total = a3 + sizeof(int) * static_cast<size_t>(nnz) + 256;
-  char *ws = static_cast<char *>(workspace(h, total));
+  char *ws = static_cast<char *>(workspace(h, total, s));
   if (!ws) return SPMM_ERR_ALLOC;
   int *cursor = reinterpret_cast<int *>(ws + a0);
   int *part = reinterpret_cast<int *>(ws + a1);
   int *pos = reinterpret_cast<int *>(ws + a2);
   int *rowidx = reinterpret_cast<int *>(ws + a3);
 
-  SPMM_HIP(hipMemsetAsync(tRowPtr, 0, sizeof(int) * (static_cast<size_t>(K) + 1), s));
+  const int kgrid = std::max(1, std::min(ceil_div(K + 1, kAdjThreads), 1024));
+  hipLaunchKernelGGL(fill_ints_kernel, dim3(kgrid), dim3(kAdjThreads), 0, s, tRowPtr, K + 1, 0);
+  SPMM_LAUNCHED();
   const int grid = std::max(1, std::min(ceil_div(nnz, kAdjThreads), 4096));
   if (nnz > 0) {
     hipLaunchKernelGGL(col_count_kernel, dim3(grid), dim3(kAdjThreads), 0, s, nnz, colind,
@@ -315,8 +327,9 @@ int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, con
   }
   SPMM_HIP(exclusive_scan(tRowPtr, K + 1, part, s));
   if (nnz == 0) return SPMM_OK;
-  SPMM_HIP(hipMemcpyAsync(cursor, tRowPtr, sizeof(int) * (static_cast<size_t>(K) + 1),
-                          hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(copy_ints_kernel, dim3(kgrid), dim3(kAdjThreads), 0, s, tRowPtr, cursor,
+                     K + 1);
+  SPMM_LAUNCHED();
   hipLaunchKernelGGL(scatter_kernel, dim3(grid), dim3(kAdjThreads), 0, s, nnz, colind, cursor, pos);
   SPMM_LAUNCHED();
   if (K > 0) {
@@ -329,8 +342,24 @@ int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, con
   hipLaunchKernelGGL(gather_kernel, dim3(grid), dim3(kAdjThreads), 0, s, nnz, pos, rowidx, val,
                      tColInd, tVal, perm);
   SPMM_LAUNCHED();
-  // The workspace is reused by the next call on this handle.
-  SPMM_HIP(hipStreamSynchronize(s));
+  return SPMM_OK;
+}
+
+}  // namespace spmm
+
+using namespace spmm;
+
+extern "C" {
+
+int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr, const int *colind,
+                  const float *val, int *tRowPtr, int *tColInd, float *tVal, int *perm,
+                  void *stream) {
+  const int st = transpose_async(h, M, K, nnz, rowptr, colind, val, tRowPtr, tColInd, tVal, perm,
+                                 as_stream(stream));
+  if (st) return st;
+  // the workspace is reused by the next call on this handle, whatever stream
+  // that call uses
+  SPMM_HIP(hipStreamSynchronize(as_stream(stream)));
   return SPMM_OK;
 }
 

@@ -53,9 +53,6 @@ template <typename U> __device__ __forceinline__ int top_bit(U x) {  // x != 0
 constexpr int kB2Threads = 256;
 constexpr int kB2Waves = kB2Threads / kWave;
 constexpr int kLongChunks = 2;  // block rows above this many tile chunks go to b2sr_long_kernel
-// The handle's plan-free state: kStateHead counters (zero between calls;
-// each kernel that counts resets those it used), then the block-row lists.
-constexpr int kStateHead = 8;
 // plan-free wide tiles: block rows of at least this many tiles go first
 // (on a power-law graph: T = 16 at 256 tiles 105 us against 124 at 512 and
 // 173 at 128; T = 32 and 64 at 512 tiles 145 and 264 us against 208 and
@@ -856,47 +853,7 @@ b2sr_brow_kernel(LaunchTable tab, const int *__restrict__ brows, int M,
 }
 
 // ------------------------------------------- plan-free form, T <= 8
-// The planned form's classes found on the device each call: block rows of
-// at least w_tiles tiles (one wave chunk) are appended to the
-// workgroup-class list (state + kStateHead, count state[0]), the others of
-// at least a_tiles tiles (one subgroup chunk) to the wave-class list
-// (state + kStateHead + MB, count state[2]); one atomic per workgroup and
-// list, positions from the waves' ballots.
-__global__ void __launch_bounds__(kB2Threads)
-b2sr_class_kernel(int MB, const int *__restrict__ brp, int a_tiles, int w_tiles,
-                  int *__restrict__ state) {
-  __shared__ int cnt[2][kB2Waves];
-  __shared__ int base[2];
-  const int br = static_cast<int>(blockIdx.x) * kB2Threads + static_cast<int>(threadIdx.x);
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int n = br < MB ? brp[br + 1] - brp[br] : 0;
-  const bool in[2] = {br < MB && n >= w_tiles, br < MB && n < w_tiles && n >= a_tiles};
-  const unsigned long long below = (1ull << lane) - 1ull;
-  unsigned long long m[2];
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    m[c] = __ballot(in[c]);
-    if (lane == 0) cnt[c][wave] = __popcll(m[c]);
-  }
-  __syncthreads();
-  if (threadIdx.x < 2) {
-    const int c = threadIdx.x;
-    int tot = 0;
-#pragma unroll
-    for (int w = 0; w < kB2Waves; ++w) tot += cnt[c][w];
-    base[c] = tot ? atomicAdd(&state[c == 0 ? 0 : 2], tot) : 0;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    if (!in[c]) continue;
-    int pos = base[c] + __popcll(m[c] & below);
-    for (int w = 0; w < wave; ++w) pos += cnt[c][w];
-    state[kStateHead + (c == 0 ? 0 : MB) + pos] = br;
-  }
-}
-
-// One launch over the classes b2sr_class_kernel listed, with the planned
+// One launch over the classes class_list_kernel listed, with the planned
 // form's per-class code and static shares (one counter shared by every
 // workgroup would serialise their atomics across the XCDs). The first E
 // workgroups (as many as the chip holds at once, or half) take the listed work:
@@ -1296,7 +1253,7 @@ __device__ __forceinline__ void wide_brow(int br, int2 sp, int M, const int *__r
 
 // Planned (kPlanFree false): workgroup x takes the plan's block row x.
 // Plan-free: the block rows of at least heavy_tiles tiles, listed by
-// b2sr_class_kernel (state + kStateHead + MB, count state[2]), go first to
+// class_list_kernel (state + kStateHead + MB, count state[2]), go first to
 // the E workgroups resident from the start (x, x + E, ...); workgroup E + k
 // then takes block row k unless it is listed. The last workgroup to finish
 // zeroes the list count.
@@ -1480,31 +1437,6 @@ static void dispatch_fbf(const Geo &g, int M, int N, const int *brp, const int *
   }
 }
 
-// The long-block-row list of the handle, sized for MB block rows. Growing
-// allocates zeroed counters (synchronously); afterwards the kernels keep
-// the counters at zero between calls.
-// The handle's plan-free state for MB block rows: kStateHead counters and
-// two lists of MB block rows; zeroed when (re)allocated.
-static int b2sr_state(spmm_handle_s *h, int MB, int **out) {
-  const size_t need = kStateHead + 2 * static_cast<size_t>(MB);
-  if (h->b2sr_cap < need) {
-    if (h->b2sr_state) {
-      if (hipDeviceSynchronize() != hipSuccess) SPMM_FAIL(SPMM_ERR_HIP, "sync before regrowing");
-      (void)hipFree(h->b2sr_state);
-      h->b2sr_state = nullptr;
-      h->b2sr_cap = 0;
-    }
-    const size_t cap = need + need / 4 + 64;
-    if (hipMalloc(&h->b2sr_state, sizeof(int) * cap) != hipSuccess)
-      SPMM_FAIL(SPMM_ERR_ALLOC, "plan-free state allocation failed");
-    if (hipMemset(h->b2sr_state, 0, sizeof(int) * cap) != hipSuccess)
-      SPMM_FAIL(SPMM_ERR_HIP, "plan-free state init");
-    h->b2sr_cap = cap;
-  }
-  *out = h->b2sr_state;
-  return SPMM_OK;
-}
-
 // Workgroups of `kernel` the chip holds at once (per-CU occupancy, cached
 // per kernel, times the CU count).
 static int resident_wgs(const void *kernel, int cus, std::atomic<int> *cache) {
@@ -1525,7 +1457,7 @@ static void launch_dropin(int M, int K, const int *brp, const int *bci, const vo
                           int cus, hipStream_t s) {
   constexpr int SW = kB2Waves * (kWave / G);
   const int MB = (M + T - 1) / T;
-  hipLaunchKernelGGL(b2sr_class_kernel, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0, s,
+  hipLaunchKernelGGL(class_list_kernel<kB2Threads>, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0, s,
                      MB, brp, BrowChunk<G>::kTiles, BrowChunk<G>::kWaveTiles, state);
   const auto *tw = static_cast<const typename TileWord<T>::type *>(val);
   const int b_bytes = buffer_span(K, ldb, sizeof(TB));
@@ -1594,7 +1526,7 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
     if (ext_state) {
       state = ext_state;
     } else {
-      const int st = b2sr_state(h, (M + T - 1) / T, &state);
+      const int st = list_state(h, (M + T - 1) / T, s, &state);
       if (st) return st;
     }
     int dev = 0;
@@ -1693,7 +1625,7 @@ static void launch_wide(const spmm_csr_plan_s *p, const int *brp, int *state, in
                    : &b2sr_wide_kernel<T, TB, G, false, true>);
   int E = 0;
   if (!p) {
-    hipLaunchKernelGGL(b2sr_class_kernel, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0,
+    hipLaunchKernelGGL(class_list_kernel<kB2Threads>, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0,
                        s, MB, brp, wide_heavy(T), INT_MAX, state);
     static std::atomic<int> per_cu[2];
     E = std::min(resident_wgs(reinterpret_cast<const void *>(kern), cus,

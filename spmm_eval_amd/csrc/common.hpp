@@ -7,6 +7,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
+#include <vector>
 
 #include "../../include/spmm_mi355x.h"
 
@@ -50,28 +51,50 @@ inline long long ceil_div_ll(long long a, long long b) { return (a + b - 1) / b;
 
 }  // namespace spmm
 
-// Handle: owns one grow-only device workspace used by the synchronous
-// helper paths (analysis, conversions, transposes). Not thread-safe; use one
-// handle per host thread.
+// Handle: owns the grow-only device buffers of the calls made through it
+// (the workspace of the synchronous helper paths -- analysis, conversions,
+// transposes --, the plan-free list state, the plan-free transposed
+// structure). Not thread-safe; use one handle per host thread, and order the
+// plan-free calls made through one handle on one stream.
 struct spmm_handle_s {
   int device = 0;
   void *ws = nullptr;
   size_t ws_bytes = 0;
   void *pinned = nullptr;  // small pinned host buffer for count read-backs
   size_t pinned_bytes = 0;
-  // plan-free B2SR state: counters, block-row lists and per-column-tile
-  // counters; counters start at 0 and are reset by the kernel that drains
-  // the lists.
-  int *b2sr_state = nullptr;
-  size_t b2sr_cap = 0;
+  // plan-free list state (B2SR and CSR drop-ins): kStateHead counters, then
+  // two row / block-row lists; counters start at 0 and are reset by the
+  // kernel that drains the lists
+  int *list_state = nullptr;
+  size_t list_cap = 0;  // ints
+  // the plan-free transposed SpMM's structure: trowptr | tcol | perm
+  int *tstruct = nullptr;
+  size_t tstruct_cap = 0;  // ints
+  // buffers replaced by larger ones, freed by spmm_destroy: work queued
+  // earlier, or a graph captured from an earlier call, may still use them
+  std::vector<void *> retired;
 };
 
 namespace spmm {
+// Counters at the head of the plan-free list state (zero between calls; each
+// kernel that counts resets those it used), then the lists.
+constexpr int kStateHead = 8;
 // Returns a workspace of at least `bytes` (device) or nullptr with the
-// error already recorded. Growing frees the previous buffer after a device
-// synchronisation, so pointers from earlier calls must not be held.
-void *workspace(spmm_handle_s *h, size_t bytes);
+// error already recorded. A buffer that grows is replaced, not freed (the
+// old one is retired until spmm_destroy); with `s` given, growing while s
+// is being captured into a graph is refused.
+void *workspace(spmm_handle_s *h, size_t bytes, hipStream_t s = nullptr);
 void *pinned(spmm_handle_s *h, size_t bytes);
+// Grow-only zeroed int buffer of at least `ints` elements in *buf / *cap,
+// same retirement and capture rules as the workspace.
+int grow_ints(spmm_handle_s *h, int **buf, size_t *cap, size_t ints, hipStream_t s);
+// The plan-free list state for lists of up to n entries each.
+inline int list_state(spmm_handle_s *h, int n, hipStream_t s, int **out) {
+  const int st = grow_ints(h, &h->list_state, &h->list_cap,
+                           kStateHead + 2 * static_cast<size_t>(n), s);
+  *out = h->list_state;
+  return st;
+}
 }  // namespace spmm
 
 // ------------------------------------------------------------------ device

@@ -133,6 +133,129 @@ __device__ __forceinline__ Seg find_seg(const LaunchTable &tab, int wg) {
   return tab.seg[lo];
 }
 
+// ------------------------------------------------ plan-free row classes
+// The classes of one plan-free call found on the device: units (CSR rows, or
+// B2SR block rows) of at least c_limit entries are appended to list 0
+// (state + kStateHead, count state[0]), the others of at least a_limit
+// entries to list 1 (state + kStateHead + n, count state[2]); one atomic per
+// workgroup and list, positions from the waves' ballots. The kernel that
+// drains the lists resets the counts.
+template <int NT>
+__global__ void __launch_bounds__(NT)
+class_list_kernel(int n, const int *__restrict__ ptr, int a_limit, int c_limit,
+                  int *__restrict__ state) {
+  constexpr int NW = NT / kWave;
+  __shared__ int cnt[2][NW];
+  __shared__ int base[2];
+  const int u = static_cast<int>(blockIdx.x) * NT + static_cast<int>(threadIdx.x);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int d = u < n ? ptr[u + 1] - ptr[u] : 0;
+  const bool in[2] = {u < n && d >= c_limit, u < n && d < c_limit && d >= a_limit};
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long m[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    m[c] = __ballot(in[c]);
+    if (lane == 0) cnt[c][wave] = __popcll(m[c]);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int c = threadIdx.x;
+    int tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) tot += cnt[c][w];
+    base[c] = tot ? atomicAdd(&state[c == 0 ? 0 : 2], tot) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (!in[c]) continue;
+    int pos = base[c] + __popcll(m[c] & below);
+    for (int w = 0; w < wave; ++w) pos += cnt[c][w];
+    state[kStateHead + (c == 0 ? 0 : n) + pos] = u;
+  }
+}
+
+// One row of each class (the per-row code of the planned kernel and of the
+// plan-free kernel, so both give the same bits at the same lane geometry).
+// Class A: one subgroup of G lanes sums the row and stores it.
+template <typename TB, int G, int VEC, bool kPerm>
+__device__ __forceinline__ void csr_row_sub(int row, int2 sp, int lane_sub, int col0,
+                                            const int *__restrict__ colind,
+                                            const float *__restrict__ val,
+                                            const TB *__restrict__ Bc, int ldb,
+                                            float *__restrict__ C, int ldc,
+                                            const int *__restrict__ perm) {
+  constexpr int E = Vec16<TB>::kElems;
+  constexpr int EPL = VEC * E;
+  float acc[EPL];
+#pragma unroll
+  for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+  accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, 0, G, lane_sub, colind, val, Bc, ldb, acc,
+                                      perm);
+  store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+}
+
+// Class B: the S subgroups of one wave split the row's chunks of G entries
+// (subgroup s takes chunks s, s + S, ...), then a butterfly; subgroup 0
+// stores.
+template <typename TB, int G, int VEC, bool kPerm>
+__device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane_sub, int col0,
+                                             const int *__restrict__ colind,
+                                             const float *__restrict__ val,
+                                             const TB *__restrict__ Bc, int ldb,
+                                             float *__restrict__ C, int ldc,
+                                             const int *__restrict__ perm) {
+  constexpr int E = Vec16<TB>::kElems;
+  constexpr int EPL = VEC * E;
+  constexpr int S = kWave / G;
+  float acc[EPL];
+#pragma unroll
+  for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+  accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, sub * G, S * G, lane_sub, colind, val, Bc, ldb,
+                                      acc, perm);
+  wave_reduce_subgroups<G, EPL>(acc);
+  if (sub == 0) store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+}
+
+// Class C: the 4 S subgroups of one workgroup split the row, the waves meet
+// in LDS in a fixed order; wave 0's subgroup 0 stores. Every thread of the
+// workgroup must call it (two barriers).
+template <typename TB, int G, int VEC, bool kPerm>
+__device__ __forceinline__ void csr_row_wg(int row, int2 sp, int wave, int sub, int lane_sub,
+                                           int col0, const int *__restrict__ colind,
+                                           const float *__restrict__ val,
+                                           const TB *__restrict__ Bc, int ldb,
+                                           float *__restrict__ C, int ldc,
+                                           const int *__restrict__ perm) {
+  constexpr int E = Vec16<TB>::kElems;
+  constexpr int EPL = VEC * E;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * EPL;
+  __shared__ float red[kWavesPerWG - 1][CW];
+  float acc[EPL];
+#pragma unroll
+  for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
+  const int gsub = wave * S + sub;
+  accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, gsub * G, kWavesPerWG * S * G, lane_sub, colind,
+                                      val, Bc, ldb, acc, perm);
+  wave_reduce_subgroups<G, EPL>(acc);
+  if (wave > 0 && sub == 0) {
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) red[wave - 1][lane_sub * EPL + q] = acc[q];
+  }
+  __syncthreads();
+  if (wave == 0 && sub == 0) {
+#pragma unroll
+    for (int w = 0; w < kWavesPerWG - 1; ++w) {
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
+    }
+    store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+  }
+  __syncthreads();  // red[] may be reused by the next row
+}
+
 // The work of one 256-thread workgroup of the CSR launch table: virtual
 // workgroup `wg`, column tile `ctile`. Used by csr_spmm_kernel and by the
 // fused batched kernel.
@@ -148,7 +271,6 @@ __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile
   constexpr int EPL = VEC * Vec16<TB>::kElems;
   constexpr int S = kWave / G;
   constexpr int CW = G * EPL;  // columns covered per column tile
-  __shared__ float red[kWavesPerWG - 1][CW];
 
   const Seg sg = find_seg(tab, wg);
   const int lane = threadIdx.x & (kWave - 1);
@@ -184,12 +306,7 @@ __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile
         row_n = rows[pos_n];
         sp_n = span[pos_n];
       }
-      float acc[EPL];
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-      accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, 0, G, lane_sub, colind, val, Bc, ldb, acc,
-                                          perm);
-      store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+      csr_row_sub<TB, G, VEC, kPerm>(row, sp, lane_sub, col0, colind, val, Bc, ldb, C, ldc, perm);
       row = row_n;
       sp = sp_n;
       pos = pos_n;
@@ -197,39 +314,11 @@ __device__ __forceinline__ void csr_wg(const LaunchTable &tab, int wg, int ctile
   } else if (sg.kind == kSegB) {
     const int pos = sg.list_off + local_wg * kWavesPerWG + wave;
     if (pos >= sg.list_off + sg.count) return;  // whole wave
-    const int row = rows[pos];
-    const int2 sp = span[pos];
-    float acc[EPL];
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, sub * G, S * G, lane_sub, colind, val, Bc,
-                                        ldb, acc, perm);
-    wave_reduce_subgroups<G, EPL>(acc);
-    if (sub == 0) store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+    csr_row_wave<TB, G, VEC, kPerm>(rows[pos], span[pos], sub, lane_sub, col0, colind, val, Bc,
+                                    ldb, C, ldc, perm);
   } else {
-    const int row = rows[sg.list_off + local_wg];
-    const int2 sp = span[sg.list_off + local_wg];
-    float acc[EPL];
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-    const int gsub = wave * S + sub;
-    accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, gsub * G, kWavesPerWG * S * G, lane_sub,
-                                        colind, val, Bc, ldb, acc, perm);
-    wave_reduce_subgroups<G, EPL>(acc);
-    if (wave > 0 && sub == 0) {
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) red[wave - 1][lane_sub * EPL + q] = acc[q];
-    }
-    __syncthreads();
-    if (wave == 0 && sub == 0) {
-#pragma unroll
-      for (int w = 0; w < kWavesPerWG - 1; ++w) {
-#pragma unroll
-        for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
-      }
-      store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
-    }
-    __syncthreads();  // red[] may be reused by the next work item
+    csr_row_wg<TB, G, VEC, kPerm>(rows[sg.list_off + local_wg], span[sg.list_off + local_wg], wave,
+                                  sub, lane_sub, col0, colind, val, Bc, ldb, C, ldc, perm);
   }
 }
 
@@ -691,6 +780,19 @@ int run_sddmm(const spmm_csr_plan_s *plan, int M, int K, int N, int nnz, const i
 // Bins rows by degree rowptr[(r >> shift) + 1] - rowptr[r >> shift].
 int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
             spmm_csr_plan_s **out, void *stream);
+// The stable transpose queued on `s` without a synchronisation (the public
+// csr_transpose adds one); capturable once the workspace has grown.
+int transpose_async(spmm_handle_s *h, int M, int K, int nnz, const int *rowptr, const int *colind,
+                    const float *val, int *tRowPtr, int *tColInd, float *tVal, int *perm,
+                    hipStream_t s);
+// Row classes of the CSR launch for G lanes per row: rows of fewer than
+// *a_limit entries run one subgroup each, rows of at least *c_limit one
+// workgroup each, the rest one wave each (both powers of two, so the
+// classes of a degree bin never straddle a limit).
+inline void csr_class_limits(int G, int *a_limit, int *c_limit) {
+  *a_limit = G <= 16 ? (8 * G > 32 ? 8 * G : 32) : 2 * G;
+  *c_limit = 64 * (kWave / G);
+}
 }  // namespace spmm
 
 // Row schedule of one CSR structure: rows permuted into degree bins

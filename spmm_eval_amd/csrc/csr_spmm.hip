@@ -17,6 +17,7 @@
 #include "csr_core.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <new>
 #include <vector>
 
@@ -107,6 +108,69 @@ csr_t_sddmm_kernel(LaunchTable tab, const int *__restrict__ rows,
                    float *__restrict__ dX, int lddx, float *__restrict__ dval) {
   csr_t_sddmm_wg<TB, G, VEC, false>(tab, blockIdx.x, rows, span, tcol, val, perm, dY, lddy, X,
                                     ldx, dX, lddx, dval);
+}
+
+// Plan-free CSR SpMM (the drop-in): one launch over the classes that
+// class_list_kernel found on the device, with the planned kernel's per-row
+// code, so a row's sum is bitwise the planned form's at the same lane
+// geometry. Workgroups x < E (the workgroups the chip holds at once) take
+// the listed rows: workgroup-class rows (list 0) x, x + E, ..., then the
+// wave-class rows (list 1) dealt to their waves from the last workgroup
+// backwards, so they land on workgroups with one workgroup-class row fewer.
+// Workgroup E + k takes rows k SW R .. (k + 1) SW R - 1, one subgroup per
+// row and round (R rounds), skipping the listed ones. Shares are static: one
+// counter shared by every workgroup would serialise their atomics across the
+// XCDs. The last workgroup to finish zeroes the list counts, so consecutive
+// or graph-replayed calls need no memset.
+constexpr int kDropinRounds = 2;
+
+template <typename TB, int G, int VEC, bool kPerm>
+__global__ void __launch_bounds__(kThreads)
+csr_dropin_kernel(int M, int E, int a_limit, const int *__restrict__ rowptr,
+                  const int *__restrict__ colind, const float *__restrict__ val,
+                  const int *__restrict__ perm, const TB *__restrict__ B, int ldb,
+                  float *__restrict__ C, int ldc, int *__restrict__ state) {
+  constexpr int E16 = Vec16<TB>::kElems;
+  constexpr int S = kWave / G, SW = kWavesPerWG * S;
+  constexpr int CW = G * VEC * E16;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1), sub = lane / G;
+  const int col0 = blockIdx.y * CW + lane_sub * E16;
+  const TB *Bc = B + col0;
+  const int x = blockIdx.x;
+  if (x < E) {
+    const int n_wg = state[0], n_wave = state[2];
+    for (int i = x; i < n_wg; i += E) {  // uniform over the workgroup
+      const int row = state[kStateHead + i];
+      csr_row_wg<TB, G, VEC, kPerm>(row, make_int2(rowptr[row], rowptr[row + 1]), wave, sub,
+                                    lane_sub, col0, colind, val, Bc, ldb, C, ldc, perm);
+    }
+    for (int j = (E - 1 - x) * kWavesPerWG + wave; j < n_wave; j += E * kWavesPerWG) {
+      const int row = state[kStateHead + M + j];
+      csr_row_wave<TB, G, VEC, kPerm>(row, make_int2(rowptr[row], rowptr[row + 1]), sub, lane_sub,
+                                      col0, colind, val, Bc, ldb, C, ldc, perm);
+    }
+  } else {
+    const int base = (x - E) * SW * kDropinRounds + wave * S + sub;
+    for (int k = 0; k < kDropinRounds; ++k) {
+      const int row = base + k * SW;
+      if (row < M) {
+        const int2 sp = make_int2(rowptr[row], rowptr[row + 1]);
+        if (sp.y - sp.x < a_limit)  // else listed
+          csr_row_sub<TB, G, VEC, kPerm>(row, sp, lane_sub, col0, colind, val, Bc, ldb, C, ldc,
+                                         perm);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // every workgroup has read the counts and lists before it counts itself
+    if (atomicAdd(&state[1], 1) == static_cast<int>(gridDim.x * gridDim.y) - 1) {
+      atomicExch(&state[0], 0);
+      atomicExch(&state[2], 0);
+      atomicExch(&state[1], 0);
+    }
+  }
 }
 
 // Narrow or unaligned N (N not a multiple of one 16-byte vector, or B / C /
@@ -237,11 +301,18 @@ static Geometry pick_geometry(const spmm_csr_plan_s *p, int N, int elem_bytes, c
   }
   g.rows_scale = t.rows_scale > 0 ? t.rows_scale : 1;
   int vec = t.vec == 2 ? 2 : 1;
+  if (t.lanes_per_row == 0 && t.vec == 0 && elem_bytes == 2) vec = 2;  // default below
   if (N % (e16 * vec) != 0) vec = 1;
   int G = tune_valid(N, e16, t.lanes_per_row, vec) ? t.lanes_per_row : 0;
   if (G == 0) {
+    // Default (no tune, nothing autotuned for this N and dtype): two 16-byte
+    // vectors per lane for BF16 B, one for FP32, and the widest power of two
+    // up to 4 (BF16) or 8 (FP32) lanes per row that divides the vectors of N;
+    // the autotuner's picks on the 232,965-row workload at N = 64 and 128
+    // (FP32 8 x 1, BF16 4 x 2) and within a few percent of them elsewhere.
     const int lanes = N / (e16 * vec);
-    for (int c = 64; c >= 1; c >>= 1)
+    const int cap = elem_bytes == 2 ? 4 : 8;
+    for (int c = cap; c >= 1; c >>= 1)
       if (c <= lanes && lanes % c == 0) { G = c; break; }
   }
   g.G = G;
@@ -292,9 +363,9 @@ void build_table_ex(const int *counts, const int *offs, int S, long long a_limit
 // N=32 7.9 -> 7.2 us, 89k-row N=32 18.1 -> 16.8 us), 2G for wider subgroups
 // (89k-row N=128 51.4 us against 52.7 with 8G).
 void build_table(const int *counts, const int *offs, int G, int rows_scale, LaunchTable *t) {
-  const int S = kWave / G;
-  const int a_limit = G <= 16 ? std::max(8 * G, 32) : 2 * G;
-  build_table_ex(counts, offs, S, a_limit, 64ll * S, 8, 16, rows_scale, t);
+  int a_limit, c_limit;
+  csr_class_limits(G, &a_limit, &c_limit);
+  build_table_ex(counts, offs, kWave / G, a_limit, c_limit, 8, 16, rows_scale, t);
 }
 
 template <typename TB, int G, int VEC>
@@ -511,18 +582,101 @@ int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
   return SPMM_OK;
 }
 
+// Workgroups of `kernel` the chip holds at once (per-CU occupancy, cached per
+// kernel instantiation, times the CU count).
+static int resident_wgs(const void *kernel, int dev, std::atomic<int> *cache) {
+  int per_cu = cache->load(std::memory_order_relaxed);
+  if (per_cu == 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess ||
+        per_cu <= 0)
+      per_cu = 4;
+    cache->store(per_cu, std::memory_order_relaxed);
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return per_cu * cus;
+}
+
+template <typename TB, int G, int VEC>
+static void launch_dropin(int M, int col_tiles, int dev, const int *rowptr, const int *colind,
+                          const float *val, const int *perm, const TB *B, int ldb, float *C,
+                          int ldc, int *state, hipStream_t s) {
+  constexpr int SW = kWavesPerWG * (kWave / G);
+  int a_limit, c_limit;
+  csr_class_limits(G, &a_limit, &c_limit);
+  hipLaunchKernelGGL(class_list_kernel<kThreads>, dim3(ceil_div(M, kThreads)), dim3(kThreads), 0,
+                     s, M, rowptr, a_limit, c_limit, state);
+  auto *const kern = perm ? &csr_dropin_kernel<TB, G, VEC, true> : &csr_dropin_kernel<TB, G, VEC, false>;
+  static std::atomic<int> per_cu[2];
+  // listed work on as many workgroups as the chip holds at once (fewer for
+  // small M, where most of them would find nothing to take)
+  const int E = std::max(1, std::min(resident_wgs(reinterpret_cast<const void *>(kern), dev,
+                                                  &per_cu[perm ? 1 : 0]),
+                                     ceil_div(M, 16)));
+  const dim3 grid(E + ceil_div(M, SW * kDropinRounds), col_tiles);
+  hipLaunchKernelGGL(kern, grid, dim3(kThreads), 0, s, M, E, a_limit, rowptr, colind, val, perm,
+                     B, ldb, C, ldc, state);
+}
+
+template <typename TB, int VEC>
+static void dispatch_dropin(int G, int M, int col_tiles, int dev, const int *rowptr,
+                            const int *colind, const float *val, const int *perm, const TB *B,
+                            int ldb, float *C, int ldc, int *state, hipStream_t s) {
+#define SPMM_DL(g) \
+  launch_dropin<TB, g, VEC>(M, col_tiles, dev, rowptr, colind, val, perm, B, ldb, C, ldc, state, s)
+  switch (G) {
+    case 1: SPMM_DL(1); break;
+    case 2: SPMM_DL(2); break;
+    case 4: SPMM_DL(4); break;
+    case 8: SPMM_DL(8); break;
+    case 16: SPMM_DL(16); break;
+    case 32: SPMM_DL(32); break;
+    default: SPMM_DL(64); break;
+  }
+#undef SPMM_DL
+}
+
+// The plan-free form: no host analysis, no allocation after the first call
+// at a given size, no synchronisation (the narrow kernel's case aside, which
+// analyses per call). perm != nullptr: entry e's value is val[perm[e]].
 template <typename TB>
 static int run_dropin(spmm_handle_s *h, int M, int K, int N, int nnz,
                       const int *rowptr, const int *colind, const float *val,
-                      const TB *B, int ldb, float *C, int ldc, void *stream) {
-  spmm_csr_plan_s *p = nullptr;
-  int st = analyze(h, M, nnz, rowptr, 0, &p, stream);
-  if (st != SPMM_OK) return st;
-  st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, nullptr, stream);
-  // The plan's row list is read by the kernel just queued.
-  if (hipStreamSynchronize(as_stream(stream)) != hipSuccess && st == SPMM_OK) st = SPMM_ERR_HIP;
-  spmm_csr_plan_destroy(p);
-  return st;
+                      const TB *B, int ldb, float *C, int ldc, void *stream,
+                      const int *perm = nullptr) {
+  SPMM_CHECK_ARG(h != nullptr, "null handle");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0 && nnz >= 0, "negative size");
+  if (M == 0 || N == 0) return SPMM_OK;
+  SPMM_CHECK_ARG(rowptr && C, "null rowptr or C");
+  SPMM_CHECK_ARG(nnz == 0 || (colind && val && B), "null colind, val or B");
+  SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldb >= N), "ldb/ldc smaller than N");
+  hipStream_t s = as_stream(stream);
+  const Geometry g = pick_geometry(nullptr, N, sizeof(TB), B, ldb, C, ldc, nullptr);
+  if (g.G == 0) {
+    // narrow or unaligned operands: the narrow kernel on a per-call analysis
+    spmm_csr_plan_s *p = nullptr;
+    int st = analyze(h, M, nnz, rowptr, 0, &p, stream);
+    if (st != SPMM_OK) return st;
+    st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, nullptr, stream,
+                         perm);
+    // the plan's row list is read by the kernel just queued
+    if (hipStreamSynchronize(s) != hipSuccess && st == SPMM_OK) st = SPMM_ERR_HIP;
+    spmm_csr_plan_destroy(p);
+    return st;
+  }
+  int *state = nullptr;
+  const int st = list_state(h, M, s, &state);
+  if (st) return st;
+  if (g.VEC == 2)
+    dispatch_dropin<TB, 2>(g.G, M, g.col_tiles, h->device, rowptr, colind, val, perm, B, ldb, C,
+                           ldc, state, s);
+  else
+    dispatch_dropin<TB, 1>(g.G, M, g.col_tiles, h->device, rowptr, colind, val, perm, B, ldb, C,
+                           ldc, state, s);
+  SPMM_LAUNCHED();
+  return SPMM_OK;
 }
 
 }  // namespace spmm
@@ -553,17 +707,26 @@ static int run_t_planned(spmm_csrt_plan_s *p, int M, int K, int N, int nnz, cons
                          stream, p->perm);
 }
 
+// Plan-free C = A^T B: A's stable transpose built into the handle's buffers
+// on the stream (no synchronisation), then the plan-free SpMM over it with
+// values read through the permutation.
 template <typename TB>
 static int run_t_dropin(spmm_handle_s *h, int M, int K, int N, int nnz, const int *rowptr,
                         const int *colind, const float *val, const TB *B, int ldb, float *C,
                         int ldc, void *stream) {
-  spmm_csrt_plan_s *p = nullptr;
-  int st = spmm_csr_t_analyze(h, M, K, nnz, rowptr, colind, &p, stream);
-  if (st != SPMM_OK) return st;
-  st = run_t_planned<TB>(p, M, K, N, nnz, val, B, ldb, C, ldc, nullptr, stream);
-  if (hipStreamSynchronize(as_stream(stream)) != hipSuccess && st == SPMM_OK) st = SPMM_ERR_HIP;
-  spmm_csr_t_plan_destroy(p);
-  return st;
+  SPMM_CHECK_ARG(h != nullptr, "null handle");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0 && nnz >= 0, "negative size");
+  if (K == 0 || N == 0) return SPMM_OK;
+  hipStream_t s = as_stream(stream);
+  const size_t need = static_cast<size_t>(K) + 1 + 2 * static_cast<size_t>(nnz > 0 ? nnz : 1);
+  int st = grow_ints(h, &h->tstruct, &h->tstruct_cap, need, s);
+  if (st) return st;
+  int *trp = h->tstruct;
+  int *tcol = trp + K + 1;
+  int *perm = tcol + (nnz > 0 ? nnz : 1);
+  st = transpose_async(h, M, K, nnz, rowptr, colind, nullptr, trp, tcol, nullptr, perm, s);
+  if (st) return st;
+  return run_dropin<TB>(h, K, M, N, nnz, trp, tcol, val, B, ldb, C, ldc, stream, perm);
 }
 
 // The fused backward's geometry: one column tile (G lanes x VEC vectors

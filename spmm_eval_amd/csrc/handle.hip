@@ -9,28 +9,53 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
-void *workspace(spmm_handle_s *h, size_t bytes) {
+// True when `s` is being captured into a graph (growing a buffer then would
+// allocate inside the capture and leave the graph pointing at a buffer a
+// later call could outgrow).
+static bool capturing(hipStream_t s) {
+  if (!s) return false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+void *workspace(spmm_handle_s *h, size_t bytes, hipStream_t s) {
   if (bytes == 0) bytes = 256;
   if (h->ws_bytes >= bytes) return h->ws;
-  if (h->ws) {
-    // Earlier asynchronous work may still read the old buffer.
-    if (hipDeviceSynchronize() != hipSuccess) {
-      set_error("workspace: hipDeviceSynchronize failed");
-      return nullptr;
-    }
-    (void)hipFree(h->ws);
-    h->ws = nullptr;
-    h->ws_bytes = 0;
-  }
-  size_t want = bytes + bytes / 4;  // headroom against repeated growth
-  if (hipMalloc(&h->ws, want) != hipSuccess) {
-    h->ws = nullptr;
-    set_error("workspace: hipMalloc of " + std::to_string(want) +
-              " bytes failed");
+  if (capturing(s)) {
+    set_error("workspace: the handle's workspace must grow, which cannot happen during graph "
+              "capture; make one call of this size before capturing");
     return nullptr;
   }
+  size_t want = bytes + bytes / 4;  // headroom against repeated growth
+  void *p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    set_error("workspace: hipMalloc of " + std::to_string(want) + " bytes failed");
+    return nullptr;
+  }
+  if (h->ws) h->retired.push_back(h->ws);  // earlier work may still use it
+  h->ws = p;
   h->ws_bytes = want;
   return h->ws;
+}
+
+int grow_ints(spmm_handle_s *h, int **buf, size_t *cap, size_t ints, hipStream_t s) {
+  if (*cap >= ints) return SPMM_OK;
+  if (capturing(s))
+    SPMM_FAIL(SPMM_ERR_UNSUPPORTED,
+              "plan-free state must grow, which cannot happen during graph capture; make one "
+              "call at the largest size before capturing");
+  const size_t want = ints + ints / 4 + 64;
+  int *p = nullptr;
+  if (hipMalloc(&p, sizeof(int) * want) != hipSuccess)
+    SPMM_FAIL(SPMM_ERR_ALLOC, "plan-free state allocation failed");
+  if (hipMemset(p, 0, sizeof(int) * want) != hipSuccess) {
+    (void)hipFree(p);
+    SPMM_FAIL(SPMM_ERR_HIP, "plan-free state initialisation failed");
+  }
+  if (*buf) h->retired.push_back(*buf);  // earlier work may still use it
+  *buf = p;
+  *cap = want;
+  return SPMM_OK;
 }
 
 void *pinned(spmm_handle_s *h, size_t bytes) {
@@ -66,9 +91,13 @@ int spmm_create(spmm_handle_t *out) {
 
 int spmm_destroy(spmm_handle_t h) {
   if (!h) return SPMM_OK;
+  // work queued through the handle may still use its buffers
+  (void)hipDeviceSynchronize();
   if (h->ws) (void)hipFree(h->ws);
   if (h->pinned) (void)hipHostFree(h->pinned);
-  if (h->b2sr_state) (void)hipFree(h->b2sr_state);
+  if (h->list_state) (void)hipFree(h->list_state);
+  if (h->tstruct) (void)hipFree(h->tstruct);
+  for (void *p : h->retired) (void)hipFree(p);
   delete h;
   return SPMM_OK;
 }
