@@ -828,8 +828,14 @@ def main():
     g, N = hd["g"], hd["N"]
     comp = csr_compulsory_bytes(g.M, g.K, g.nnz, N)
     gath = csr_gather_bytes(g.M, g.nnz, N)
-    G, VEC, _ = hd["geo"]
-    kname = f"void spmm::csr_spmm_kernel<float, {G}, {VEC}>"
+    G, VEC, _, SL = hd["geo"]
+    tiles = N // (G * VEC * 4)
+    if SL > 1:  # tiles x SL launches per step: slice 0 stores C, the others add into it
+        kname = f"void spmm::csr_sliced_kernel<float, {G}, {VEC}, false|true>"
+        launches = tiles * SL
+    else:
+        kname = f"void spmm::csr_spmm_kernel<float, {G}, {VEC}>"
+        launches = 1
     traffic = profiled_traffic(f"csr_f32_233k_n{N}.md", kname)
     ach = comp / (hd["ms_med"] * 1e-3) / 1e9
     result = {
@@ -848,7 +854,7 @@ def main():
         "config": {"workload": "csr_spmm_f32_powerlaw_233k", "rows": g.M, "nnz": g.nnz,
                    "max_row_degree": int(g.degrees().max()), "hidden": N,
                    "plan": "row schedule analysed once, geometry autotuned",
-                   "geometry_lanes_vec_rows": list(hd["geo"]), "parallelism": "single GPU"},
+                   "geometry_lanes_vec_rows_slices": list(hd["geo"]), "parallelism": "single GPU"},
         "gflops": round(2.0 * g.nnz * N / (hd["ms_step"] * 1e-3) / 1e9, 1),
         "pct_hbm_peak": round(100 * comp / (hd["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
         "effective_GBps_gather_model": rate(gath, hd["ms_step"]),
@@ -856,7 +862,8 @@ def main():
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": f"profiles/csr_f32_233k_n{N}.md (FETCH_SIZE and WRITE_SIZE passes "
                                        "scaled by the factors measured there)" if traffic else None,
-                     "kernel": kname, "bytes_per_launch": comp,
+                     "kernel": kname, "kernel_launches_per_step": launches,
+                     "bytes_per_launch": comp,
                      "bytes_model": "compulsory: rowptr + colind + val + B + C, each once",
                      "mean_launch_ms": round(hd["ms_med"], 5),
                      "launch_ms_source": "median of 5 batches of back-to-back launches, events on the "

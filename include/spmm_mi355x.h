@@ -53,11 +53,22 @@ typedef struct spmm_batch_plan_s *spmm_batch_plan_t;
 /* Launch geometry override (the autotuner's knobs). Zero fields = default.
  *   lanes_per_row : lanes that share one output row (power of two, 4..64)
  *   vec           : 16-byte vectors of B loaded per lane per gather (1 or 2)
- *   rows_scale    : multiplier on rows handled per subgroup for short rows */
+ *   rows_scale    : multiplier on rows handled per subgroup for short rows
+ *   slices        : CSR SpMM only: 2, 4, 8 or 16 runs the product as that
+ *                   many launches per column tile, launch s taking the
+ *                   entries whose column lies in the s-th equal range of B's
+ *                   rows (so one launch gathers from a slice of B small
+ *                   enough to stay in an XCD's L2), each adding its partial
+ *                   sums into C in slice order; 0 or 1 = one launch. Needs
+ *                   column indices ascending within each row; the plan keeps
+ *                   per-row slice offsets (built on first use, which
+ *                   allocates and synchronises), so a plan used with slices
+ *                   is valid only for the colind it saw. */
 typedef struct spmm_tune_s {
   int lanes_per_row;
   int vec;
   int rows_scale;
+  int slices;
 } spmm_tune_t;
 
 /* ---------------------------------------------------------------- handle */

@@ -33,7 +33,8 @@ class SpmmError(RuntimeError):
 
 
 class Tune(C.Structure):
-    _fields_ = [("lanes_per_row", C.c_int), ("vec", C.c_int), ("rows_scale", C.c_int)]
+    _fields_ = [("lanes_per_row", C.c_int), ("vec", C.c_int), ("rows_scale", C.c_int),
+                ("slices", C.c_int)]
 
 
 _vp, _i, _ip = C.c_void_p, C.c_int, C.POINTER(C.c_int)

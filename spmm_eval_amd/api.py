@@ -75,14 +75,14 @@ class CsrPlan:
     def autotune(self, rowptr, colind, val, B, C_out, K: int | None = None, stream=None) -> tuple:
         """Times the valid launch geometries on these operands, keeps the
         fastest in the plan for (N, dtype) and returns it as
-        (lanes_per_row, vec, rows_scale)."""
+        (lanes_per_row, vec, rows_scale, slices)."""
         K = B.shape[0] if K is None else K
         N = B.shape[1]
         best = Tune()
         call("spmm_csr_autotune", self.p, self.M, K, N, self.nnz, _p(rowptr), _p(colind), _p(val),
              _p(B), int(_is_bf16(B)), B.stride(0), _p(C_out), C_out.stride(0), C.byref(best),
              _stream(stream))
-        return (best.lanes_per_row, best.vec, best.rows_scale)
+        return (best.lanes_per_row, best.vec, best.rows_scale, best.slices)
 
     def bins(self) -> list[int]:
         arr = (C.c_int * 32)()

@@ -107,6 +107,28 @@ __device__ __forceinline__ void store_row(float *__restrict__ Crow, const float 
   }
 }
 
+// C row segment += acc (the partial sums of one column slice added to those
+// of the slices before it; same lane layout as store_row).
+template <int G, int E, int EPL>
+__device__ __forceinline__ void add_row(float *__restrict__ Crow, const float (&acc)[EPL]) {
+#pragma unroll
+  for (int q = 0; q < EPL / E; ++q) {
+#pragma unroll
+    for (int k = 0; k < E; k += 4) {
+      float4 *p = reinterpret_cast<float4 *>(Crow + q * G * E + k);
+      const float4 c = *p;
+      *p = make_float4(c.x + acc[q * E + k], c.y + acc[q * E + k + 1], c.z + acc[q * E + k + 2],
+                       c.w + acc[q * E + k + 3]);
+    }
+  }
+}
+
+template <int G, int E, int EPL, bool kAcc>
+__device__ __forceinline__ void put_row(float *__restrict__ Crow, const float (&acc)[EPL]) {
+  if constexpr (kAcc) add_row<G, E, EPL>(Crow, acc);
+  else store_row<G, E, EPL>(Crow, acc);
+}
+
 // Butterfly over the S subgroups of a wave: every subgroup ends with the
 // same sum (IEEE addition is commutative, so partners agree bitwise).
 template <int G, int EPL>
@@ -179,7 +201,7 @@ class_list_kernel(int n, const int *__restrict__ ptr, int a_limit, int c_limit,
 // One row of each class (the per-row code of the planned kernel and of the
 // plan-free kernel, so both give the same bits at the same lane geometry).
 // Class A: one subgroup of G lanes sums the row and stores it.
-template <typename TB, int G, int VEC, bool kPerm>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false>
 __device__ __forceinline__ void csr_row_sub(int row, int2 sp, int lane_sub, int col0,
                                             const int *__restrict__ colind,
                                             const float *__restrict__ val,
@@ -193,13 +215,13 @@ __device__ __forceinline__ void csr_row_sub(int row, int2 sp, int lane_sub, int 
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
   accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, 0, G, lane_sub, colind, val, Bc, ldb, acc,
                                       perm);
-  store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+  put_row<G, E, EPL, kAcc>(C + static_cast<size_t>(row) * ldc + col0, acc);
 }
 
 // Class B: the S subgroups of one wave split the row's chunks of G entries
 // (subgroup s takes chunks s, s + S, ...), then a butterfly; subgroup 0
 // stores.
-template <typename TB, int G, int VEC, bool kPerm>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false>
 __device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane_sub, int col0,
                                              const int *__restrict__ colind,
                                              const float *__restrict__ val,
@@ -215,13 +237,13 @@ __device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane
   accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, sub * G, S * G, lane_sub, colind, val, Bc, ldb,
                                       acc, perm);
   wave_reduce_subgroups<G, EPL>(acc);
-  if (sub == 0) store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+  if (sub == 0) put_row<G, E, EPL, kAcc>(C + static_cast<size_t>(row) * ldc + col0, acc);
 }
 
 // Class C: the 4 S subgroups of one workgroup split the row, the waves meet
 // in LDS in a fixed order; wave 0's subgroup 0 stores. Every thread of the
 // workgroup must call it (two barriers).
-template <typename TB, int G, int VEC, bool kPerm>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false>
 __device__ __forceinline__ void csr_row_wg(int row, int2 sp, int wave, int sub, int lane_sub,
                                            int col0, const int *__restrict__ colind,
                                            const float *__restrict__ val,
@@ -251,7 +273,7 @@ __device__ __forceinline__ void csr_row_wg(int row, int2 sp, int wave, int sub, 
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[q] += red[w][lane_sub * EPL + q];
     }
-    store_row<G, E, EPL>(C + static_cast<size_t>(row) * ldc + col0, acc);
+    put_row<G, E, EPL, kAcc>(C + static_cast<size_t>(row) * ldc + col0, acc);
   }
   __syncthreads();  // red[] may be reused by the next row
 }
@@ -747,6 +769,7 @@ struct Geometry {
   int VEC = 1;  // 16-byte vectors per lane
   int col_tiles = 1;
   int rows_scale = 1;
+  int slices = 1;  // launches per column tile over ranges of B's rows
 };
 
 constexpr int kTuneSlots = 8;
@@ -808,4 +831,12 @@ struct spmm_csr_plan_s {
   int ntuned = 0;
   spmm::TuneEntry tuned[spmm::kTuneSlots];
   int tile = 0;  // 0: CSR plan; T: B2SR plan (rows binned by block-row tile count)
+  // column-sliced launches: per (slices S, B rows K), the entry offsets
+  // off[s * M + i] (s = 0..S) where the s-th range of B's rows starts in the
+  // row at plan position i (the rows' entries are ascending in column)
+  struct SliceSet {
+    int S = 0, K = 0;
+    int *d_off = nullptr;
+  } slice_sets[4];
+  int sorted = -1;  // colind ascending within every row: -1 not checked yet
 };

@@ -173,6 +173,102 @@ csr_dropin_kernel(int M, int E, int a_limit, const int *__restrict__ rowptr,
   }
 }
 
+// ------------------------------------------------- column-sliced launches
+// Per-row slice offsets of a plan: thread i walks the row at plan position i
+// once and records where each range of W columns starts (off[s * M + i] for
+// s = 1 .. S-1; off[0 * M + i] and off[S * M + i] are the row's span), and
+// flags a column index below its predecessor (the offsets need ascending
+// columns within a row).
+__global__ void __launch_bounds__(kThreads)
+slice_offsets_kernel(int M, int S, int W, const int2 *__restrict__ span,
+                     const int *__restrict__ colind, int *__restrict__ off,
+                     int *__restrict__ unsorted) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < M; i += gridDim.x * kThreads) {
+    const int2 sp = span[i];
+    off[i] = sp.x;
+    int s = 1, prev = -1;
+    bool bad = false;
+    for (int e = sp.x; e < sp.y; ++e) {
+      const int c = colind[e];
+      bad |= c < prev;
+      prev = c;
+      while (s < S && static_cast<long long>(c) >= static_cast<long long>(s) * W) {
+        off[static_cast<size_t>(s) * M + i] = e;
+        ++s;
+      }
+    }
+    for (; s < S; ++s) off[static_cast<size_t>(s) * M + i] = sp.y;
+    off[static_cast<size_t>(S) * M + i] = sp.y;
+    if (bad) atomicOr(unsorted, 1);
+  }
+}
+
+// One launch of a column-sliced product: for column tile `ctile`, the
+// entries of each row whose column lies in one range of B's rows
+// ([lo[i], hi[i]) at plan position i), in the row classes of a launch table
+// built on the rows' degrees divided by the slice count. kAcc = false (the
+// first slice) stores C, rows with no entry in it storing zeros; kAcc = true
+// adds the slice's partial sums into C, so a row's result is
+// ((p_0 + p_1) + p_2) + ... in slice order, the same at every call. Launches
+// run in slice order on one stream, so no two of them touch C at once.
+template <typename TB, int G, int VEC, bool kAcc>
+__global__ void __launch_bounds__(kThreads)
+csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
+                  const int *__restrict__ lo, const int *__restrict__ hi,
+                  const int *__restrict__ colind, const float *__restrict__ val,
+                  const TB *__restrict__ B, int ldb, float *__restrict__ C, int ldc) {
+  constexpr int E = Vec16<TB>::kElems;
+  constexpr int S = kWave / G;
+  constexpr int CW = G * VEC * E;
+  const Seg sg = find_seg(tab, blockIdx.x);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1), sub = lane / G;
+  const int col0 = ctile * CW + lane_sub * E;
+  const TB *Bc = B + col0;
+  const int local_wg = blockIdx.x - sg.wg_begin;
+  if (sg.kind == kSegA) {
+    const int R = sg.rows_per_sub;
+    const int subs_per_wg = kWavesPerWG * S;
+    const int end = sg.list_off + sg.count;
+    int pos = sg.list_off + local_wg * subs_per_wg * R + wave * S + sub;
+    int row = -1;
+    int2 sp = make_int2(0, 0);
+    if (pos < end) {
+      row = rows[pos];
+      sp = make_int2(lo[pos], hi[pos]);
+    }
+    for (int k = 0; k < R && row >= 0; ++k) {
+      // the next row's id and range are requested before this row's gathers
+      const int pos_n = pos + subs_per_wg;
+      int row_n = -1;
+      int2 sp_n = make_int2(0, 0);
+      if (k + 1 < R && pos_n < end) {
+        row_n = rows[pos_n];
+        sp_n = make_int2(lo[pos_n], hi[pos_n]);
+      }
+      if (!kAcc || sp.x < sp.y)
+        csr_row_sub<TB, G, VEC, false, kAcc>(row, sp, lane_sub, col0, colind, val, Bc, ldb, C,
+                                             ldc, nullptr);
+      row = row_n;
+      sp = sp_n;
+      pos = pos_n;
+    }
+  } else if (sg.kind == kSegB) {
+    const int pos = sg.list_off + local_wg * kWavesPerWG + wave;
+    if (pos >= sg.list_off + sg.count) return;  // whole wave
+    const int2 sp = make_int2(lo[pos], hi[pos]);
+    if (kAcc && sp.x == sp.y) return;
+    csr_row_wave<TB, G, VEC, false, kAcc>(rows[pos], sp, sub, lane_sub, col0, colind, val, Bc,
+                                          ldb, C, ldc, nullptr);
+  } else {
+    const int pos = sg.list_off + local_wg;
+    const int2 sp = make_int2(lo[pos], hi[pos]);
+    if (kAcc && sp.x == sp.y) return;  // whole workgroup
+    csr_row_wg<TB, G, VEC, false, kAcc>(rows[pos], sp, wave, sub, lane_sub, col0, colind, val, Bc,
+                                        ldb, C, ldc, nullptr);
+  }
+}
+
 // Narrow or unaligned N (N not a multiple of one 16-byte vector, or B / C /
 // ld not 16-byte aligned): a row's entries are split across lanes rather
 // than its columns. A subgroup is NC column lanes (a power of two covering
@@ -318,6 +414,8 @@ static Geometry pick_geometry(const spmm_csr_plan_s *p, int N, int elem_bytes, c
   g.G = G;
   g.VEC = vec;
   g.col_tiles = N / (G * e16 * vec);
+  const int sl = t.slices;
+  g.slices = (sl == 2 || sl == 4 || sl == 8 || sl == 16) && G >= 4 && G <= 32 ? sl : 1;
   return g;
 }
 
@@ -366,6 +464,109 @@ void build_table(const int *counts, const int *offs, int G, int rows_scale, Laun
   int a_limit, c_limit;
   csr_class_limits(G, &a_limit, &c_limit);
   build_table_ex(counts, offs, kWave / G, a_limit, c_limit, 8, 16, rows_scale, t);
+}
+
+// Launch table of one slice of an S-sliced product: the classes and rows
+// per subgroup of build_table, on degrees divided by S (a slice holds about
+// 1/S of each row's entries).
+static void build_table_sliced(const int *counts, const int *offs, int G, int rows_scale, int S,
+                               LaunchTable *t) {
+  int a_limit, c_limit;
+  csr_class_limits(G, &a_limit, &c_limit);
+  build_table_ex(counts, offs, kWave / G, static_cast<long long>(a_limit) * S,
+                 static_cast<long long>(c_limit) * S, 8 * S, 16, rows_scale, t);
+}
+
+// The plan's slice offsets for (S, K), built on first use: one allocation,
+// one kernel and a read-back of the sortedness flag (synchronises). Up to
+// four (S, K) sets are kept; a fifth replaces the last. SPMM_ERR_UNSUPPORTED
+// when colind is not ascending within its rows.
+static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hipStream_t s,
+                         const int **out) {
+  auto &sets = p->slice_sets;
+  for (auto &ss : sets)
+    if (ss.d_off && ss.S == S && ss.K == K) { *out = ss.d_off; return SPMM_OK; }
+  if (p->sorted == 0) SPMM_FAIL(SPMM_ERR_UNSUPPORTED, "column slices need ascending colind in every row");
+  int slot = 3;
+  for (int i = 0; i < 4; ++i)
+    if (!sets[i].d_off) { slot = i; break; }
+  if (sets[slot].d_off) {
+    SPMM_HIP(hipStreamSynchronize(s));  // launches queued earlier may read it
+    (void)hipFree(sets[slot].d_off);
+    sets[slot] = {};
+  }
+  const size_t n = (static_cast<size_t>(S) + 1) * static_cast<size_t>(p->M);
+  int *d = nullptr;
+  if (hipMalloc(&d, sizeof(int) * (n + 1)) != hipSuccess)
+    SPMM_FAIL(SPMM_ERR_ALLOC, "hipMalloc of the slice offsets failed");
+  int *flag = d + n;
+  int unsorted = 0;
+  const int W = ceil_div(K, S);
+  hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(slice_offsets_kernel, dim3(std::min(ceil_div(p->M, kThreads), 4096)),
+                       dim3(kThreads), 0, s, p->M, S, W, p->d_span, colind, d, flag);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&unsorted, flag, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    SPMM_FAIL(SPMM_ERR_HIP, std::string("slice offsets: ") + hipGetErrorString(e));
+  }
+  p->sorted = unsorted ? 0 : 1;
+  if (unsorted) {
+    (void)hipFree(d);
+    SPMM_FAIL(SPMM_ERR_UNSUPPORTED, "column slices need ascending colind in every row");
+  }
+  sets[slot].S = S;
+  sets[slot].K = K;
+  sets[slot].d_off = d;
+  *out = d;
+  return SPMM_OK;
+}
+
+template <typename TB, int G, int VEC>
+static void launch_sliced(const LaunchTable &t, int col_tiles, int S, const spmm_csr_plan_s *p,
+                          const int *off, const int *colind, const float *val, const TB *B,
+                          int ldb, float *C, int ldc, hipStream_t s) {
+  const size_t M = static_cast<size_t>(p->M);
+  for (int ct = 0; ct < col_tiles; ++ct)
+    for (int sl = 0; sl < S; ++sl) {
+      const int *lo = off + sl * M, *hi = off + (sl + 1) * M;
+      if (sl == 0)
+        hipLaunchKernelGGL((csr_sliced_kernel<TB, G, VEC, false>), dim3(t.total_wg),
+                           dim3(kThreads), 0, s, t, ct, p->d_rows, lo, hi, colind, val, B, ldb,
+                           C, ldc);
+      else
+        hipLaunchKernelGGL((csr_sliced_kernel<TB, G, VEC, true>), dim3(t.total_wg),
+                           dim3(kThreads), 0, s, t, ct, p->d_rows, lo, hi, colind, val, B, ldb,
+                           C, ldc);
+    }
+}
+
+template <typename TB>
+static void dispatch_sliced(const Geometry &g, const LaunchTable &t, const spmm_csr_plan_s *p,
+                            const int *off, const int *colind, const float *val, const TB *B,
+                            int ldb, float *C, int ldc, hipStream_t s) {
+#define SPMM_SL(gg, vv) \
+  launch_sliced<TB, gg, vv>(t, g.col_tiles, g.slices, p, off, colind, val, B, ldb, C, ldc, s)
+  if (g.VEC == 2) {
+    switch (g.G) {
+      case 4: SPMM_SL(4, 2); break;
+      case 8: SPMM_SL(8, 2); break;
+      case 16: SPMM_SL(16, 2); break;
+      default: SPMM_SL(32, 2); break;
+    }
+  } else {
+    switch (g.G) {
+      case 4: SPMM_SL(4, 1); break;
+      case 8: SPMM_SL(8, 1); break;
+      case 16: SPMM_SL(16, 1); break;
+      default: SPMM_SL(32, 1); break;
+    }
+  }
+#undef SPMM_SL
 }
 
 template <typename TB, int G, int VEC>
@@ -458,6 +659,16 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
     return SPMM_OK;
   }
   LaunchTable t;
+  if (g.slices > 1 && perm == nullptr && nnz > 0) {
+    const int *off = nullptr;
+    const int st = slice_offsets(p, g.slices, K, colind, s, &off);
+    if (st) return st;
+    build_table_sliced(p->counts, p->offs, g.G, g.rows_scale, g.slices, &t);
+    if (t.total_wg == 0) return SPMM_OK;
+    dispatch_sliced<TB>(g, t, p, off, colind, val, B, ldb, C, ldc, s);
+    SPMM_LAUNCHED();
+    return SPMM_OK;
+  }
   build_table(p->counts, p->offs, g.G, g.rows_scale, &t);
   if (t.total_wg == 0) return SPMM_OK;
   if (g.VEC == 2)
@@ -468,8 +679,9 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
   return SPMM_OK;
 }
 
-// Times every valid (lanes per row, vector width, rows per subgroup) on the
-// caller's operands and records the fastest in the plan for (N, dtype).
+// Times every valid (lanes per row, vector width, rows per subgroup, column
+// slices) on the caller's operands and records the fastest in the plan for
+// (N, dtype).
 template <typename TB>
 static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int *rowptr,
                     const int *colind, const float *val, const TB *B, int ldb, float *C,
@@ -477,8 +689,9 @@ static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int 
   SPMM_CHECK_ARG(p != nullptr, "plan is null");
   hipStream_t s = as_stream(stream);
   const int e16 = 16 / static_cast<int>(sizeof(TB));
-  spmm_tune_t best = {0, 0, 0};
+  spmm_tune_t best = {0, 0, 0, 0};
   float best_ms = 1e30f;
+  bool slicing = perm == nullptr && nnz > 0 && p->sorted != 0;
   hipEvent_t e0, e1;
   SPMM_HIP(hipEventCreate(&e0));
   SPMM_HIP(hipEventCreate(&e1));
@@ -488,10 +701,21 @@ static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int 
     const int lanes = N / (e16 * vec);
     for (int G = 64; G >= 1 && st == SPMM_OK; G >>= 1) {
       if (!tune_valid(N, e16, G, vec) || G * 4 < std::min(lanes, 64)) continue;  // <= 4 column tiles
-      for (int rs = 1; rs <= 8 && st == SPMM_OK; rs <<= 1) {
-        spmm_tune_t cand = {G, vec, rs};
+      for (int cand_i = 0; cand_i < 4 * 5 && st == SPMM_OK; ++cand_i) {
+        const int rs = 1 << (cand_i % 4), sl = cand_i < 4 ? 0 : 1 << (cand_i / 4);
+        // column slices: only where one tile of B outgrows an XCD's L2, only
+        // on a plan that can hold them, at rows_scale 1 and 2
+        if (sl && (!slicing || G < 4 || G > 32 || rs > 2 ||
+                   static_cast<long long>(K) * G * vec * 16 < (4ll << 20)))
+          continue;
+        spmm_tune_t cand = {G, vec, rs, sl};
         st = run_planned<TB>(p, M, K, N, nnz, rowptr, colind, val, B, ldb, C, ldc, &cand, stream,
                              perm);
+        if (sl && st == SPMM_ERR_UNSUPPORTED) {  // colind not ascending: no slices
+          slicing = false;
+          st = SPMM_OK;
+          continue;
+        }
         if (st) break;
         if (hipEventRecord(e0, s) != hipSuccess) { st = SPMM_ERR_HIP; break; }
         for (int r = 0; r < 3 && st == SPMM_OK; ++r)
@@ -925,6 +1149,8 @@ int spmm_csr_plan_destroy(spmm_csr_plan_t plan) {
   if (!plan) return SPMM_OK;
   if (plan->d_rows) (void)hipFree(plan->d_rows);
   if (plan->d_span) (void)hipFree(plan->d_span);
+  for (auto &ss : plan->slice_sets)
+    if (ss.d_off) (void)hipFree(ss.d_off);
   delete plan;
   return SPMM_OK;
 }

@@ -221,11 +221,97 @@ def test_autotune_picks_a_valid_geometry(handle, dev, bf16):
         Bd = api.to_bf16_bits(Bd)
     plan = api.CsrPlan(handle, rp, g.nnz)
     C = torch.empty((g.M, N), device=dev)
-    G, vec, rs = plan.autotune(rp, ci, v, Bd, C)
-    assert G in (4, 8, 16, 32, 64) and vec in (1, 2) and rs in (1, 2, 4, 8)
+    G, vec, rs, sl = plan.autotune(rp, ci, v, Bd, C)
+    assert G in (4, 8, 16, 32, 64) and vec in (1, 2) and rs in (1, 2, 4, 8) and sl in (0, 2, 4, 8, 16)
     C1 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan)            # uses the tuned entry
-    C2 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=(G, vec, rs))
+    C2 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=(G, vec, rs, sl))
     assert torch.equal(C1, C2)
     ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B) if not bf16 else \
         oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, Bd.view(torch.int16).cpu().numpy().view(np.uint16))
     assert rel_fro(C1.cpu().numpy(), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("S", [2, 4, 8, 16])
+@pytest.mark.parametrize("tune", [(8, 1, 1), (16, 1, 2), (4, 2, 1), (32, 1, 1)])
+def test_csr_column_slices(handle, dev, S, tune):
+    """Column-sliced launches (tune.slices = S): the entries of each row split
+    by ranges of B's rows, one launch per (column tile, slice), partial sums
+    added into C in slice order. FP32 within the oracle's bounds, +-1 B exact,
+    two calls bitwise equal; BF16 within 1e-5 of FP64 of the rounded B."""
+    g = graphs.chung_lu(12000, 600000, seed=S, max_deg=5000)
+    g.val = graphs.row_normalised_uniform(g, S)
+    K = g.K
+    N = 128
+    B = graphs.uniform_pm1((K, N), seed=S + 1)
+    rp, ci, v = to_dev(g, dev)
+    Bd = torch.from_numpy(B).to(dev)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    t = tune + (S,)
+    C1 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=t)
+    C2 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=t)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    check_f32(g, B, C1.cpu().numpy())
+    Sg = np.random.default_rng(S).choice([-1, 1], size=(K, N)).astype(np.int32)
+    ones = torch.ones(g.nnz, device=dev)
+    Cs = api.spmm_csr(handle, rp, ci, ones, torch.from_numpy(Sg.astype(np.float32)).to(dev), plan=plan,
+                      tune=t)
+    np.testing.assert_array_equal(Cs.cpu().numpy(), oracle.csr_spmm_i64(g.rowptr, g.colind, Sg).astype(np.float32))
+    Bh = api.to_bf16_bits(Bd)
+    Ch = api.spmm_csr(handle, rp, ci, v, Bh, plan=plan, tune=t).cpu().numpy()
+    refq = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, Bh.view(torch.int16).cpu().numpy().view(np.uint16))
+    assert rel_fro(Ch, refq) <= 1e-5
+    plan.close()
+
+
+def test_csr_column_slices_edge_shapes(handle, dev):
+    """Empty rows, rows wholly inside one slice, K not a multiple of S, a
+    second K on the same plan (a new offset set), N with one column tile."""
+    g = graphs.random_csr(700, 5003, 0.01, seed=4, long_rows=3, long_len=5003)
+    rp, ci, v = to_dev(g, dev)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    for N, tune in ((32, (8, 1, 1, 16)), (64, (16, 1, 1, 4)), (128, (8, 1, 2, 8))):
+        B = graphs.uniform_pm1((g.K, N), seed=N)
+        C = api.spmm_csr(handle, rp, ci, v, torch.from_numpy(B).to(dev), plan=plan, tune=tune)
+        check_f32(g, B, C.cpu().numpy())
+    # the same structure as A[:, :4000] against a B of 4000 rows (columns >= 4000 dropped)
+    keep = g.colind < 4000
+    rows = np.repeat(np.arange(g.M), g.degrees())[keep]
+    rp2 = np.zeros(g.M + 1, np.int32)
+    rp2[1:] = np.cumsum(np.bincount(rows, minlength=g.M))
+    g2 = graphs.Csr(rp2, g.colind[keep].copy(), g.val[keep].copy(), g.M, 4000)
+    rpd, cid, vd = to_dev(g2, dev)
+    plan2 = api.CsrPlan(handle, rpd, g2.nnz)
+    B = graphs.uniform_pm1((4000, 64), seed=9)
+    for S in (4, 8):
+        C = api.spmm_csr(handle, rpd, cid, vd, torch.from_numpy(B).to(dev), plan=plan2, tune=(8, 1, 1, S))
+        check_f32(g2, B, C.cpu().numpy())
+
+
+def test_csr_column_slices_need_sorted_rows(handle, dev):
+    g = graphs.random_csr(300, 3000, 0.02, seed=6)
+    ci = g.colind.copy()
+    r = int(np.argmax(g.degrees()))
+    s, e = g.rowptr[r], g.rowptr[r + 1]
+    ci[s:e] = ci[s:e][::-1]  # one row with descending columns
+    gu = graphs.Csr(g.rowptr, ci, g.val, g.M, g.K)
+    rp, cid, v = to_dev(gu, dev)
+    B = torch.from_numpy(graphs.uniform_pm1((g.K, 64), seed=1)).to(dev)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    with pytest.raises(SpmmError) as ei:
+        api.spmm_csr(handle, rp, cid, v, B, plan=plan, tune=(8, 1, 1, 4))
+    assert ei.value.status == 2
+    C = api.spmm_csr(handle, rp, cid, v, B, plan=plan, tune=(8, 1, 1))  # unsliced still runs
+    check_f32(gu, B.cpu().numpy(), C.cpu().numpy())
+    big = graphs.random_csr(2000, 40000, 0.002, seed=7)
+    ci2 = big.colind.copy()
+    r = int(np.argmax(big.degrees()))
+    s, e = big.rowptr[r], big.rowptr[r + 1]
+    ci2[s:e] = ci2[s:e][::-1]
+    gb = graphs.Csr(big.rowptr, ci2, big.val, big.M, big.K)
+    rp, cid, v = to_dev(gb, dev)
+    B = torch.from_numpy(graphs.uniform_pm1((big.K, 128), seed=2)).to(dev)
+    plan = api.CsrPlan(handle, rp, gb.nnz)
+    C = torch.empty((gb.M, 128), device=dev)
+    G, vec, rs, sl = plan.autotune(rp, cid, v, B, C)  # B is 20 MB: slices would be tried
+    assert sl == 0
