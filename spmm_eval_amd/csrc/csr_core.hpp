@@ -29,11 +29,19 @@ struct LaunchTable {
   Seg seg[kBins];
 };
 
+// Loads of a stream read once (kNT: the non-temporal policy, so the lines do
+// not push longer-lived data -- a slice of B -- out of the caches).
+template <bool kNT, typename T>
+__device__ __forceinline__ T ld_once(const T *p) {
+  if constexpr (kNT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // One subgroup (G lanes) accumulates entries [e_begin, e_end) taking chunks
 // of G entries at stride `chunk_stride`, starting at chunk `first`. With
 // kPerm the value of entry e is val[perm[e]] (a transposed structure whose
 // values stay in the source order, so learnable values need no re-gather).
-template <typename TB, int G, int EPL, bool kPerm = false>
+template <typename TB, int G, int EPL, bool kPerm = false, bool kNT = false>
 __device__ __forceinline__ void accumulate_range(
     int e_begin, int e_end, int first_chunk_off, int chunk_stride, int lane_sub,
     const int *__restrict__ colind, const float *__restrict__ val,
@@ -46,8 +54,8 @@ __device__ __forceinline__ void accumulate_range(
   int cidx = 0;
   float v = 0.f;
   if (e + lane_sub < e_end) {
-    cidx = colind[e + lane_sub];
-    v = val[kPerm ? perm[e + lane_sub] : e + lane_sub];
+    cidx = ld_once<kNT>(colind + e + lane_sub);
+    v = ld_once<kNT>(val + (kPerm ? perm[e + lane_sub] : e + lane_sub));
   }
   for (; e < e_end; e += chunk_stride) {
     const int n = min(G, e_end - e);
@@ -56,8 +64,8 @@ __device__ __forceinline__ void accumulate_range(
     int cidx_n = 0;
     float v_n = 0.f;
     if (my_n < e_end) {
-      cidx_n = colind[my_n];
-      v_n = val[kPerm ? perm[my_n] : my_n];
+      cidx_n = ld_once<kNT>(colind + my_n);
+      v_n = ld_once<kNT>(val + (kPerm ? perm[my_n] : my_n));
     }
     for (int k = 0; k < n; k += U) {
       int cc[U];
@@ -109,23 +117,46 @@ __device__ __forceinline__ void store_row(float *__restrict__ Crow, const float 
 
 // C row segment += acc (the partial sums of one column slice added to those
 // of the slices before it; same lane layout as store_row).
-template <int G, int E, int EPL>
+// kNT: C is streamed (read and written once per launch) with the
+// non-temporal policy.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int G, int E, int EPL, bool kNT>
 __device__ __forceinline__ void add_row(float *__restrict__ Crow, const float (&acc)[EPL]) {
 #pragma unroll
   for (int q = 0; q < EPL / E; ++q) {
 #pragma unroll
     for (int k = 0; k < E; k += 4) {
-      float4 *p = reinterpret_cast<float4 *>(Crow + q * G * E + k);
-      const float4 c = *p;
-      *p = make_float4(c.x + acc[q * E + k], c.y + acc[q * E + k + 1], c.z + acc[q * E + k + 2],
-                       c.w + acc[q * E + k + 3]);
+      f32x4 *p = reinterpret_cast<f32x4 *>(Crow + q * G * E + k);
+      f32x4 c = ld_once<kNT>(p);
+      c.x += acc[q * E + k];
+      c.y += acc[q * E + k + 1];
+      c.z += acc[q * E + k + 2];
+      c.w += acc[q * E + k + 3];
+      if constexpr (kNT) __builtin_nontemporal_store(c, p);
+      else *p = c;
     }
   }
 }
 
-template <int G, int E, int EPL, bool kAcc>
+template <int G, int E, int EPL, bool kNT>
+__device__ __forceinline__ void store_row_nt(float *__restrict__ Crow, const float (&acc)[EPL]) {
+#pragma unroll
+  for (int q = 0; q < EPL / E; ++q) {
+#pragma unroll
+    for (int k = 0; k < E; k += 4) {
+      const f32x4 c = {acc[q * E + k], acc[q * E + k + 1], acc[q * E + k + 2], acc[q * E + k + 3]};
+      f32x4 *p = reinterpret_cast<f32x4 *>(Crow + q * G * E + k);
+      if constexpr (kNT) __builtin_nontemporal_store(c, p);
+      else *p = c;
+    }
+  }
+}
+
+template <int G, int E, int EPL, bool kAcc, bool kNT = false>
 __device__ __forceinline__ void put_row(float *__restrict__ Crow, const float (&acc)[EPL]) {
-  if constexpr (kAcc) add_row<G, E, EPL>(Crow, acc);
+  if constexpr (kAcc) add_row<G, E, EPL, kNT>(Crow, acc);
+  else if constexpr (kNT) store_row_nt<G, E, EPL, true>(Crow, acc);
   else store_row<G, E, EPL>(Crow, acc);
 }
 
@@ -201,7 +232,7 @@ class_list_kernel(int n, const int *__restrict__ ptr, int a_limit, int c_limit,
 // One row of each class (the per-row code of the planned kernel and of the
 // plan-free kernel, so both give the same bits at the same lane geometry).
 // Class A: one subgroup of G lanes sums the row and stores it.
-template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false>
 __device__ __forceinline__ void csr_row_sub(int row, int2 sp, int lane_sub, int col0,
                                             const int *__restrict__ colind,
                                             const float *__restrict__ val,
@@ -213,15 +244,15 @@ __device__ __forceinline__ void csr_row_sub(int row, int2 sp, int lane_sub, int 
   float acc[EPL];
 #pragma unroll
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-  accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, 0, G, lane_sub, colind, val, Bc, ldb, acc,
-                                      perm);
+  accumulate_range<TB, G, EPL, kPerm, kNT>(sp.x, sp.y, 0, G, lane_sub, colind, val, Bc, ldb, acc,
+                                           perm);
   put_row<G, E, EPL, kAcc>(C + static_cast<size_t>(row) * ldc + col0, acc);
 }
 
 // Class B: the S subgroups of one wave split the row's chunks of G entries
 // (subgroup s takes chunks s, s + S, ...), then a butterfly; subgroup 0
 // stores.
-template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false>
 __device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane_sub, int col0,
                                              const int *__restrict__ colind,
                                              const float *__restrict__ val,
@@ -234,8 +265,8 @@ __device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane
   float acc[EPL];
 #pragma unroll
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-  accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, sub * G, S * G, lane_sub, colind, val, Bc, ldb,
-                                      acc, perm);
+  accumulate_range<TB, G, EPL, kPerm, kNT>(sp.x, sp.y, sub * G, S * G, lane_sub, colind, val, Bc,
+                                           ldb, acc, perm);
   wave_reduce_subgroups<G, EPL>(acc);
   if (sub == 0) put_row<G, E, EPL, kAcc>(C + static_cast<size_t>(row) * ldc + col0, acc);
 }
@@ -243,7 +274,7 @@ __device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane
 // Class C: the 4 S subgroups of one workgroup split the row, the waves meet
 // in LDS in a fixed order; wave 0's subgroup 0 stores. Every thread of the
 // workgroup must call it (two barriers).
-template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false>
 __device__ __forceinline__ void csr_row_wg(int row, int2 sp, int wave, int sub, int lane_sub,
                                            int col0, const int *__restrict__ colind,
                                            const float *__restrict__ val,
@@ -259,8 +290,8 @@ __device__ __forceinline__ void csr_row_wg(int row, int2 sp, int wave, int sub, 
 #pragma unroll
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
   const int gsub = wave * S + sub;
-  accumulate_range<TB, G, EPL, kPerm>(sp.x, sp.y, gsub * G, kWavesPerWG * S * G, lane_sub, colind,
-                                      val, Bc, ldb, acc, perm);
+  accumulate_range<TB, G, EPL, kPerm, kNT>(sp.x, sp.y, gsub * G, kWavesPerWG * S * G, lane_sub,
+                                           colind, val, Bc, ldb, acc, perm);
   wave_reduce_subgroups<G, EPL>(acc);
   if (wave > 0 && sub == 0) {
 #pragma unroll
@@ -831,12 +862,13 @@ struct spmm_csr_plan_s {
   int ntuned = 0;
   spmm::TuneEntry tuned[spmm::kTuneSlots];
   int tile = 0;  // 0: CSR plan; T: B2SR plan (rows binned by block-row tile count)
-  // column-sliced launches: per (slices S, B rows K), the entry offsets
+  // column-sliced launches: per (slices S, B rows K) -- one set for each
+  // slice count the autotuner tries --, the entry offsets
   // off[s * M + i] (s = 0..S) where the s-th range of B's rows starts in the
   // row at plan position i (the rows' entries are ascending in column)
   struct SliceSet {
     int S = 0, K = 0;
     int *d_off = nullptr;
-  } slice_sets[4];
+  } slice_sets[6];
   int sorted = -1;  // colind ascending within every row: -1 not checked yet
 };

@@ -211,6 +211,8 @@ slice_offsets_kernel(int M, int S, int W, const int2 *__restrict__ span,
 // adds the slice's partial sums into C, so a row's result is
 // ((p_0 + p_1) + p_2) + ... in slice order, the same at every call. Launches
 // run in slice order on one stream, so no two of them touch C at once.
+// colind, val and C are read (and C written) once per launch, with the
+// non-temporal policy, so the slice of B keeps the caches.
 template <typename TB, int G, int VEC, bool kAcc>
 __global__ void __launch_bounds__(kThreads)
 csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
@@ -247,7 +249,7 @@ csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
         sp_n = make_int2(lo[pos_n], hi[pos_n]);
       }
       if (!kAcc || sp.x < sp.y)
-        csr_row_sub<TB, G, VEC, false, kAcc>(row, sp, lane_sub, col0, colind, val, Bc, ldb, C,
+        csr_row_sub<TB, G, VEC, false, kAcc, true>(row, sp, lane_sub, col0, colind, val, Bc, ldb, C,
                                              ldc, nullptr);
       row = row_n;
       sp = sp_n;
@@ -258,13 +260,13 @@ csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
     if (pos >= sg.list_off + sg.count) return;  // whole wave
     const int2 sp = make_int2(lo[pos], hi[pos]);
     if (kAcc && sp.x == sp.y) return;
-    csr_row_wave<TB, G, VEC, false, kAcc>(rows[pos], sp, sub, lane_sub, col0, colind, val, Bc,
+    csr_row_wave<TB, G, VEC, false, kAcc, true>(rows[pos], sp, sub, lane_sub, col0, colind, val, Bc,
                                           ldb, C, ldc, nullptr);
   } else {
     const int pos = sg.list_off + local_wg;
     const int2 sp = make_int2(lo[pos], hi[pos]);
     if (kAcc && sp.x == sp.y) return;  // whole workgroup
-    csr_row_wg<TB, G, VEC, false, kAcc>(rows[pos], sp, wave, sub, lane_sub, col0, colind, val, Bc,
+    csr_row_wg<TB, G, VEC, false, kAcc, true>(rows[pos], sp, wave, sub, lane_sub, col0, colind, val, Bc,
                                         ldb, C, ldc, nullptr);
   }
 }
@@ -415,7 +417,7 @@ static Geometry pick_geometry(const spmm_csr_plan_s *p, int N, int elem_bytes, c
   g.VEC = vec;
   g.col_tiles = N / (G * e16 * vec);
   const int sl = t.slices;
-  g.slices = (sl == 2 || sl == 4 || sl == 8 || sl == 16) && G >= 4 && G <= 32 ? sl : 1;
+  g.slices = (sl == 2 || sl == 4 || sl == 8 || sl == 16 || sl == 32) && G >= 4 && G <= 32 ? sl : 1;
   return g;
 }
 
@@ -479,7 +481,7 @@ static void build_table_sliced(const int *counts, const int *offs, int G, int ro
 
 // The plan's slice offsets for (S, K), built on first use: one allocation,
 // one kernel and a read-back of the sortedness flag (synchronises). Up to
-// four (S, K) sets are kept; a fifth replaces the last. SPMM_ERR_UNSUPPORTED
+// six (S, K) sets are kept; a seventh replaces the last. SPMM_ERR_UNSUPPORTED
 // when colind is not ascending within its rows.
 static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hipStream_t s,
                          const int **out) {
@@ -487,8 +489,8 @@ static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hi
   for (auto &ss : sets)
     if (ss.d_off && ss.S == S && ss.K == K) { *out = ss.d_off; return SPMM_OK; }
   if (p->sorted == 0) SPMM_FAIL(SPMM_ERR_UNSUPPORTED, "column slices need ascending colind in every row");
-  int slot = 3;
-  for (int i = 0; i < 4; ++i)
+  int slot = 5;
+  for (int i = 0; i < 6; ++i)
     if (!sets[i].d_off) { slot = i; break; }
   if (sets[slot].d_off) {
     SPMM_HIP(hipStreamSynchronize(s));  // launches queued earlier may read it
@@ -701,7 +703,7 @@ static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int 
     const int lanes = N / (e16 * vec);
     for (int G = 64; G >= 1 && st == SPMM_OK; G >>= 1) {
       if (!tune_valid(N, e16, G, vec) || G * 4 < std::min(lanes, 64)) continue;  // <= 4 column tiles
-      for (int cand_i = 0; cand_i < 4 * 5 && st == SPMM_OK; ++cand_i) {
+      for (int cand_i = 0; cand_i < 4 * 6 && st == SPMM_OK; ++cand_i) {
         const int rs = 1 << (cand_i % 4), sl = cand_i < 4 ? 0 : 1 << (cand_i / 4);
         // column slices: only where one tile of B outgrows an XCD's L2, only
         // on a plan that can hold them, at rows_scale 1 and 2
