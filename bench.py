@@ -98,7 +98,7 @@ def profiled_traffic(profile: str, kernel: str):
     key = f"`{kernel}`"
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 5 and cells[0] == key:
+        if len(cells) >= 3 and cells[0] == key:
             try:
                 return int(cells[-1])
             except ValueError:
@@ -247,20 +247,27 @@ def headline(args, h, dev):
     plan = api.CsrPlan(h, rp, g.nnz)
     torch.cuda.synchronize()
     analyze_ms = (time.time() - t) * 1e3
-    geo = plan.autotune(rp, ci, v, Bd, C)
+    if args.geometry:  # a fixed geometry (profiling runs: no autotuner dispatches in the trace)
+        geo = tuple(int(x) for x in args.geometry.split(","))
+        tune = geo
+    else:
+        geo = plan.autotune(rp, ci, v, Bd, C)
+        tune = None
 
     def step():
-        api.spmm_csr(h, rp, ci, v, Bd, C_out=C, plan=plan)
+        api.spmm_csr(h, rp, ci, v, Bd, C_out=C, plan=plan, tune=tune)
 
     ms_step = time_region(step, args.steps, args.warmup)
     ms_med, batches = median_batches(step, launches=args.batch_launches)
     rows = check_rows(g, 6)
     err = sampled_rel_err(g, B.astype(np.float64), C.cpu().numpy(), rows)
     # the plan-free drop-in (spmm_csr_f32: schedule built inside the call)
-    Cd = torch.empty((g.M, N), device=dev)
-    dropin_ms, _ = median_batches(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=Cd),
-                                  launches=20, warmup=3)
-    dropin_err = sampled_rel_err(g, B.astype(np.float64), Cd.cpu().numpy(), rows)
+    dropin_ms = dropin_err = None
+    if not args.no_dropin:
+        Cd = torch.empty((g.M, N), device=dev)
+        dropin_ms, _ = median_batches(lambda: api.spmm_csr(h, rp, ci, v, Bd, C_out=Cd),
+                                      launches=20, warmup=3)
+        dropin_err = sampled_rel_err(g, B.astype(np.float64), Cd.cpu().numpy(), rows)
     return dict(g=g, B=B, N=N, geo=geo, plan=plan, dev_arrays=(rp, ci, v), ms_step=ms_step,
                 ms_med=ms_med, batches=batches, err=err, rows=rows, analyze_ms=analyze_ms,
                 dropin_ms=dropin_ms, dropin_err=dropin_err)
@@ -788,6 +795,9 @@ def main():
     ap.add_argument("--batch-launches", type=int, default=100,
                     help="launches per batch of the median-of-5 timing")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads)")
+    ap.add_argument("--geometry", default=None,
+                    help="headline geometry lanes,vec,rows_scale,slices instead of autotuning it")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the headline's drop-in timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--only", choices=["csr_small", "fwd_bwd", "batched", "spmv_aligned", "b2sr",
                                        "b2sr_grid", "bmm", "gather_calib"], default=None,
@@ -831,7 +841,7 @@ def main():
     G, VEC, _, SL = hd["geo"]
     tiles = N // (G * VEC * 4)
     if SL > 1:  # tiles x SL launches per step: slice 0 stores C, the others add into it
-        kname = f"void spmm::csr_sliced_kernel<float, {G}, {VEC}, false|true>"
+        kname = f"void spmm::csr_sliced_kernel<float, {G}, {VEC}, false/true> x {SL} slices"
         launches = tiles * SL
     else:
         kname = f"void spmm::csr_spmm_kernel<float, {G}, {VEC}>"
@@ -873,7 +883,7 @@ def main():
                    "launches_per_batch": args.batch_launches},
         "checks": {"rel_err_fp64_sampled_rows_incl_heaviest": hd["err"], "bar": 1e-5},
         "analyze_ms_host_clock": round(hd["analyze_ms"], 2),
-        "dropin_plan_free": {"ms_median_of_5": round(hd["dropin_ms"], 4),
+        "dropin_plan_free": {"ms_median_of_5": round(hd["dropin_ms"], 4) if hd["dropin_ms"] else None,
                              "rel_err_fp64_sampled": hd["dropin_err"]},
     }
     if not args.no_cpu:

@@ -174,32 +174,36 @@ csr_dropin_kernel(int M, int E, int a_limit, const int *__restrict__ rowptr,
 }
 
 // ------------------------------------------------- column-sliced launches
-// Per-row slice offsets of a plan: thread i walks the row at plan position i
-// once and records where each range of W columns starts (off[s * M + i] for
-// s = 1 .. S-1; off[0 * M + i] and off[S * M + i] are the row's span), and
-// flags a column index below its predecessor (the offsets need ascending
-// columns within a row).
+// Per-row slice offsets of a plan, one wave per row (plan position i): lane
+// s in [1, S) finds by binary search the first entry whose column is at
+// least s * W (off[s * M + i]); off[0 * M + i] and off[S * M + i] are the
+// row's span. The lanes also check that the row's columns ascend (the
+// offsets need it) and flag it otherwise.
 __global__ void __launch_bounds__(kThreads)
 slice_offsets_kernel(int M, int S, int W, const int2 *__restrict__ span,
                      const int *__restrict__ colind, int *__restrict__ off,
                      int *__restrict__ unsorted) {
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < M; i += gridDim.x * kThreads) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int waves = gridDim.x * kWavesPerWG;
+  for (int i = (blockIdx.x * kThreads + threadIdx.x) / kWave; i < M; i += waves) {
     const int2 sp = span[i];
-    off[i] = sp.x;
-    int s = 1, prev = -1;
     bool bad = false;
-    for (int e = sp.x; e < sp.y; ++e) {
-      const int c = colind[e];
-      bad |= c < prev;
-      prev = c;
-      while (s < S && static_cast<long long>(c) >= static_cast<long long>(s) * W) {
-        off[static_cast<size_t>(s) * M + i] = e;
-        ++s;
+    for (int e = sp.x + lane; e + 1 < sp.y; e += kWave) bad |= colind[e + 1] < colind[e];
+    if (__ballot(bad) && lane == 0) atomicOr(unsorted, 1);
+    if (lane >= 1 && lane < S) {
+      const long long bound = static_cast<long long>(lane) * W;
+      int lo = sp.x, hi = sp.y;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (colind[mid] < bound) lo = mid + 1;
+        else hi = mid;
       }
+      off[static_cast<size_t>(lane) * M + i] = lo;
     }
-    for (; s < S; ++s) off[static_cast<size_t>(s) * M + i] = sp.y;
-    off[static_cast<size_t>(S) * M + i] = sp.y;
-    if (bad) atomicOr(unsorted, 1);
+    if (lane == 0) {
+      off[i] = sp.x;
+      off[static_cast<size_t>(S) * M + i] = sp.y;
+    }
   }
 }
 
@@ -506,7 +510,7 @@ static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hi
   const int W = ceil_div(K, S);
   hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(slice_offsets_kernel, dim3(std::min(ceil_div(p->M, kThreads), 4096)),
+    hipLaunchKernelGGL(slice_offsets_kernel, dim3(std::min(ceil_div(p->M, kWavesPerWG), 8192)),
                        dim3(kThreads), 0, s, p->M, S, W, p->d_span, colind, d, flag);
     e = hipGetLastError();
   }
