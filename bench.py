@@ -458,7 +458,9 @@ def extra_b2sr(args, h, dev):
 
 def extra_b2sr_grid(args, h, dev):
     """Planned FBF SpMM for tiles 2..64 x N = 1/32/64/128 on the 89k graph
-    (N = 1 runs the narrow kernel), beside CSR at the same N."""
+    (N = 1 runs the narrow kernel), beside CSR at the same N; then the B2SR
+    autotuner's pick for each (T, N) (variant, lanes per row, rows_scale)
+    and its time."""
     import torch
 
     from spmm_eval_amd import api, graphs
@@ -470,7 +472,7 @@ def extra_b2sr_grid(args, h, dev):
     widths = (1, 32, 64, 128)
     Bs = {N: torch.from_numpy(graphs.uniform_pm1((g.K, N), seed=N + 17)).to(dev) for N in widths}
     Cs = {N: torch.empty((g.M, N), device=dev) for N in widths}
-    out = {"csr_us": {}, "b2sr_us": {}}
+    out = {"csr_us": {}, "b2sr_us": {}, "b2sr_tuned_us": {}, "b2sr_tuned_choice": {}}
     for N in widths:
         ms = time_graph(lambda: api.spmm_csr(h, rp, ci, v, Bs[N], C_out=Cs[N], plan=cp), Cs[N],
                         steps=50, replays=5)
@@ -482,6 +484,12 @@ def extra_b2sr_grid(args, h, dev):
             ms = time_graph(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bs[N], C_out=Cs[N],
                                                   plan=plan), Cs[N], steps=50, replays=5)
             out["b2sr_us"][f"T{T}_N{N}"] = round(ms * 1e3, 2)
+            # the autotuner's pick among the lane mappings (variant, lanes, rows_scale)
+            choice = plan.autotune(brp, bci, words, Bs[N], Cs[N])
+            ms = time_graph(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bs[N], C_out=Cs[N],
+                                                  plan=plan), Cs[N], steps=50, replays=5)
+            out["b2sr_tuned_us"][f"T{T}_N{N}"] = round(ms * 1e3, 2)
+            out["b2sr_tuned_choice"][f"T{T}_N{N}"] = list(choice)
         plan.close()
         del brp, bci, words
     torch.cuda.empty_cache()

@@ -63,12 +63,23 @@ typedef struct spmm_batch_plan_s *spmm_batch_plan_t;
  *                   column indices ascending within each row; the plan keeps
  *                   per-row slice offsets (built on first use, which
  *                   allocates and synchronises), so a plan used with slices
- *                   is valid only for the colind it saw. */
+ *                   is valid only for the colind it saw.
+ *   variant       : B2SR FBF SpMM only (the lane mappings of the reference's
+ *                   variant kernels as candidates): 0 = block-row kernels
+ *                   (lanes_per_row lanes own a column segment of a block
+ *                   row, S = 64 / lanes_per_row block rows per wave -- two
+ *                   per wave at 32 lanes, the shape of spmm4_full_full_new3,
+ *                   backend/spmm.cu:970-1050; rows_scale block rows per
+ *                   subgroup for the shortest); 1 = word lanes (a lane per
+ *                   tile row, four tiles a round, any N); 2 = B staged in LDS
+ *                   (N <= 8, T <= 8: the mapping of
+ *                   spmm4_full_full_1_1024_lessthan8, backend/spmm.cu:417-475). */
 typedef struct spmm_tune_s {
   int lanes_per_row;
   int vec;
   int rows_scale;
   int slices;
+  int variant;
 } spmm_tune_t;
 
 /* ---------------------------------------------------------------- handle */
@@ -193,6 +204,23 @@ int spmm_b2sr_bf16_planned(spmm_csr_plan_t plan, int T, int M, int K, int N,
                            const int *bsrRowPtr, const int *bsrColInd,
                            const void *bsrVal, const uint16_t *B, int ldb,
                            float *C, int ldc, void *stream);
+
+/* B2SR autotuner: times the candidate lane mappings of the planned FBF SpMM
+ * (variant 0 at every lanes_per_row dividing N's 16-byte vectors x
+ * rows_scale 1 / 2 / 4, variant 1, and variant 2 where N <= 8 and T <= 8) on
+ * these operands (B is uint16 BF16 when bf16 != 0), records the fastest in
+ * the plan for this (N, dtype) -- later planned calls use it -- and returns
+ * it in *best. Writes C; SYNCHRONISES. spmm_b2sr_plan_set pins a candidate
+ * for (N, dtype) instead (SPMM_ERR_INVALID_ARG for one that cannot run at
+ * that N); a pinned variant 0 needs the planned form's alignment at the
+ * call, else the call runs variant 1. No reference counterpart: the
+ * reference launches one kernel, and keeps its variants in the source. */
+int spmm_b2sr_autotune(spmm_csr_plan_t plan, int T, int M, int K, int N,
+                       const int *bsrRowPtr, const int *bsrColInd,
+                       const void *bsrVal, const void *B, int bf16, int ldb,
+                       float *C, int ldc, spmm_tune_t *best, void *stream);
+int spmm_b2sr_plan_set(spmm_csr_plan_t plan, int N, int bf16,
+                       const spmm_tune_t *tune);
 
 /* Binary A x sign-packed B -> int32: C[i][k] = sum over stored (i,j) of
  * (+1 if B[j][k] >= 0 else -1), with B packed by pack_sign_rows_u32 into

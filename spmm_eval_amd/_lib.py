@@ -34,7 +34,7 @@ class SpmmError(RuntimeError):
 
 class Tune(C.Structure):
     _fields_ = [("lanes_per_row", C.c_int), ("vec", C.c_int), ("rows_scale", C.c_int),
-                ("slices", C.c_int)]
+                ("slices", C.c_int), ("variant", C.c_int)]
 
 
 _vp, _i, _ip = C.c_void_p, C.c_int, C.POINTER(C.c_int)
@@ -52,6 +52,8 @@ _SIGS = {
     "spmm_csr_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_csr_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_csr_autotune": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, C.POINTER(Tune), _vp],
+    "spmm_b2sr_autotune": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, C.POINTER(Tune), _vp],
+    "spmm_b2sr_plan_set": [_vp, _i, _i, C.POINTER(Tune)],
     "csr2b2sr_nnzb": [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _ip, _vp],
     "csr2b2sr": [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "spmm_b2sr_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],

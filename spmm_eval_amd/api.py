@@ -204,6 +204,23 @@ class B2srPlan(CsrPlan):
         call("spmm_b2sr_analyze", h.h, T, M, _p(brp), C.byref(p), _stream(stream))
         self.p = p
 
+    def autotune(self, brp, bci, words, B, C_out, stream=None) -> tuple:
+        """Times the candidate lane mappings on these operands, keeps the
+        fastest in the plan for (N, dtype) and returns it as
+        (variant, lanes_per_row, rows_scale)."""
+        K, N = B.shape
+        best = Tune()
+        call("spmm_b2sr_autotune", self.p, self.T, self.M, K, N, _p(brp), _p(bci), _p(words), _p(B),
+             int(_is_bf16(B)), B.stride(0), _p(C_out), C_out.stride(0), C.byref(best),
+             _stream(stream))
+        return (best.variant, best.lanes_per_row, best.rows_scale)
+
+    def set_geometry(self, N: int, bf16: bool, variant: int, lanes_per_row: int = 0,
+                     rows_scale: int = 1):
+        """Pins a candidate for (N, dtype) (spmm_b2sr_plan_set)."""
+        call("spmm_b2sr_plan_set", self.p, N, int(bf16),
+             C.byref(Tune(lanes_per_row, 1, rows_scale, 0, variant)))
+
 
 def spmm_b2sr(h: Handle, T: int, M: int, brp, bci, words, B, C_out=None,
               plan: B2srPlan | None = None, stream=None):

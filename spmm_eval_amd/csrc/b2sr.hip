@@ -1046,6 +1046,72 @@ b2sr_narrow_kernel(int M, int N, const int *__restrict__ brp, const int *__restr
   }
 }
 
+// N <= 8, T <= 8: B staged in LDS, the lane mapping of the reference's
+// spmm4_full_full_1_1024_lessthan8 (backend/spmm.cu:417-475) for wave64. A
+// wave per block row takes Q = 64 / T tiles a round: lane l stages the N
+// values of B row (block column of tile l / T) * T + l % T into the wave's
+// slab and loads tile l / T's row l % T word; then output lane o = (r, n)
+// (o < T N) adds, tile by tile and column by column (so in ascending column
+// order, as the CSR kernels sum), the staged values of its row's set bits.
+// The slab is rewritten next round.
+template <int T, typename TB>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_lds_kernel(int M, int N, const int *__restrict__ brp, const int *__restrict__ bci,
+                const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
+                int ldb, float *__restrict__ C, int ldc) {
+  constexpr int Q = kWave / T;
+  constexpr int SP = 9;  // slab stride (N <= 8), odd to spread the LDS banks
+  __shared__ float slab[kB2Waves][kWave * SP];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int MB = (M + T - 1) / T;
+  const int br = blockIdx.x * kB2Waves + wave;
+  if (br >= MB) return;  // whole wave; the kernel has no workgroup barrier
+  const int t0 = brp[br], t1 = brp[br + 1];
+  const bool out = lane < T * N;
+  const int r = out ? lane / N : 0, n = out ? lane % N : 0;
+  float *sw = slab[wave];
+  float acc = 0.f;
+  for (int t = t0; t < t1; t += Q) {
+    const int tl = t + lane / T, c = lane % T;
+    uint32_t w = 0;
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = 0.f;
+    if (tl < t1) {
+      w = words[static_cast<size_t>(tl) * T + c];  // tile tl, row c
+      const TB *Brow = B + static_cast<size_t>(bci[tl] * T + c) * ldb;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < N) v[k] = load_scalar<TB>(Brow + k);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sw[lane * SP + k] = v[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int nq = min(Q, t1 - t);
+    for (int q = 0; q < nq; ++q) {
+      const uint32_t bits = __shfl(w, q * T + r);  // tile q's row r
+      if (out) {
+#pragma unroll
+        for (int cc = 0; cc < T; ++cc)
+          if ((bits >> (T - 1 - cc)) & 1u) acc += sw[(q * T + cc) * SP + n];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // the slab is rewritten next round
+  }
+  if (out && br * T + r < M) C[static_cast<size_t>(br * T + r) * ldc + n] = acc;
+}
+
+template <int T, typename TB>
+static void launch_lds(int M, int N, const int *brp, const int *bci, const void *val,
+                       const TB *B, int ldb, float *C, int ldc, hipStream_t s) {
+  const int MB = (M + T - 1) / T;
+  hipLaunchKernelGGL((b2sr_lds_kernel<T, TB>), dim3(ceil_div(MB, kB2Waves)), dim3(kB2Threads), 0,
+                     s, M, N, brp, bci, static_cast<const typename Word<T>::type *>(val), B, ldb,
+                     C, ldc);
+}
+
 // Binary A x sign-packed B -> int32. Lane owns 4 consecutive columns and
 // counts the +1 bits; C = 2 * count - (number of stored entries in the row).
 template <int T, int G>
@@ -1370,7 +1436,7 @@ static void convert(int T, int M, int K, const int *rowptr, const int *colind, i
 }
 
 struct Geo {
-  int G = 0, VEC = 1, tiles = 1;
+  int G = 0, VEC = 1, tiles = 1, rows_scale = 1;
 };
 
 static Geo geometry(int N, int elem_bytes, const void *B, int ldb, const void *C, int ldc) {
@@ -1572,14 +1638,14 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
 template <int T, typename TB, int G>
 static void launch_brow(const spmm_csr_plan_s *p, int M, int K, const int *brp, const int *bci,
                         const void *val, const TB *B, int ldb, float *C, int ldc, int tiles,
-                        hipStream_t s) {
+                        int rows_scale, hipStream_t s) {
   constexpr int S = kWave / G;
   constexpr int TC = BrowChunk<G>::kTiles;
   LaunchTable t;
   // bins wholly below one subgroup chunk of tiles: a subgroup per block row
   // (several only below 8 tiles); wholly below one wave chunk: a wave (the
   // wave path decodes exactly one chunk); the rest: a workgroup per row
-  build_table_ex(p->counts, p->offs, S, TC, BrowChunk<G>::kWaveTiles, 8, 16, 1, &t, 1, T);
+  build_table_ex(p->counts, p->offs, S, TC, BrowChunk<G>::kWaveTiles, 8, 16, rows_scale, &t, 1, T);
   if (t.total_wg == 0) return;
   const auto *tw = static_cast<const typename TileWord<T>::type *>(val);
   // B rows are read through a buffer resource when B spans < 2 GiB
@@ -1599,13 +1665,13 @@ static void dispatch_brow(const Geo &g, const spmm_csr_plan_s *p, int M, int K, 
                           const int *bci, const void *val, const TB *B, int ldb, float *C,
                           int ldc, hipStream_t s) {
   switch (g.G) {
-    case 1: launch_brow<T, TB, 1>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 2: launch_brow<T, TB, 2>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 4: launch_brow<T, TB, 4>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 8: launch_brow<T, TB, 8>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 16: launch_brow<T, TB, 16>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    case 32: launch_brow<T, TB, 32>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
-    default: launch_brow<T, TB, 64>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, s); break;
+    case 1: launch_brow<T, TB, 1>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
+    case 2: launch_brow<T, TB, 2>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
+    case 4: launch_brow<T, TB, 4>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
+    case 8: launch_brow<T, TB, 8>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
+    case 16: launch_brow<T, TB, 16>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
+    case 32: launch_brow<T, TB, 32>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
+    default: launch_brow<T, TB, 64>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
   }
 }
 
@@ -1656,10 +1722,40 @@ static void dispatch_wide(const Geo &g, const spmm_csr_plan_s *p, const int *brp
 #undef SPMM_WD
 }
 
+// The candidate a planned call runs: `tune` when given, else the plan's
+// entry for (N, dtype), else variant 0 at the default lane count (the widest
+// power of two dividing N's 16-byte vectors). Variant 0 without the
+// alignment of the vector kernels, or variant 2 outside N <= 8 and T <= 8,
+// runs variant 1.
+static Geo b2sr_candidate(const spmm_csr_plan_s *p, int T, int N, int elem_bytes, const void *B,
+                          int ldb, const void *C, int ldc, const spmm_tune_t *tune,
+                          int *variant) {
+  spmm_tune_t t = {0, 0, 0, 0, 0};
+  if (tune) {
+    t = *tune;
+  } else if (p) {
+    for (int i = 0; i < p->ntuned; ++i)
+      if (p->tuned[i].N == N && p->tuned[i].elem_bytes == elem_bytes) t = p->tuned[i].tune;
+  }
+  Geo g = geometry(N, elem_bytes, B, ldb, C, ldc);
+  *variant = t.variant;
+  if (t.variant == 2 && !(N <= 8 && T <= 8)) *variant = 1;
+  if (*variant == 0 && g.G == 0) *variant = 1;
+  if (*variant != 0) return Geo{};
+  const int lanes = N / (16 / elem_bytes);
+  const int G = t.lanes_per_row;
+  if (G >= 1 && G <= 64 && (G & (G - 1)) == 0 && lanes % G == 0) {
+    g.G = G;
+    g.tiles = lanes / G;
+  }
+  g.rows_scale = t.rows_scale == 2 || t.rows_scale == 4 ? t.rows_scale : 1;
+  return g;
+}
+
 template <typename TB>
 static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const int *brp,
                            const int *bci, const void *val, const TB *B, int ldb, float *C,
-                           int ldc, void *stream) {
+                           int ldc, void *stream, const spmm_tune_t *tune = nullptr) {
   SPMM_CHECK_ARG(p != nullptr, "plan is null");
   SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
@@ -1677,8 +1773,18 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
                  "bsrVal must be 16-byte aligned for T = 16, 32 and 64");
   SPMM_CHECK_ARG(K < (1 << 27), "planned B2SR needs K < 2^27");
   hipStream_t s = as_stream(stream);
-  const Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
-  if (g.G == 0) {  // the narrow path needs no schedule
+  int variant = 0;
+  const Geo g = b2sr_candidate(p, T, N, sizeof(TB), B, ldb, C, ldc, tune, &variant);
+  if (variant == 2) {  // B staged in LDS (N <= 8, T <= 8)
+    switch (T) {
+      case 2: launch_lds<2, TB>(M, N, brp, bci, val, B, ldb, C, ldc, s); break;
+      case 4: launch_lds<4, TB>(M, N, brp, bci, val, B, ldb, C, ldc, s); break;
+      default: launch_lds<8, TB>(M, N, brp, bci, val, B, ldb, C, ldc, s); break;
+    }
+    SPMM_LAUNCHED();
+    return SPMM_OK;
+  }
+  if (g.G == 0) {  // word lanes (variant 1): no schedule needed
 #define SPMM_FBF(t) dispatch_fbf<t, TB>(g, M, N, brp, bci, val, B, ldb, C, ldc, nullptr, 0, s)
     switch (T) {
       case 2: SPMM_FBF(2); break;
@@ -1875,6 +1981,107 @@ int spmm_b2sr_bf16_planned(spmm_csr_plan_t plan, int T, int M, int K, int N, con
                            float *C, int ldc, void *stream) {
   return run_planned_fbf<uint16_t>(plan, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, B, ldb, C, ldc,
                                    stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Records `t` in the plan for (N, dtype): the entry's slot, else a free one,
+// else the last.
+void record_tune(spmm_csr_plan_s *p, int N, int elem_bytes, const spmm_tune_t &t) {
+  int slot = -1;
+  for (int i = 0; i < p->ntuned; ++i)
+    if (p->tuned[i].N == N && p->tuned[i].elem_bytes == elem_bytes) slot = i;
+  if (slot < 0) slot = p->ntuned < kTuneSlots ? p->ntuned++ : kTuneSlots - 1;
+  p->tuned[slot] = {N, elem_bytes, t};
+}
+
+template <typename TB>
+int b2sr_autotune(spmm_csr_plan_s *p, int T, int M, int K, int N, const int *brp, const int *bci,
+                  const void *val, const TB *B, int ldb, float *C, int ldc,
+                  spmm_tune_t *best_out, void *stream) {
+  SPMM_CHECK_ARG(p != nullptr, "plan is null");
+  if (p->tile != T || p->M != M)
+    SPMM_FAIL(SPMM_ERR_MISMATCH, "plan was not built by spmm_b2sr_analyze for this T and M");
+  SPMM_CHECK_ARG(N > 0, "N must be positive");
+  hipStream_t s = as_stream(stream);
+  const int e16 = 16 / static_cast<int>(sizeof(TB));
+  spmm_tune_t cands[32];
+  int nc = 0;
+  if (geometry(N, sizeof(TB), B, ldb, C, ldc).G) {
+    const int lanes = N / e16;
+    for (int G = 64; G >= 1; G >>= 1) {
+      if (lanes % G || G * 8 < std::min(lanes, 64)) continue;  // at most 8 column tiles
+      for (int rs = 1; rs <= (T <= 8 ? 4 : 1); rs <<= 1) cands[nc++] = {G, 1, rs, 0, 0};
+    }
+  }
+  cands[nc++] = {0, 0, 1, 0, 1};
+  if (N <= 8 && T <= 8) cands[nc++] = {0, 0, 1, 0, 2};
+  hipEvent_t e0, e1;
+  SPMM_HIP(hipEventCreate(&e0));
+  SPMM_HIP(hipEventCreate(&e1));
+  spmm_tune_t best = cands[0];
+  float best_ms = 1e30f;
+  int st = SPMM_OK;
+  for (int c = 0; c < nc && st == SPMM_OK; ++c) {
+    st = run_planned_fbf<TB>(p, T, M, K, N, brp, bci, val, B, ldb, C, ldc, stream, &cands[c]);
+    if (st) break;
+    if (hipEventRecord(e0, s) != hipSuccess) { st = SPMM_ERR_HIP; break; }
+    for (int r = 0; r < 3 && st == SPMM_OK; ++r)
+      st = run_planned_fbf<TB>(p, T, M, K, N, brp, bci, val, B, ldb, C, ldc, stream, &cands[c]);
+    if (st) break;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      st = SPMM_ERR_HIP;
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cands[c];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  record_tune(p, N, static_cast<int>(sizeof(TB)), best);
+  if (best_out) *best_out = best;
+  return SPMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spmm_b2sr_autotune(spmm_csr_plan_t plan, int T, int M, int K, int N, const int *bsrRowPtr,
+                       const int *bsrColInd, const void *bsrVal, const void *B, int bf16, int ldb,
+                       float *C, int ldc, spmm_tune_t *best, void *stream) {
+  if (bf16)
+    return b2sr_autotune<uint16_t>(plan, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal,
+                                   static_cast<const uint16_t *>(B), ldb, C, ldc, best, stream);
+  return b2sr_autotune<float>(plan, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal,
+                              static_cast<const float *>(B), ldb, C, ldc, best, stream);
+}
+
+int spmm_b2sr_plan_set(spmm_csr_plan_t plan, int N, int bf16, const spmm_tune_t *tune) {
+  SPMM_CHECK_ARG(plan && tune, "null plan or tune");
+  SPMM_CHECK_ARG(plan->tile != 0, "not a B2SR plan (spmm_b2sr_analyze)");
+  SPMM_CHECK_ARG(N > 0, "N must be positive");
+  const int e16 = bf16 ? 8 : 4;
+  const int v = tune->variant, G = tune->lanes_per_row, rs = tune->rows_scale;
+  SPMM_CHECK_ARG(v >= 0 && v <= 2, "variant must be 0, 1 or 2");
+  SPMM_CHECK_ARG(v != 2 || (N <= 8 && plan->tile <= 8), "variant 2 needs N <= 8 and T <= 8");
+  SPMM_CHECK_ARG(v != 0 || N % e16 == 0, "variant 0 needs N a multiple of one 16-byte vector");
+  SPMM_CHECK_ARG(v != 0 || G == 0 ||
+                     (G <= 64 && (G & (G - 1)) == 0 && (N / e16) % G == 0),
+                 "lanes_per_row must be a power of two dividing N's 16-byte vectors");
+  SPMM_CHECK_ARG(rs == 0 || rs == 1 || rs == 2 || rs == 4, "rows_scale must be 1, 2 or 4");
+  spmm_tune_t t = *tune;
+  t.vec = 1;
+  t.slices = 0;
+  record_tune(plan, N, bf16 ? 2 : 4, t);
+  return SPMM_OK;
 }
 
 int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,

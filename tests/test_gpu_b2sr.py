@@ -372,3 +372,86 @@ def test_harness_flow_from_mtx(handle, dev, tmp_path):
     C_csr = api.spmm_csr(handle, rp, ci, v, S)
     torch.cuda.synchronize()
     assert torch.equal(C_b2sr, C_csr)  # "TEST PASSED: 1"
+
+
+def b2sr_candidates(T, N, elem):
+    """The candidate lane mappings spmm_b2sr_autotune times (variant, lanes,
+    rows_scale): block-row kernels at every power-of-two lane count dividing
+    N's 16-byte vectors (at most 8 column tiles) x rows_scale, word lanes,
+    and B staged in LDS for N <= 8 and T <= 8."""
+    e16 = 16 // elem
+    out = []
+    if N % e16 == 0:
+        lanes = N // e16
+        G = 64
+        while G >= 1:
+            if lanes % G == 0 and G * 8 >= min(lanes, 64):
+                for rs in ((1, 2, 4) if T <= 8 else (1,)):
+                    out.append((0, G, rs))
+            G //= 2
+    out.append((1, 0, 1))
+    if N <= 8 and T <= 8:
+        out.append((2, 0, 1))
+    return out
+
+
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
+def test_b2sr_every_tuner_candidate(handle, dev, T):
+    """Every candidate of the B2SR autotuner, pinned on the plan: +-1 B bit-exact
+    against integer CSR, uniform B within 1e-5 of FP64 (FP32 and BF16 B), on a
+    power-law graph whose hub block rows take every row class."""
+    g = graphs.chung_lu(6000, 120000, seed=T, max_deg=2500)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    for N in (1, 4, 8, 32, 64, 128):
+        S = np.random.default_rng(N).choice([-1, 1], size=(g.K, N)).astype(np.int32)
+        exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.float32)
+        Sd = torch.from_numpy(S.astype(np.float32)).to(dev)
+        B = graphs.uniform_pm1((g.K, N), seed=N + T)
+        Bd = torch.from_numpy(B).to(dev)
+        ref = oracle.csr_spmm_f64(g.rowptr, g.colind, np.ones(g.nnz, np.float32), B)
+        Bh = api.to_bf16_bits(Bd)
+        refh = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, np.ones(g.nnz, np.float32), np_words(Bh))
+        for elem, BB, rr in ((4, Bd, ref), (2, Bh, refh)):
+            for var, G, rs in b2sr_candidates(T, N, elem):
+                plan.set_geometry(N, elem == 2, var, G, rs)
+                C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, BB, plan=plan).cpu().numpy()
+                err = np.linalg.norm(C - rr) / max(np.linalg.norm(rr), 1e-30)
+                assert err <= 1e-5, (T, N, elem, var, G, rs, err)
+                if elem == 4:
+                    Cs = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Sd, plan=plan).cpu().numpy()
+                    assert np.array_equal(Cs, exact), (T, N, var, G, rs)
+    plan.close()
+
+
+@pytest.mark.parametrize("T,N", [(4, 64), (4, 8), (16, 32), (8, 4)])
+def test_b2sr_autotune_keeps_a_candidate(handle, dev, T, N):
+    g = graphs.chung_lu(20000, 400000, seed=5)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    Bd = torch.from_numpy(graphs.uniform_pm1((g.K, N), seed=3)).to(dev)
+    C = torch.empty((g.M, N), device=dev)
+    var, G, rs = plan.autotune(brp, bci, words, Bd, C)
+    assert (var, G, rs) in b2sr_candidates(T, N, 4)
+    C1 = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=plan)  # the tuned entry
+    plan.set_geometry(N, False, var, G, rs)
+    C2 = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=plan)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    ref = oracle.csr_spmm_f64(g.rowptr, g.colind, np.ones(g.nnz, np.float32), Bd.cpu().numpy())
+    assert np.linalg.norm(C1.cpu().numpy() - ref) / np.linalg.norm(ref) <= 1e-5
+    plan.close()
+
+
+def test_b2sr_plan_set_rejects(handle, dev):
+    g = graphs.chung_lu(500, 4000, seed=2)
+    rp, ci = dev_csr(g, dev)
+    brp, _, _ = api.csr2b2sr(handle, rp, ci, g.M, g.K, 16)
+    plan = api.B2srPlan(handle, 16, g.M, brp)
+    for args in ((8, False, 2), (64, False, 3), (6, False, 0), (64, False, 0, 3), (64, False, 0, 4, 3)):
+        with pytest.raises(SpmmError) as ei:
+            plan.set_geometry(*args)
+        assert ei.value.status == 1, args
+    plan.close()
