@@ -312,14 +312,14 @@ def unpack_cols(words, H: int, W: int, stream=None):
     return X
 
 
-def bmm32_xnor_f32(A, B, M: int, N: int, Kw: int, stream=None):
-    Cm = torch.empty((M, N), dtype=torch.float32, device=A.device)
-    call("bmm32_xnor_f32", M, N, Kw, _p(A), _p(B), _p(Cm), N, _stream(stream))
+def bmm32_xnor_f32(A, B, M: int, N: int, Kw: int, out=None, stream=None):
+    Cm = torch.empty((M, N), dtype=torch.float32, device=A.device) if out is None else out
+    call("bmm32_xnor_f32", M, N, Kw, _p(A), _p(B), _p(Cm), Cm.stride(0), _stream(stream))
     return Cm
 
 
-def bmm32_xnor_bin(A, B, M: int, N: int, Kbits: int, stream=None):
-    Cw = torch.empty(((N + 31) // 32, M), dtype=torch.uint32, device=A.device)
+def bmm32_xnor_bin(A, B, M: int, N: int, Kbits: int, out=None, stream=None):
+    Cw = torch.empty(((N + 31) // 32, M), dtype=torch.uint32, device=A.device) if out is None else out
     call("bmm32_xnor_bin", M, N, Kbits, _p(A), _p(B), _p(Cw), _stream(stream))
     return Cw
 

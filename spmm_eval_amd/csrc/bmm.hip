@@ -16,12 +16,22 @@ namespace spmm {
 constexpr int kBmmThreads = 256;
 constexpr int kBmmKC = 32;  // K words per LDS stage
 
+// FP32 output: each wave's 64 x 32 result tile is staged in LDS (row stride
+// 36 floats) and written as 16-byte stores, eight lanes per 128-byte row
+// segment (a lane writing its own row's 32 values would touch 64 rows per
+// store instruction). Workgroups at the right edge or with an unaligned C /
+// ldc store element by element.
+constexpr int kBmmCS = 36;
+constexpr int kBmmLdsWords = 4 * 64 * kBmmCS;  // >= kBmmKC * 128 (the B stage)
+
 template <bool kBin>
 __global__ void __launch_bounds__(kBmmThreads)
 bmm32_kernel(int M, int N, int Kw, int Kbits, const uint32_t *__restrict__ A,
              const uint32_t *__restrict__ B, float *__restrict__ Cf, int ldc,
-             uint32_t *__restrict__ Cb) {
-  __shared__ uint32_t Bs[kBmmKC][4 * 32];
+             uint32_t *__restrict__ Cb, int vec_c) {
+  static_assert(kBmmLdsWords >= kBmmKC * 128, "the C stage reuses the B stage");
+  __shared__ uint32_t lds[kBmmLdsWords];
+  uint32_t(*Bs)[4 * 32] = reinterpret_cast<uint32_t(*)[4 * 32]>(lds);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = blockIdx.x * 64 + lane;
   const int n0 = (blockIdx.y * 4 + wave) * 32;  // first column of this thread's word
@@ -45,8 +55,8 @@ bmm32_kernel(int M, int N, int Kw, int Kbits, const uint32_t *__restrict__ A,
       for (int j = 0; j < 32; ++j) pop[j] += __popc(a ^ Bs[i][wave * 32 + j]);
     }
   }
-  if (m >= M || n0 >= N) return;
   if (kBin) {
+    if (m >= M || n0 >= N) return;
     uint32_t word = 0;
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
@@ -54,12 +64,38 @@ bmm32_kernel(int M, int N, int Kw, int Kbits, const uint32_t *__restrict__ A,
       word = (word << 1) | ((n0 + j < N && t >= 0.f) ? 1u : 0u);
     }
     Cb[static_cast<size_t>(n0 / 32) * M + m] = word;
-  } else {
-    const float full = static_cast<float>(32 * Kw);
+    return;
+  }
+  const float full = static_cast<float>(32 * Kw);
+  // one decision per workgroup (its barrier below needs every wave): the
+  // staged path when all 128 of its columns lie inside N
+  if (!vec_c || static_cast<int>(blockIdx.y) * 128 + 128 > N) {  // edge tile or unaligned C
+    if (m >= M || n0 >= N) return;
     float *row = Cf + static_cast<size_t>(m) * ldc + n0;
 #pragma unroll
     for (int j = 0; j < 32; ++j)
       if (n0 + j < N) row[j] = full - static_cast<float>(pop[j]) * 2.0f;
+    return;
+  }
+  __syncthreads();  // the B stage is reused for the C tiles
+  float *ct = reinterpret_cast<float *>(lds) + wave * 64 * kBmmCS;
+#pragma unroll
+  for (int j = 0; j < 32; j += 4)
+    *reinterpret_cast<float4 *>(ct + lane * kBmmCS + j) =
+        make_float4(full - 2.0f * static_cast<float>(pop[j]),
+                    full - 2.0f * static_cast<float>(pop[j + 1]),
+                    full - 2.0f * static_cast<float>(pop[j + 2]),
+                    full - 2.0f * static_cast<float>(pop[j + 3]));
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int m0 = blockIdx.x * 64;
+#pragma unroll
+  for (int it = 0; it < 8; ++it) {
+    const int idx = it * 64 + lane, r = idx >> 3, c4 = (idx & 7) * 4;
+    if (m0 + r < M)
+      *reinterpret_cast<float4 *>(Cf + static_cast<size_t>(m0 + r) * ldc + n0 + c4) =
+          *reinterpret_cast<const float4 *>(ct + r * kBmmCS + c4);
   }
 }
 
@@ -76,8 +112,9 @@ int bmm32_xnor_f32(int M, int N, int Kw, const uint32_t *A, const uint32_t *B, f
   SPMM_CHECK_ARG(C && ldc >= N, "null C or ldc < N");
   SPMM_CHECK_ARG(Kw == 0 || (A && B), "null A or B");
   dim3 grid(ceil_div(M, 64), ceil_div(N, 128));
+  const int vec_c = aligned16(C) && ldc % 4 == 0;
   hipLaunchKernelGGL((bmm32_kernel<false>), grid, dim3(kBmmThreads), 0, as_stream(stream), M, N,
-                     Kw, 32 * Kw, A, B, C, ldc, nullptr);
+                     Kw, 32 * Kw, A, B, C, ldc, nullptr, vec_c);
   SPMM_LAUNCHED();
   return SPMM_OK;
 }
@@ -91,7 +128,7 @@ int bmm32_xnor_bin(int M, int N, int Kbits, const uint32_t *A, const uint32_t *B
   SPMM_CHECK_ARG(Kw == 0 || (A && B), "null A or B");
   dim3 grid(ceil_div(M, 64), ceil_div(N, 128));
   hipLaunchKernelGGL((bmm32_kernel<true>), grid, dim3(kBmmThreads), 0, as_stream(stream), M, N, Kw,
-                     Kbits, A, B, nullptr, 0, C);
+                     Kbits, A, B, nullptr, 0, C, 0);
   SPMM_LAUNCHED();
   return SPMM_OK;
 }

@@ -47,7 +47,7 @@ __device__ __forceinline__ void accumulate_range(
     const int *__restrict__ colind, const float *__restrict__ val,
     const TB *__restrict__ Bc, int ldb, float (&acc)[EPL],
     const int *__restrict__ perm = nullptr) {
-  constexpr int U0 = EPL >= 16 ? 4 : 8;  // loads in flight per lane
+  constexpr int U0 = EPL > 16 ? 4 : 8;  // loads in flight per lane (8 x 32 B for BF16 at 2 vectors)
   constexpr int U = G < U0 ? G : U0;
   constexpr int V = EPL / Vec16<TB>::kElems;
   int e = e_begin + first_chunk_off;
@@ -395,7 +395,7 @@ __device__ __forceinline__ void accumulate_range_sddmm(
     const int *__restrict__ tcol, const float *__restrict__ val, const int *__restrict__ perm,
     const TB *__restrict__ dYc, int lddy, const float (&x)[EPL], float (&acc)[EPL],
     float *__restrict__ dval) {
-  constexpr int U0 = EPL >= 16 ? 4 : 8;
+  constexpr int U0 = EPL > 16 ? 4 : 8;
   constexpr int U = G < U0 ? G : U0;
   constexpr int V = EPL / Vec16<TB>::kElems;
   int e = e_begin + first_chunk_off;

@@ -26,7 +26,8 @@ def np_u32(t):
 
 
 @pytest.mark.parametrize("M,N,Kw", [(1, 1, 1), (32, 32, 1), (64, 128, 4), (100, 70, 7), (256, 512, 33),
-                                    (128, 128, 4), (1024, 1024, 32), (4096, 4096, 128)])
+                                    (128, 128, 4), (130, 384, 5), (70, 260, 3), (1024, 1024, 32),
+                                    (4096, 4096, 128)])
 def test_bmm(dev, M, N, Kw):
     r = np.random.default_rng(M * N + Kw)
     A = r.integers(0, 2 ** 32, (Kw, M), dtype=np.uint64).astype(np.uint32)
