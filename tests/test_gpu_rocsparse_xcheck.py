@@ -1,7 +1,19 @@
-"""Cross-check against an independent implementation on the same GPU:
-torch.sparse on ROCm runs hipSPARSE/rocSPARSE. Both sides compute in FP32,
-so the bar is the FP32 tolerance (relative Frobenius <= 1e-5), not bits.
-A torch build whose sparse backend lacks an operation skips that case."""
+"""Cross-checks against independent implementations.
+
+On the GPU, through rocSPARSE: torch.sparse CSR @ dense and sampled_addmm
+run rocsparse::csrmmnn_row_split_kernel and rocsparse::sddmm_csx_kernel on
+ROCm (kernel trace in profiles/rocsparse_xcheck.md); csr2bsr is called in
+librocsparse directly (rocsparse_csr2bsr_nnz + rocsparse_scsr2bsr, ctypes,
+on torch's stream), its float blocks packed to words on the host. Products
+are FP32 on both sides, so their bar is the FP32 tolerance (relative
+Frobenius <= 1e-5), not bits; the conversion must match word for word.
+
+On the CPU: torch's to_sparse_bsr (it converts on the host even for device
+tensors in this torch build, as its warning says) as a second, independent
+CSR -> BSR for the same comparison. A build lacking an operation skips that
+case."""
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -67,10 +79,77 @@ def pack_blocks(vals: np.ndarray, T: int) -> np.ndarray:
     return b.reshape(-1, T // 8).view(f">u{T // 8}").reshape(-1).astype(f"<u{T // 8}")
 
 
+def _rocsparse():
+    for name in ("librocsparse.so", "/opt/rocm/lib/librocsparse.so"):
+        try:
+            L = C.CDLL(name)
+            break
+        except OSError:
+            L = None
+    if L is None:
+        return None
+    vp, i = C.c_void_p, C.c_int
+    L.rocsparse_create_handle.argtypes = [C.POINTER(vp)]
+    L.rocsparse_destroy_handle.argtypes = [vp]
+    L.rocsparse_set_stream.argtypes = [vp, vp]
+    L.rocsparse_create_mat_descr.argtypes = [C.POINTER(vp)]
+    L.rocsparse_destroy_mat_descr.argtypes = [vp]
+    L.rocsparse_csr2bsr_nnz.argtypes = [vp, i, i, i, vp, vp, vp, i, vp, vp, C.POINTER(i)]
+    L.rocsparse_scsr2bsr.argtypes = [vp, i, i, i, vp, vp, vp, vp, i, vp, vp, vp, vp]
+    for f in ("rocsparse_create_handle", "rocsparse_destroy_handle", "rocsparse_set_stream",
+              "rocsparse_create_mat_descr", "rocsparse_destroy_mat_descr",
+              "rocsparse_csr2bsr_nnz", "rocsparse_scsr2bsr"):
+        getattr(L, f).restype = i
+    return L
+
+
 @pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
-def test_csr2b2sr_vs_rocsparse_bsr(handle, dev, T):
-    """Our device conversion against rocSPARSE csr2bsr (torch's
-    to_sparse_bsr on the GPU) on the same 0/1 matrix, padded to whole
+def test_csr2b2sr_vs_rocsparse_csr2bsr(handle, dev, T):
+    """Our device conversion against rocSPARSE's csr2bsr on the GPU (block
+    rows by row, row-major blocks), on the same 0/1 matrix: block row
+    pointers, block columns and every tile word equal."""
+    L = _rocsparse()
+    if L is None:
+        pytest.skip("librocsparse not loadable")
+    g = graphs.random_csr(700, 900, 0.01, seed=T, long_rows=2, long_len=900)
+    rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
+    ones = torch.ones(max(g.nnz, 1), device=dev)
+    MB = -(-g.M // T)
+    h, da, db = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert L.rocsparse_create_handle(C.byref(h)) == 0
+    assert L.rocsparse_set_stream(h, C.c_void_p(torch.cuda.current_stream().cuda_stream)) == 0
+    assert L.rocsparse_create_mat_descr(C.byref(da)) == 0
+    assert L.rocsparse_create_mat_descr(C.byref(db)) == 0
+    try:
+        brp = torch.empty(MB + 1, dtype=torch.int32, device=dev)
+        nb = C.c_int()
+        assert L.rocsparse_csr2bsr_nnz(h, 0, g.M, g.K, da, C.c_void_p(rp.data_ptr()),
+                                       C.c_void_p(ci.data_ptr()), T, db, C.c_void_p(brp.data_ptr()),
+                                       C.byref(nb)) == 0
+        bval = torch.empty(max(nb.value, 1) * T * T, device=dev)
+        bci = torch.empty(max(nb.value, 1), dtype=torch.int32, device=dev)
+        assert L.rocsparse_scsr2bsr(h, 0, g.M, g.K, da, C.c_void_p(ones.data_ptr()),
+                                    C.c_void_p(rp.data_ptr()), C.c_void_p(ci.data_ptr()), T, db,
+                                    C.c_void_p(bval.data_ptr()), C.c_void_p(brp.data_ptr()),
+                                    C.c_void_p(bci.data_ptr())) == 0
+        torch.cuda.synchronize()
+    finally:
+        L.rocsparse_destroy_mat_descr(da)
+        L.rocsparse_destroy_mat_descr(db)
+        L.rocsparse_destroy_handle(h)
+    ours = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    assert ours[0].cpu().tolist() == brp.cpu().tolist()
+    assert ours[1].cpu().tolist() == bci[: nb.value].cpu().tolist()
+    ref = pack_blocks(bval[: nb.value * T * T].cpu().numpy().reshape(nb.value, T, T), T)
+    signed = {torch.uint16: torch.int16, torch.uint32: torch.int32, torch.uint64: torch.int64}
+    w = ours[2].view(signed[ours[2].dtype]) if ours[2].dtype in signed else ours[2]
+    np.testing.assert_array_equal(w.cpu().numpy().view(ref.dtype), ref)
+
+
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64])
+def test_csr2b2sr_vs_torch_cpu_bsr(handle, dev, T):
+    """Our device conversion against torch's to_sparse_bsr (a host
+    conversion in this torch build) on the same 0/1 matrix, padded to whole
     blocks: block row pointers, block columns and every tile word equal."""
     g = graphs.random_csr(700, 900, 0.01, seed=T, long_rows=2, long_len=900)
     MB, KB = -(-g.M // T), -(-g.K // T)
