@@ -84,6 +84,9 @@ typedef struct spmm_tune_s {
 
 /* ---------------------------------------------------------------- handle */
 int spmm_create(spmm_handle_t *out);
+/* Waits for the device, then frees the handle's buffers, including those a
+ * larger call replaced (kept until here: work queued earlier, or a graph
+ * captured from an earlier call, may still use them). */
 int spmm_destroy(spmm_handle_t h);
 const char *spmm_status_string(int status);
 const char *spmm_last_error(void);
@@ -109,7 +112,17 @@ int spmm_csr_plan_destroy(spmm_csr_plan_t plan);
  * bin b >= 1: degree in [2^(b-1), 2^b)). */
 int spmm_csr_plan_bins(spmm_csr_plan_t plan, int *counts32);
 
-/* Drop-in form: analyses then computes (synchronises once, in analysis). */
+/* Drop-in form: no host analysis and no synchronisation. A small kernel
+ * classifies the rows on the device (wave- and workgroup-sized rows listed
+ * in state the handle owns), then one kernel runs them with the planned
+ * kernel's per-row code, so the result equals the planned form's at the same
+ * lane geometry bit for bit (the default geometry when neither is tuned).
+ * The handle's state grows (hipMalloc) on the first call at a larger M;
+ * growing while the stream is being captured returns SPMM_ERR_UNSUPPORTED,
+ * so a caller that captures makes one call at the largest size first. Calls
+ * through one handle must be ordered on one stream. N not a multiple of one
+ * 16-byte vector, or unaligned B / C / ld, runs the narrow kernel on a
+ * per-call analysis instead (synchronises). */
 int spmm_csr_f32(spmm_handle_t h, int M, int K, int N, int nnz,
                  const int *rowptr, const int *colind, const float *val,
                  const float *B, int ldb, float *C, int ldc, void *stream);
@@ -163,7 +176,10 @@ int csr2b2sr(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
  * bsrVal aligned as the planned form requires; otherwise a row-per-subgroup
  * kernel plus a hub-row kernel). Calls on one handle must therefore be
  * ordered on one stream. Asynchronous; graph-capturable after a first call
- * at the largest block-row count (which sizes the state). */
+ * at the largest block-row count (which sizes the state): growing it during
+ * a capture returns SPMM_ERR_UNSUPPORTED, and state replaced by a larger
+ * call stays allocated until spmm_destroy, so a captured graph never points
+ * at freed memory. */
 int spmm_b2sr_f32(spmm_handle_t h, int T, int M, int K, int N,
                   const int *bsrRowPtr, const int *bsrColInd,
                   const void *bsrVal, const float *B, int ldb, float *C,
@@ -315,7 +331,10 @@ int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
  * transpose, its entry permutation and its row schedule once (SYNCHRONISES);
  * the planned calls read A's values through the permutation, so val may
  * change between calls (the backward pass dX = A^T dY of learnable edge
- * weights). The drop-in forms analyse, run and free per call. */
+ * weights). The drop-in forms build the transpose into buffers the handle
+ * owns (kernels only: no synchronisation; they grow on the first call at a
+ * larger size, refused during graph capture as for spmm_csr_f32) and run the
+ * plan-free SpMM over it. */
 int spmm_csr_t_analyze(spmm_handle_t h, int M, int K, int nnz,
                        const int *rowptr, const int *colind,
                        spmm_csrt_plan_t *plan, void *stream);
