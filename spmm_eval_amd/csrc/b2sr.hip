@@ -22,7 +22,7 @@
 // then streams that list with several 16-byte B gathers in flight per lane,
 // as the CSR kernel does. Power-law graphs leave ~1 set bit per 4x4 tile,
 // so decoding a whole block row per round keeps the gathers deep.
-#include "csr_core.hpp"
+#include "b2sr_common.hpp"
 
 #include <atomic>
 #include <climits>
@@ -30,34 +30,6 @@
 #include <type_traits>
 
 namespace spmm {
-
-template <int T> struct Word;
-template <> struct Word<2> { using type = uint8_t; };
-template <> struct Word<4> { using type = uint8_t; };
-template <> struct Word<8> { using type = uint8_t; };
-template <> struct Word<16> { using type = uint16_t; };
-template <> struct Word<32> { using type = uint32_t; };
-template <> struct Word<64> { using type = uint64_t; };
-
-// A tile row's bits as a register: 64-bit only for T = 64.
-template <int T> using Bits = typename std::conditional<(T > 32), uint64_t, uint32_t>::type;
-template <typename U> __device__ __forceinline__ int popcount(U x) {
-  return sizeof(U) == 8 ? __popcll(static_cast<unsigned long long>(x))
-                        : __popc(static_cast<uint32_t>(x));
-}
-template <typename U> __device__ __forceinline__ int top_bit(U x) {  // x != 0
-  return sizeof(U) == 8 ? 63 - __clzll(static_cast<unsigned long long>(x))
-                        : 31 - __clz(static_cast<uint32_t>(x));
-}
-
-constexpr int kB2Threads = 256;
-constexpr int kB2Waves = kB2Threads / kWave;
-constexpr int kLongChunks = 2;  // block rows above this many tile chunks go to b2sr_long_kernel
-// plan-free wide tiles: block rows of at least this many tiles go first
-// (on a power-law graph: T = 16 at 256 tiles 105 us against 124 at 512 and
-// 173 at 128; T = 32 and 64 at 512 tiles 145 and 264 us against 208 and
-// 380 at 256)
-constexpr int wide_heavy(int T) { return T <= 16 ? 256 : 512; }
 
 static bool valid_T(int T) {
   return T == 2 || T == 4 || T == 8 || T == 16 || T == 32 || T == 64;
@@ -619,15 +591,6 @@ __device__ __forceinline__ int decode_rows(int t, int t1, int lane_l, int base,
   return seg[T];
 }
 
-template <int EPL>
-__device__ __forceinline__ void store_row(float *__restrict__ C, int ldc, int row, int col0,
-                                          const float (&v)[EPL]) {
-  float *Crow = C + static_cast<size_t>(row) * ldc + col0;
-#pragma unroll
-  for (int q = 0; q < EPL; q += 4)
-    *reinterpret_cast<float4 *>(Crow + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
-}
-
 // One subgroup, one block row of at most one subgroup chunk: the list is
 // streamed flat in groups of 8 gathers into ONE running sum, stored to C at
 // each row's last entry (a block row of ~40 bits costs ~5 gather round
@@ -1163,202 +1126,6 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
     if (col0 + q < N) Crow[col0 + q] = 2 * cnt[q] - deg;
 }
 
-// ------------------------------------- block-row FBF, wide tiles (T >= 16)
-// A tile is T words of T bits (32, 128 or 512 bytes) and power-law graphs
-// leave about one bit per tile, so the words, not the gathers, are most of
-// the bytes. One workgroup per block row (the plan's block-row order). Lane
-// (slot, j) -- j = tid mod T, SLOTS = 256 / T -- keeps tile row j and loads
-// row j's word of U = 16 consecutive tiles (slot s takes tiles s U ..
-// s U + U - 1 of the round: 256 / 128 / 64 tiles a round for T = 16 / 32 /
-// 64), so consecutive lanes read consecutive words. The round's set bits get
-// positions in a row-segmented LDS list (row 0's B rows, then row 1's, ...,
-// ascending in column within each row) from one strided scan of the lanes'
-// bit counts over the slots and a scan over the T row totals. Subgroup g
-// owns rows [g RPS, (g+1) RPS) and streams that contiguous stretch of the
-// list with 8 gathers in flight into per-row accumulators kept across
-// rounds. Rounds whose bits exceed the list run in windows. Every word is
-// read once, coalesced; rows are summed in ascending column order as the
-// CSR kernel sums them.
-constexpr int kWideCap = 2048;  // list entries (8 KB)
-
-template <int T, typename TB, int G, bool kBuf>
-__device__ __forceinline__ void wide_brow(int br, int2 sp, int M, const int *__restrict__ bci,
-                                          const typename Word<T>::type *__restrict__ words,
-                                          const TB *__restrict__ B, int ldb, int b_bytes,
-                                          float *__restrict__ C, int ldc) {
-  constexpr int EPL = Vec16<TB>::kElems;
-  constexpr int S = kWave / G, SW = kB2Waves * S;
-  constexpr int RPS = T >= SW ? T / SW : 1;  // rows per subgroup
-  constexpr int SLOTS = kB2Threads / T;
-  constexpr int U = 16;                      // consecutive tiles per lane per round
-  __shared__ int wtot[kB2Waves][T];
-  __shared__ int seg[T + 1];
-  __shared__ int list[kWideCap];
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const int j = tid % T, slot = tid / T;
-  const int lane_sub = lane & (G - 1), sub = lane / G, gsub = wave * S + sub;
-  const int col0 = blockIdx.y * G * EPL + lane_sub * EPL;
-  const RowLoader<TB, kBuf> ld(B, ldb, col0, b_bytes);
-  const int r0 = gsub * RPS;  // this subgroup's first row
-  const bool owner = r0 < T;
-  float acc[RPS][EPL];
-#pragma unroll
-  for (int i = 0; i < RPS; ++i)
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[i][q] = 0.f;
-
-  for (int rb = sp.x; rb < sp.y; rb += SLOTS * U) {
-    // slot s takes tiles rb + s U .. rb + s U + U - 1, so slot-major order is
-    // tile order and one count per lane places all of its entries
-    Bits<T> a[U];
-    int bc[U];
-    const int t0 = rb + slot * U;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int tt = min(t0 + u, sp.y - 1);
-      a[u] = words[static_cast<size_t>(tt) * T + j];
-      bc[u] = bci[tt];
-    }
-    int cnt = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (t0 + u >= sp.y) a[u] = 0;
-      cnt += popcount(a[u]);
-    }
-    // strided inclusive scan over the slots of this wave (lanes j, j+T, ...)
-    int inc = cnt;
-#pragma unroll
-    for (int d = T; d < kWave; d <<= 1) {
-      const int v = __shfl_up(inc, d, kWave);
-      if (lane >= d) inc += v;
-    }
-    if (lane >= kWave - T) wtot[wave][j] = inc;
-    __syncthreads();
-    int before = 0, total = 0;  // row j's entries in earlier waves; in all
-#pragma unroll
-    for (int w = 0; w < kB2Waves; ++w) {
-      const int x = wtot[w][j];
-      if (w < wave) before += x;
-      total += x;
-    }
-    if (tid < T) {  // row totals -> row segments (one wave, T <= 64 lanes)
-      int x = total;
-#pragma unroll
-      for (int d = 1; d < T; d <<= 1) {
-        const int y = __shfl_up(x, d, kWave);
-        if (lane >= d) x += y;
-      }
-      seg[j + 1] = x;
-      if (j == 0) seg[0] = 0;
-    }
-    __syncthreads();
-    const int ntot = seg[T];
-    int pos[U];  // this lane's first entry of tile u within row j's segment
-    {
-      int run = seg[j] + before + inc - cnt;
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        pos[u] = run;
-        run += popcount(a[u]);
-      }
-    }
-    int lo = 0, hi = 0;  // this subgroup's stretch of the list
-    if (owner) {
-      lo = seg[r0];
-      hi = seg[min(r0 + RPS, T)];
-    }
-    int bnd[RPS];  // row boundaries inside the stretch
-#pragma unroll
-    for (int i = 0; i < RPS; ++i) bnd[i] = owner && r0 + i < T ? seg[r0 + i] : hi;
-    for (int wb = 0; wb < ntot; wb += kWideCap) {
-      if (wb > 0) __syncthreads();  // the previous window is consumed
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        Bits<T> x = a[u];
-        int q = pos[u] - wb;
-        const int b0 = bc[u] * T + T - 1;
-        while (x) {
-          const int b = top_bit(x);
-          x ^= static_cast<Bits<T>>(1) << b;
-          if (q >= 0 && q < kWideCap) list[q] = b0 - b;
-          ++q;
-        }
-      }
-      __syncthreads();
-      const int e0 = max(lo, wb), e1 = min(hi, wb + kWideCap);
-      for (int k = e0; k < e1; k += 8) {
-        typename Vec16<TB>::Raw raw[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) raw[u] = ld(list[(k + u < e1 ? k + u : k) - wb]);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          if (k + u < e1) {
-            float b[EPL];
-            Vec16<TB>::widen(raw[u], b);
-#pragma unroll
-            for (int i = 0; i < RPS; ++i) {
-              const bool mine = k + u >= bnd[i] && (i + 1 == RPS || k + u < bnd[i + 1]);
-              if (mine) {
-#pragma unroll
-                for (int qq = 0; qq < EPL; ++qq) acc[i][qq] += b[qq];
-              }
-            }
-          }
-        }
-      }
-    }
-    __syncthreads();  // list, seg and wtot are rewritten by the next round
-  }
-  if (!owner) return;
-#pragma unroll
-  for (int i = 0; i < RPS; ++i) {
-    const int row = br * T + r0 + i;
-    if (r0 + i < T && row < M) store_row<EPL>(C, ldc, row, col0, acc[i]);
-  }
-}
-
-// Planned (kPlanFree false): workgroup x takes the plan's block row x.
-// Plan-free: the block rows of at least heavy_tiles tiles, listed by
-// class_list_kernel (state + kStateHead + MB, count state[2]), go first to
-// the E workgroups resident from the start (x, x + E, ...); workgroup E + k
-// then takes block row k unless it is listed. The last workgroup to finish
-// zeroes the list count.
-template <int T, typename TB, int G, bool kBuf, bool kPlanFree>
-__global__ void __launch_bounds__(kB2Threads)
-b2sr_wide_kernel(const int *__restrict__ brows, const int2 *__restrict__ span,
-                 const int *__restrict__ brp, int E, int heavy_tiles,
-                 int *__restrict__ state, int M, const int *__restrict__ bci,
-                 const typename Word<T>::type *__restrict__ words, const TB *__restrict__ B,
-                 int ldb, int b_bytes, float *__restrict__ C, int ldc) {
-  const int x = blockIdx.x;
-  if constexpr (!kPlanFree) {  // its own instantiation: no plan-free registers
-    wide_brow<T, TB, G, kBuf>(brows[x], span[x], M, bci, words, B, ldb, b_bytes, C, ldc);
-    return;
-  }
-  const int MB = (M + T - 1) / T;
-  if (x < E) {
-    const int n = state[2];
-    for (int j = x; j < n; j += E) {
-      const int br = state[kStateHead + MB + j];
-      wide_brow<T, TB, G, kBuf>(br, make_int2(brp[br], brp[br + 1]), M, bci, words, B, ldb,
-                                b_bytes, C, ldc);
-    }
-  } else {
-    const int br = x - E;
-    const int2 sp = make_int2(brp[br], brp[br + 1]);
-    if (sp.y - sp.x < heavy_tiles)
-      wide_brow<T, TB, G, kBuf>(br, sp, M, bci, words, B, ldb, b_bytes, C, ldc);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    // every workgroup has read the count and list before it counts itself
-    if (atomicAdd(&state[1], 1) == static_cast<int>(gridDim.x * gridDim.y) - 1) {
-      atomicExch(&state[2], 0);
-      atomicExch(&state[1], 0);
-    }
-  }
-}
-
 // ------------------------------------------------------- dense packers
 // One wave per (row, 64-column chunk): a 64-bit ballot gives two words.
 // cols == false: out[h][w] (row-major, Wc words per row);
@@ -1505,18 +1272,6 @@ static void dispatch_fbf(const Geo &g, int M, int N, const int *brp, const int *
 
 // Workgroups of `kernel` the chip holds at once (per-CU occupancy, cached
 // per kernel, times the CU count).
-static int resident_wgs(const void *kernel, int cus, std::atomic<int> *cache) {
-  int per_cu = cache->load(std::memory_order_relaxed);
-  if (per_cu == 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kB2Threads, 0) !=
-            hipSuccess ||
-        per_cu <= 0)
-      per_cu = 4;
-    cache->store(per_cu, std::memory_order_relaxed);
-  }
-  return per_cu * cus;
-}
-
 template <int T, typename TB, int G>
 static void launch_dropin(int M, int K, const int *brp, const int *bci, const void *val,
                           const TB *B, int ldb, float *C, int ldc, int tiles, int *state,
@@ -1564,10 +1319,6 @@ static void dispatch_dropin(const Geo &g, int M, int K, const int *brp, const in
 #undef SPMM_DI
 }
 
-template <int T, typename TB>
-static void dispatch_wide(const Geo &g, const spmm_csr_plan_s *p, const int *brp, int *state,
-                          int cus, int M, int K, const int *bci, const void *val, const TB *B,
-                          int ldb, float *C, int ldc, hipStream_t s);
 
 template <typename TB>
 static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp, const int *bci,
@@ -1583,9 +1334,10 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
   hipStream_t s = as_stream(stream);
   Geo g = geometry(N, sizeof(TB), B, ldb, C, ldc);
   // the planned form's kernels when bsrVal has their alignment (T bytes for
-  // T <= 8, 16 bytes above)
+  // T <= 8, 16 bytes above) and, for T > 8, B rows fit the list entries'
+  // 25 bits (above six bits of row within the block row)
   const uintptr_t va = reinterpret_cast<uintptr_t>(val);
-  const bool classed = g.G && (T > 8 ? va % 16 == 0 : va % T == 0);
+  const bool classed = g.G && (T > 8 ? va % 16 == 0 && K < (1 << 25) : va % T == 0);
   int *state = nullptr;
   int long_grid = 0, cus = 0;
   if (g.G) {
@@ -1601,14 +1353,8 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
     long_grid = 8 * cus;
   }
   if (classed && T > 8) {
-#define SPMM_WP(t) \
-  dispatch_wide<t, TB>(g, nullptr, brp, state, cus, M, K, bci, val, B, ldb, C, ldc, s)
-    switch (T) {
-      case 16: SPMM_WP(16); break;
-      case 32: SPMM_WP(32); break;
-      default: SPMM_WP(64); break;
-    }
-#undef SPMM_WP
+    b2sr_wide_launch<TB>(T, g.G, g.tiles, nullptr, brp, state, cus, M, K, bci, val, B, ldb, C, ldc,
+                         s);
     SPMM_LAUNCHED();
     return SPMM_OK;
   }
@@ -1675,53 +1421,6 @@ static void dispatch_brow(const Geo &g, const spmm_csr_plan_s *p, int M, int K, 
   }
 }
 
-// p == nullptr: the plan-free form (block rows of at least wide_heavy(T)
-// tiles listed first, on `state`).
-template <int T, typename TB, int G>
-static void launch_wide(const spmm_csr_plan_s *p, const int *brp, int *state, int cus, int K,
-                        const int *bci, const void *val, const TB *B, int ldb, float *C,
-                        int ldc, int tiles, int M, hipStream_t s) {
-  const int MB = (M + T - 1) / T;
-  const auto *w = static_cast<const typename Word<T>::type *>(val);
-  const int b_bytes = buffer_span(K, ldb, sizeof(TB));
-  auto *const kern =
-      p ? (b_bytes ? &b2sr_wide_kernel<T, TB, G, true, false>
-                   : &b2sr_wide_kernel<T, TB, G, false, false>)
-        : (b_bytes ? &b2sr_wide_kernel<T, TB, G, true, true>
-                   : &b2sr_wide_kernel<T, TB, G, false, true>);
-  int E = 0;
-  if (!p) {
-    hipLaunchKernelGGL(class_list_kernel<kB2Threads>, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0,
-                       s, MB, brp, wide_heavy(T), INT_MAX, state);
-    static std::atomic<int> per_cu[2];
-    E = std::min(resident_wgs(reinterpret_cast<const void *>(kern), cus,
-                              &per_cu[b_bytes ? 1 : 0]),
-                 MB);
-  }
-  hipLaunchKernelGGL(kern, dim3(E + MB, tiles), dim3(kB2Threads), 0, s, p ? p->d_rows : nullptr,
-                     p ? p->d_span : nullptr, brp, E, wide_heavy(T), state, M, bci, w, B, ldb,
-                     b_bytes, C, ldc);
-}
-
-// G = 64 runs as G = 32 over twice the column tiles (eight rows per subgroup
-// at most).
-template <int T, typename TB>
-static void dispatch_wide(const Geo &g, const spmm_csr_plan_s *p, const int *brp, int *state,
-                          int cus, int M, int K, const int *bci, const void *val, const TB *B,
-                          int ldb, float *C, int ldc, hipStream_t s) {
-#define SPMM_WD(gg, tl) \
-  launch_wide<T, TB, gg>(p, brp, state, cus, K, bci, val, B, ldb, C, ldc, tl, M, s)
-  switch (g.G) {
-    case 1: SPMM_WD(1, g.tiles); break;
-    case 2: SPMM_WD(2, g.tiles); break;
-    case 4: SPMM_WD(4, g.tiles); break;
-    case 8: SPMM_WD(8, g.tiles); break;
-    case 16: SPMM_WD(16, g.tiles); break;
-    default: SPMM_WD(32, g.tiles * (g.G / 32)); break;
-  }
-#undef SPMM_WD
-}
-
 // The candidate a planned call runs: `tune` when given, else the plan's
 // entry for (N, dtype), else variant 0 at the default lane count (the widest
 // power of two dividing N's 16-byte vectors). Variant 0 without the
@@ -1774,7 +1473,11 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
   SPMM_CHECK_ARG(K < (1 << 27), "planned B2SR needs K < 2^27");
   hipStream_t s = as_stream(stream);
   int variant = 0;
-  const Geo g = b2sr_candidate(p, T, N, sizeof(TB), B, ldb, C, ldc, tune, &variant);
+  Geo g = b2sr_candidate(p, T, N, sizeof(TB), B, ldb, C, ldc, tune, &variant);
+  if (T > 8 && K >= (1 << 25) && variant == 0) {  // wide list entries keep 25 bits of B row
+    variant = 1;
+    g = Geo{};
+  }
   if (variant == 2) {  // B staged in LDS (N <= 8, T <= 8)
     switch (T) {
       case 2: launch_lds<2, TB>(M, N, brp, bci, val, B, ldb, C, ldc, s); break;
@@ -1802,9 +1505,9 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     case 2: dispatch_brow<2, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 16: dispatch_wide<16, TB>(g, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s); break;
-    case 32: dispatch_wide<32, TB>(g, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s); break;
-    default: dispatch_wide<64, TB>(g, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s); break;
+    default:
+      b2sr_wide_launch<TB>(T, g.G, g.tiles, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s);
+      break;
   }
   SPMM_LAUNCHED();
   return SPMM_OK;
