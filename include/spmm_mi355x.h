@@ -54,7 +54,7 @@ typedef struct spmm_batch_plan_s *spmm_batch_plan_t;
  *   lanes_per_row : lanes that share one output row (power of two, 4..64)
  *   vec           : 16-byte vectors of B loaded per lane per gather (1 or 2)
  *   rows_scale    : multiplier on rows handled per subgroup for short rows
- *   slices        : CSR SpMM only: 2, 4, 8, 16 or 32 runs the product as that
+ *   slices        : CSR SpMM only: 2 .. 64 runs the product as that
  *                   many launches per column tile, launch s taking the
  *                   entries whose column lies in the s-th equal range of B's
  *                   rows (so one launch gathers from a slice of B small

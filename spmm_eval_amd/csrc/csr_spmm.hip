@@ -421,7 +421,7 @@ static Geometry pick_geometry(const spmm_csr_plan_s *p, int N, int elem_bytes, c
   g.VEC = vec;
   g.col_tiles = N / (G * e16 * vec);
   const int sl = t.slices;
-  g.slices = (sl == 2 || sl == 4 || sl == 8 || sl == 16 || sl == 32) && G >= 4 && G <= 32 ? sl : 1;
+  g.slices = sl >= 2 && sl <= 64 && G >= 4 && G <= 32 ? sl : 1;
   return g;
 }
 
@@ -707,8 +707,11 @@ static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int 
     const int lanes = N / (e16 * vec);
     for (int G = 64; G >= 1 && st == SPMM_OK; G >>= 1) {
       if (!tune_valid(N, e16, G, vec) || G * 4 < std::min(lanes, 64)) continue;  // <= 4 column tiles
-      for (int cand_i = 0; cand_i < 4 * 6 && st == SPMM_OK; ++cand_i) {
-        const int rs = 1 << (cand_i % 4), sl = cand_i < 4 ? 0 : 1 << (cand_i / 4);
+      // slice counts: none, then 2, 3, 4, 6, ..., 24, 32 (powers of two and
+      // the midpoints between them)
+      constexpr int kSlices[10] = {0, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+      for (int cand_i = 0; cand_i < 4 * 10 && st == SPMM_OK; ++cand_i) {
+        const int rs = 1 << (cand_i % 4), sl = kSlices[cand_i / 4];
         // column slices: only where one tile of B outgrows an XCD's L2, only
         // on a plan that can hold them, at rows_scale 1 and 2
         if (sl && (!slicing || G < 4 || G > 32 || rs > 2 ||

@@ -222,7 +222,8 @@ def test_autotune_picks_a_valid_geometry(handle, dev, bf16):
     plan = api.CsrPlan(handle, rp, g.nnz)
     C = torch.empty((g.M, N), device=dev)
     G, vec, rs, sl = plan.autotune(rp, ci, v, Bd, C)
-    assert G in (4, 8, 16, 32, 64) and vec in (1, 2) and rs in (1, 2, 4, 8) and sl in (0, 2, 4, 8, 16, 32)
+    assert G in (4, 8, 16, 32, 64) and vec in (1, 2) and rs in (1, 2, 4, 8)
+    assert sl in (0, 2, 3, 4, 6, 8, 12, 16, 24, 32)
     C1 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan)            # uses the tuned entry
     C2 = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=(G, vec, rs, sl))
     assert torch.equal(C1, C2)
@@ -231,7 +232,7 @@ def test_autotune_picks_a_valid_geometry(handle, dev, bf16):
     assert rel_fro(C1.cpu().numpy(), ref) <= 1e-5
 
 
-@pytest.mark.parametrize("S", [2, 4, 8, 16, 32])
+@pytest.mark.parametrize("S", [2, 3, 4, 6, 8, 16, 24, 32, 64])
 @pytest.mark.parametrize("tune", [(8, 1, 1), (16, 1, 2), (4, 2, 1), (32, 1, 1)])
 def test_csr_column_slices(handle, dev, S, tune):
     """Column-sliced launches (tune.slices = S): the entries of each row split
