@@ -104,11 +104,53 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
                      a.sddmm_ntile);
 }
 
+// Workgroup b runs on XCD b mod 8 (the dispatcher deals workgroups to the
+// eight XCDs in turn). Within each run of items [s0, s1) of one kind -- a
+// degree class of the forward or transposed rows, or the SDDMM chunks --
+// XCD x takes one contiguous stretch of that run, the stretches in XCD order
+// (a bijection on [s0, s1)). The union's rows ascend graph by graph within
+// a class, so an XCD then works on rows of a few graphs at a time, whose
+// feature rows its own L2 holds, instead of every eighth item of the whole
+// machine's window (eight times as many graphs); every XCD still gets an
+// eighth of each class. Only the order changes, not any item's work.
+__device__ __forceinline__ int xcd_count(int t, int x) {  // b in [0, t) with b mod 8 == x
+  return (t + 7 - x) >> 3;
+}
+__device__ __forceinline__ int xcd_stretch(int b, int s0, int s1) {
+  const int x = b & 7;
+  int v = s0 + xcd_count(b, x) - xcd_count(s0, x);
+  for (int xp = 0; xp < x; ++xp) v += xcd_count(s1, xp) - xcd_count(s0, xp);
+  return v;
+}
+// Item b of a launch table's items (wg * ctiles + tile) -> the item that
+// takes its place within b's degree class.
+__device__ __forceinline__ int xcd_stretch_table(const LaunchTable &t, int ctiles, int b) {
+  const int wg = b / ctiles;
+  int lo = 0, hi = t.nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.seg[mid].wg_begin <= wg) lo = mid;
+    else hi = mid - 1;
+  }
+  const int e = lo + 1 < t.nseg ? t.seg[lo + 1].wg_begin : t.total_wg;
+  return xcd_stretch(b, t.seg[lo].wg_begin * ctiles, e * ctiles);
+}
+
+template <int kSdd>
+__device__ __forceinline__ int xcd_item(const FusedArgs &a, int b) {
+  if (b < a.n_fwd) return xcd_stretch_table(a.fwd, a.ctiles, b);
+  int i = b - a.n_fwd;
+  if (i < a.n_bwd) return a.n_fwd + xcd_stretch_table(a.bwd, kSdd == kSddFused ? 1 : a.ctiles, i);
+  i -= a.n_bwd;
+  return a.n_fwd + a.n_bwd +
+         (kSdd == kSddRows ? xcd_stretch_table(a.sdd, 1, i) : xcd_stretch(i, 0, a.n_sddmm));
+}
+
 template <typename TB, int G, int VEC, int GS, bool kPersistent, int kSdd>
 __global__ void __launch_bounds__(kThreads) fused_step_kernel(FusedArgs a) {
   const int total = a.n_fwd + a.n_bwd + a.n_sddmm;
   if (!kPersistent) {
-    fused_item<TB, G, VEC, GS, kSdd>(a, blockIdx.x);
+    fused_item<TB, G, VEC, GS, kSdd>(a, xcd_item<kSdd>(a, blockIdx.x));
     return;
   }
   __shared__ int next;
