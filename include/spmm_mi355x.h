@@ -427,7 +427,8 @@ int spmm_mtx_read_csr(const char *path, int M, int K, int nnz, int *rowptr,
  * feature rows follow the same row offsets (square graphs). The plan holds
  * the analysed forward and transposed schedules and runs one fwd+bwd step
  * (Y = A X, dX = A^T dY, dval = SDDMM(dY, X)) as ONE fused kernel launch
- * (optionally persistent over an atomic work queue), captured as a hipGraph.
+ * (optionally persistent: a grid of 8 workgroups per CU striding over the
+ * work items), captured as a hipGraph.
  * Creation autotunes the launch geometry and SYNCHRONISES. N must be a
  * multiple of 16 (FP32 features) or 32 (BF16). */
 int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off,

@@ -5,9 +5,10 @@
 // for dX = A^T dY, and runs one fwd+bwd step -- Y = A X, dX = A^T dY,
 // dval = SDDMM(dY, X) -- as ONE kernel launch whose work items are the
 // forward CSR workgroups, the transposed CSR workgroups and the SDDMM
-// workgroups (on the forward plan's rows). In persistent mode a grid sized to the machine pulls items from
-// an atomic queue that the last finishing workgroup resets (no memset node
-// is needed when the launch is captured). The plan's autotuner times every
+// workgroups (on the forward plan's rows). In persistent mode a grid sized to
+// the machine (a multiple of eight workgroups) strides over the items, so a
+// workgroup stays on one XCD's stretch of every class and no queue or
+// counter is shared between workgroups. The plan's autotuner times every
 // (lanes per row, vector width, rows per subgroup, persistent or not)
 // candidate at creation and keeps the fastest; the step is captured as a
 // hipGraph and replayed. Each row's arithmetic is the same as in a
@@ -15,6 +16,8 @@
 // for bit.
 #include "csr_core.hpp"
 
+#include <algorithm>
+#include <atomic>
 #include <new>
 
 struct spmm_batch_plan_s {
@@ -26,7 +29,6 @@ struct spmm_batch_plan_s {
   int *trowptr = nullptr, *tcol = nullptr;  // transposed union (owned)
   float *tval = nullptr;
   int *perm = nullptr;          // transposed entry -> source entry (owned)
-  int *counters = nullptr;      // [queue, done] of the persistent mode
   spmm_csr_plan_t fwd = nullptr, bwd = nullptr;
   // chosen geometry
   int G = 0, VEC = 1, rows_scale = 1, persistent = 0, sddmm_rows = 1, grid = 0;
@@ -69,7 +71,6 @@ struct FusedArgs {
   const int *perm;
   const void *X, *dY;
   float *Y, *dX, *dval;
-  int *queue, *done;          // persistent mode only
 };
 
 // kSdd: the SDDMM form, one per kernel, so another form's registers do not
@@ -153,22 +154,11 @@ __global__ void __launch_bounds__(kThreads) fused_step_kernel(FusedArgs a) {
     fused_item<TB, G, VEC, GS, kSdd>(a, xcd_item<kSdd>(a, blockIdx.x));
     return;
   }
-  __shared__ int next;
-  while (true) {
-    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
-    __syncthreads();
-    const int item = next;
-    __syncthreads();
-    if (item >= total) break;
-    fused_item<TB, G, VEC, GS, kSdd>(a, item);
-  }
-  if (threadIdx.x == 0) {
-    // Every workgroup stops fetching before it counts itself done, so the
-    // last one can reset both counters for the next launch.
-    if (atomicAdd(a.done, 1) == static_cast<int>(gridDim.x) - 1) {
-      atomicExch(a.queue, 0);
-      atomicExch(a.done, 0);
-    }
+  // the grid is a multiple of 8, so every item b this workgroup takes runs
+  // on the XCD that xcd_item assumes for b
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    fused_item<TB, G, VEC, GS, kSdd>(a, xcd_item<kSdd>(a, b));
+    __syncthreads();  // the next item may reuse the LDS of this one
   }
 }
 
@@ -179,12 +169,24 @@ static int sddmm_G(int N, int e16) {
   return 0;
 }
 
+// grid: 8 workgroups per CU; the persistent launch takes as many as the
+// instantiation keeps resident (a multiple of 8, so each workgroup's items
+// stay on its XCD).
 template <typename TB, int G, int VEC, int GS, int kSdd>
 static void launch_fused_k(const FusedArgs &a, bool persistent, int grid, hipStream_t s) {
-  if (persistent)
-    hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, true, kSdd>), dim3(grid),
-                       dim3(kThreads), 0, s, a);
-  else
+  if (persistent) {
+    auto *const kern = &fused_step_kernel<TB, G, VEC, GS, true, kSdd>;
+    static std::atomic<int> per_cu{0};
+    int pc = per_cu.load(std::memory_order_relaxed);
+    if (pc == 0) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, reinterpret_cast<const void *>(kern),
+                                                       kThreads, 0) != hipSuccess || pc <= 0)
+        pc = 8;
+      per_cu.store(pc, std::memory_order_relaxed);
+    }
+    const int g = std::max(8, (grid / 8) * std::min(pc, 8) / 8 * 8);
+    hipLaunchKernelGGL(kern, dim3(g), dim3(kThreads), 0, s, a);
+  } else
     hipLaunchKernelGGL((fused_step_kernel<TB, G, VEC, GS, false, kSdd>),
                        dim3(a.n_fwd + a.n_bwd + a.n_sddmm), dim3(kThreads), 0, s, a);
 }
@@ -272,8 +274,6 @@ static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, in
   a.Y = Y;
   a.dX = dX;
   a.dval = dval;
-  a.queue = p->counters;
-  a.done = p->counters + 1;
   return p->bf16 ? launch_fused<uint16_t>(G, VEC, GS, a, persistent, p->grid, s)
                  : launch_fused<float>(G, VEC, GS, a, persistent, p->grid, s);
 }
@@ -374,7 +374,6 @@ int spmm_batch_plan_destroy(spmm_batch_plan_t p) {
   if (p->tcol) (void)hipFree(p->tcol);
   if (p->tval) (void)hipFree(p->tval);
   if (p->perm) (void)hipFree(p->perm);
-  if (p->counters) (void)hipFree(p->counters);
   delete p;
   return SPMM_OK;
 }
@@ -415,11 +414,8 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off, con
       hipMalloc(&p->trowptr, sizeof(int) * (static_cast<size_t>(M) + 1)) != hipSuccess ||
       hipMalloc(&p->tcol, sizeof(int) * nn) != hipSuccess ||
       hipMalloc(&p->tval, sizeof(float) * nn) != hipSuccess ||
-      hipMalloc(&p->perm, sizeof(int) * nn) != hipSuccess ||
-      hipMalloc(&p->counters, sizeof(int) * 2) != hipSuccess)
+      hipMalloc(&p->perm, sizeof(int) * nn) != hipSuccess)
     return fail(SPMM_ERR_ALLOC, "device allocation failed");
-  if (hipMemsetAsync(p->counters, 0, sizeof(int) * 2, s) != hipSuccess)
-    return fail(SPMM_ERR_HIP, "counter init");
   int *d_off = static_cast<int *>(workspace(h, sizeof(int) * 2 * (static_cast<size_t>(ngraphs) + 1)));
   if (!d_off) return fail(SPMM_ERR_ALLOC, "workspace");
   if (hipMemcpyAsync(d_off, row_off, sizeof(int) * (ngraphs + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -446,7 +442,7 @@ int spmm_batch_plan_create(spmm_handle_t h, int ngraphs, const int *row_off, con
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return fail(SPMM_ERR_HIP, "device query");
-  p->grid = cus * 8;  // 8 workgroups of 4 waves per CU = 32 waves per CU
+  p->grid = cus * 8;  // 8 workgroups of 4 waves per CU = 32 waves per CU (a multiple of 8)
   st = tune_plan(p);
   if (st) { spmm_batch_plan_destroy(p); return st; }
   *out = p;

@@ -99,6 +99,9 @@ def test_batched_sddmm_forms(handle, dev, batch, bf16):
             out[(form, pers)] = (Y, dX, dval)
     for key, (Y, dX, dval) in out.items():
         assert torch.equal(Y, out[(0, False)][0]), key
+        assert torch.equal(dX, out[(0, False)][1]), key
+        # the persistent launch runs the same items: the same bits as the one-shot launch
+        assert torch.equal(dval, out[(key[0], False)][2]), key
     assert torch.equal(out[(1, False)][2], out[(0, False)][2])
     Y2, dX2, dv2 = out[(2, False)]
     assert torch.equal(dX2, out[(0, False)][1])  # same transposed rows, same sums
