@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <atomic>
 #include <new>
+#include <vector>
 
 struct spmm_batch_plan_s {
   spmm_handle_s *h = nullptr;
@@ -38,6 +39,13 @@ struct spmm_batch_plan_s {
   hipGraphExec_t exec = nullptr;
   const void *gX = nullptr, *gdY = nullptr;  // operands the graph was captured with
   float *gY = nullptr, *gdX = nullptr, *gdval = nullptr;
+  // item order of the fused launch per geometry (built on the host once)
+  struct Order {
+    int G = 0, VEC = 0, rows_scale = 0, sddmm_rows = -1;
+    int *d = nullptr;
+    int n_first = 0;  // positions of the first phase
+  };
+  std::vector<Order> orders;
 };
 
 namespace spmm {
@@ -71,6 +79,8 @@ struct FusedArgs {
   const int *perm;
   const void *X, *dY;
   float *Y, *dX, *dval;
+  const int *order;           // launch position -> item (see merged_item_order)
+  int n_first;                // positions of the first phase (forward and SDDMM items)
 };
 
 // kSdd: the SDDMM form, one per kernel, so another form's registers do not
@@ -105,15 +115,18 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, int item) {
                      a.sddmm_ntile);
 }
 
-// Workgroup b runs on XCD b mod 8 (the dispatcher deals workgroups to the
-// eight XCDs in turn). Within each run of items [s0, s1) of one kind -- a
-// degree class of the forward or transposed rows, or the SDDMM chunks --
-// XCD x takes one contiguous stretch of that run, the stretches in XCD order
-// (a bijection on [s0, s1)). The union's rows ascend graph by graph within
-// a class, so an XCD then works on rows of a few graphs at a time, whose
-// feature rows its own L2 holds, instead of every eighth item of the whole
-// machine's window (eight times as many graphs); every XCD still gets an
-// eighth of each class. Only the order changes, not any item's work.
+// Launch order. Workgroup b runs on XCD b mod 8 (the dispatcher deals
+// workgroups to the eight XCDs in turn). The plan merges the items of every
+// run -- each degree class of the forward and transposed rows, the SDDMM
+// rows or chunks -- by their fractional position in their run
+// (merged_item_order); rows ascend graph by graph within every run, so
+// consecutive positions of the merged order belong to the same few graphs
+// in every part. XCD x then takes one contiguous stretch of the merged order
+// (xcd_stretch: a bijection on [s0, s1) that gives XCD x the x-th stretch):
+// at any time an XCD works on a few graphs, in all parts and classes at once,
+// and their feature rows X and dY stay in its L2, read past it about once;
+// every XCD gets a share of every class. Only the order changes, not any
+// item's work.
 __device__ __forceinline__ int xcd_count(int t, int x) {  // b in [0, t) with b mod 8 == x
   return (t + 7 - x) >> 3;
 }
@@ -123,41 +136,21 @@ __device__ __forceinline__ int xcd_stretch(int b, int s0, int s1) {
   for (int xp = 0; xp < x; ++xp) v += xcd_count(s1, xp) - xcd_count(s0, xp);
   return v;
 }
-// Item b of a launch table's items (wg * ctiles + tile) -> the item that
-// takes its place within b's degree class.
-__device__ __forceinline__ int xcd_stretch_table(const LaunchTable &t, int ctiles, int b) {
-  const int wg = b / ctiles;
-  int lo = 0, hi = t.nseg - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (t.seg[mid].wg_begin <= wg) lo = mid;
-    else hi = mid - 1;
-  }
-  const int e = lo + 1 < t.nseg ? t.seg[lo + 1].wg_begin : t.total_wg;
-  return xcd_stretch(b, t.seg[lo].wg_begin * ctiles, e * ctiles);
-}
-
-template <int kSdd>
-__device__ __forceinline__ int xcd_item(const FusedArgs &a, int b) {
-  if (b < a.n_fwd) return xcd_stretch_table(a.fwd, a.ctiles, b);
-  int i = b - a.n_fwd;
-  if (i < a.n_bwd) return a.n_fwd + xcd_stretch_table(a.bwd, kSdd == kSddFused ? 1 : a.ctiles, i);
-  i -= a.n_bwd;
-  return a.n_fwd + a.n_bwd +
-         (kSdd == kSddRows ? xcd_stretch_table(a.sdd, 1, i) : xcd_stretch(i, 0, a.n_sddmm));
-}
 
 template <typename TB, int G, int VEC, int GS, bool kPersistent, int kSdd>
 __global__ void __launch_bounds__(kThreads) fused_step_kernel(FusedArgs a) {
   const int total = a.n_fwd + a.n_bwd + a.n_sddmm;
+  auto pos = [&](int b) {  // each phase stretched over the XCDs on its own
+    return b < a.n_first ? xcd_stretch(b, 0, a.n_first) : xcd_stretch(b, a.n_first, total);
+  };
   if (!kPersistent) {
-    fused_item<TB, G, VEC, GS, kSdd>(a, xcd_item<kSdd>(a, blockIdx.x));
+    fused_item<TB, G, VEC, GS, kSdd>(a, a.order[pos(blockIdx.x)]);
     return;
   }
-  // the grid is a multiple of 8, so every item b this workgroup takes runs
-  // on the XCD that xcd_item assumes for b
+  // the grid is a multiple of 8, so every position b this workgroup takes
+  // runs on the XCD that xcd_stretch assumes for b
   for (int b = blockIdx.x; b < total; b += gridDim.x) {
-    fused_item<TB, G, VEC, GS, kSdd>(a, xcd_item<kSdd>(a, b));
+    fused_item<TB, G, VEC, GS, kSdd>(a, a.order[pos(b)]);
     __syncthreads();  // the next item may reuse the LDS of this one
   }
 }
@@ -233,12 +226,12 @@ static int launch_fused(int G, int VEC, int GS, const FusedArgs &a, bool persist
   }
 }
 
-static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, int persistent,
-                        int sddmm_rows, const void *X, const void *dY, float *Y, float *dX,
-                        float *dval, hipStream_t s) {
+// The launch tables and item counts of one geometry.
+static int fill_args(const spmm_batch_plan_s *p, int G, int VEC, int rows_scale, int sddmm_rows,
+                     FusedArgs *ap, int *GS_out) {
+  FusedArgs &a = *ap;
   const int elem = p->bf16 ? 2 : 4;
   const int e16 = 16 / elem;
-  FusedArgs a;
   build_table(p->fwd->counts, p->fwd->offs, G, rows_scale, &a.fwd);
   build_table(p->bwd->counts, p->bwd->offs, G, rows_scale, &a.bwd);
   a.ctiles = p->N / (G * e16 * VEC);
@@ -255,6 +248,95 @@ static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, in
                   : static_cast<int>(ceil_div_ll(
                         p->nnz, static_cast<long long>(kThreads / GS) * kSddmmChunk));
   a.sddmm_ntile = p->N / (GS * e16);
+  *GS_out = GS;
+  return SPMM_OK;
+}
+
+// The merged order of a launch's items in two phases: first the runs that
+// gather X (each degree class of the forward table; the SDDMM table's
+// classes or its chunks), then the runs that gather dY (the transposed
+// table's classes). Within a phase every run contributes its items at
+// fractional positions (j + 1/2) / n, sorted by position (exact:
+// cross-multiplied), ties in run order. Returns the first phase's length.
+static int merged_item_order(const FusedArgs &a, std::vector<int> *out) {
+  struct Run { int first, n; };
+  auto table_runs = [](std::vector<Run> &runs, const LaunchTable &t, int ct, int off) {
+    for (int i = 0; i < t.nseg; ++i) {
+      const int b = t.seg[i].wg_begin, e = i + 1 < t.nseg ? t.seg[i + 1].wg_begin : t.total_wg;
+      if (e > b) runs.push_back({off + b * ct, (e - b) * ct});
+    }
+  };
+  auto merge = [&](const std::vector<Run> &runs) {
+    struct It { int run, j; };
+    std::vector<It> items;
+    for (int r = 0; r < static_cast<int>(runs.size()); ++r)
+      for (int j = 0; j < runs[r].n; ++j) items.push_back({r, j});
+    std::stable_sort(items.begin(), items.end(), [&](const It &x, const It &y) {
+      // (2 jx + 1) / nx < (2 jy + 1) / ny
+      return (2ll * x.j + 1) * runs[y.run].n < (2ll * y.j + 1) * runs[x.run].n;
+    });
+    for (const It &it : items) out->push_back(runs[it.run].first + it.j);
+  };
+  out->clear();
+  out->reserve(static_cast<size_t>(a.n_fwd) + a.n_bwd + a.n_sddmm);
+  std::vector<Run> first, second;
+  table_runs(first, a.fwd, a.ctiles, 0);
+  if (a.sddmm_rows == kSddRows) table_runs(first, a.sdd, 1, a.n_fwd + a.n_bwd);
+  else if (a.n_sddmm > 0) first.push_back({a.n_fwd + a.n_bwd, a.n_sddmm});
+  table_runs(second, a.bwd, a.ctiles, a.n_fwd);
+  merge(first);
+  const int n_first = static_cast<int>(out->size());
+  merge(second);
+  return n_first;
+}
+
+// The plan's device copy of a geometry's merged order, built on first use
+// (allocates and synchronises; never while s is being captured).
+static int item_order(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, int sddmm_rows,
+                      FusedArgs *a, hipStream_t s) {
+  for (const auto &o : p->orders)
+    if (o.G == G && o.VEC == VEC && o.rows_scale == rows_scale && o.sddmm_rows == sddmm_rows) {
+      a->order = o.d;
+      a->n_first = o.n_first;
+      return SPMM_OK;
+    }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  SPMM_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    SPMM_FAIL(SPMM_ERR_UNSUPPORTED, "launch order not prepared before capture");
+  std::vector<int> h;
+  const int n_first = merged_item_order(*a, &h);
+  spmm_batch_plan_s::Order o;
+  o.n_first = n_first;
+  o.G = G;
+  o.VEC = VEC;
+  o.rows_scale = rows_scale;
+  o.sddmm_rows = sddmm_rows;
+  if (hipMalloc(&o.d, sizeof(int) * std::max<size_t>(1, h.size())) != hipSuccess)
+    SPMM_FAIL(SPMM_ERR_ALLOC, "hipMalloc of the launch order failed");
+  hipError_t e = h.empty() ? hipSuccess
+                           : hipMemcpyAsync(o.d, h.data(), sizeof(int) * h.size(),
+                                            hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(o.d);
+    SPMM_FAIL(SPMM_ERR_HIP, std::string("launch order upload: ") + hipGetErrorString(e));
+  }
+  p->orders.push_back(o);
+  a->order = o.d;
+  a->n_first = n_first;
+  return SPMM_OK;
+}
+
+static int enqueue_step(spmm_batch_plan_s *p, int G, int VEC, int rows_scale, int persistent,
+                        int sddmm_rows, const void *X, const void *dY, float *Y, float *dX,
+                        float *dval, hipStream_t s) {
+  FusedArgs a;
+  int GS = 0;
+  int st = fill_args(p, G, VEC, rows_scale, sddmm_rows, &a, &GS);
+  if (st) return st;
+  st = item_order(p, G, VEC, rows_scale, sddmm_rows, &a, s);
+  if (st) return st;
   a.M = p->M;
   a.N = p->N;
   a.nnz = p->nnz;
@@ -283,6 +365,30 @@ static void destroy_graph(spmm_batch_plan_s *p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->exec = nullptr;
   p->graph = nullptr;
+}
+
+// The launch order of the plan's current geometry, built now if missing (so
+// that neither a capture nor an eager step has to).
+static int prepare_order(spmm_batch_plan_s *p) {
+  FusedArgs a;
+  int GS = 0;
+  const int st = fill_args(p, p->G, p->VEC, p->rows_scale, p->sddmm_rows, &a, &GS);
+  if (st) return st;
+  return item_order(p, p->G, p->VEC, p->rows_scale, p->sddmm_rows, &a, p->cap);
+}
+
+// Frees the launch orders of every geometry but the current one.
+static void prune_orders(spmm_batch_plan_s *p) {
+  (void)hipStreamSynchronize(p->cap);
+  std::vector<spmm_batch_plan_s::Order> keep;
+  for (auto &o : p->orders) {
+    if (o.G == p->G && o.VEC == p->VEC && o.rows_scale == p->rows_scale &&
+        o.sddmm_rows == p->sddmm_rows)
+      keep.push_back(o);
+    else
+      (void)hipFree(o.d);
+  }
+  p->orders.swap(keep);
 }
 
 // Times each geometry on scratch operands (zeros: the gather pattern, not
@@ -354,6 +460,7 @@ static int tune_plan(spmm_batch_plan_s *p) {
   if (st) return st;
   if (p->G == 0) return SPMM_ERR_UNSUPPORTED;
   p->tuned_ms = best / 3.0f;
+  prune_orders(p);
   return SPMM_OK;
 }
 
@@ -374,6 +481,7 @@ int spmm_batch_plan_destroy(spmm_batch_plan_t p) {
   if (p->tcol) (void)hipFree(p->tcol);
   if (p->tval) (void)hipFree(p->tval);
   if (p->perm) (void)hipFree(p->perm);
+  for (auto &o : p->orders) (void)hipFree(o.d);
   delete p;
   return SPMM_OK;
 }
@@ -480,7 +588,7 @@ int spmm_batch_plan_set(spmm_batch_plan_t p, const spmm_tune_t *tune, int persis
   p->rows_scale = rs;
   p->persistent = persistent;
   p->sddmm_rows = sddmm_form;
-  return SPMM_OK;
+  return prepare_order(p);  // before any step of it is captured
 }
 
 int spmm_batch_fwd_bwd(spmm_batch_plan_t p, const void *X, const void *dY, float *Y, float *dX,
@@ -496,6 +604,8 @@ int spmm_batch_fwd_bwd(spmm_batch_plan_t p, const void *X, const void *dY, float
   }
   if (!p->exec || p->gX != X || p->gdY != dY || p->gY != Y || p->gdX != dX || p->gdval != dval) {
     destroy_graph(p);
+    const int po = prepare_order(p);
+    if (po) SPMM_FAIL(po, "launch order");
     // Capture only records the launch; the graph runs on s below.
     SPMM_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
     const int st = enqueue_step(p, p->G, p->VEC, p->rows_scale, p->persistent, p->sddmm_rows, X,
