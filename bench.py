@@ -658,6 +658,13 @@ def extra_batched(args, h, dev):
             api.sddmm_csr(h, rp, ci, dY[r0:r1], X[r0:r1], out=dval[e0:e1])
 
     ms_sep, _ = median_batches(separate, launches=3, batches=5, warmup=1)
+    # the same 1,536 launches captured in one hipGraph: device time without
+    # the host's pacing of the calls
+    try:
+        ms_sep_graph = round(time_graph(separate, Y, steps=1, replays=5), 3)
+    except Exception as e:  # noqa: BLE001 -- reported, not fatal
+        log(f"separate launches not capturable: {e}")
+        ms_sep_graph = None
     info = plan.info()
     forms = {}
     for form in (0, 1, 2):
@@ -684,6 +691,8 @@ def extra_batched(args, h, dev):
         "bf16_one_graph_ms_median_of_5": round(ms_graph, 4),
         "f32_separate_launches_ms_median_of_5": round(ms_sep, 3),
         "speedup": round(ms_sep / ms_graph, 2),
+        "f32_separate_launches_one_graph_ms_median_of_5": ms_sep_graph,
+        "speedup_vs_captured_separate": round(ms_sep_graph / ms_graph, 2) if ms_sep_graph else None,
         "cpu_f32_union_step_ms": round(cms, 2), "cpu_threads": oracle.omp_threads()}}
 
 
