@@ -469,7 +469,7 @@ def extra_b2sr_grid(args, h, dev):
     rp, ci = torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev)
     v = torch.ones(g.nnz, device=dev)
     cp = api.CsrPlan(h, rp, g.nnz)
-    widths = (1, 32, 64, 128)
+    widths = tuple(int(x) for x in args.grid_widths.split(","))
     Bs = {N: torch.from_numpy(graphs.uniform_pm1((g.K, N), seed=N + 17)).to(dev) for N in widths}
     Cs = {N: torch.empty((g.M, N), device=dev) for N in widths}
     out = {"csr_us": {}, "b2sr_us": {}, "b2sr_tuned_us": {}, "b2sr_tuned_choice": {}}
@@ -477,13 +477,15 @@ def extra_b2sr_grid(args, h, dev):
         ms = time_graph(lambda: api.spmm_csr(h, rp, ci, v, Bs[N], C_out=Cs[N], plan=cp), Cs[N],
                         steps=50, replays=5)
         out["csr_us"][f"N{N}"] = round(ms * 1e3, 2)
-    for T in (2, 4, 8, 16, 32, 64):
+    for T in (int(x) for x in args.grid_tiles.split(",")):
         brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
         plan = api.B2srPlan(h, T, g.M, brp)
         for N in widths:
             ms = time_graph(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bs[N], C_out=Cs[N],
                                                   plan=plan), Cs[N], steps=50, replays=5)
             out["b2sr_us"][f"T{T}_N{N}"] = round(ms * 1e3, 2)
+            if args.grid_no_tune:
+                continue
             # the autotuner's pick among the lane mappings (variant, lanes, rows_scale)
             choice = plan.autotune(brp, bci, words, Bs[N], Cs[N])
             ms = time_graph(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bs[N], C_out=Cs[N],
@@ -820,6 +822,12 @@ def main():
     ap.add_argument("--only", choices=["csr_small", "fwd_bwd", "batched", "spmv_aligned", "b2sr",
                                        "b2sr_grid", "bmm", "gather_calib"], default=None,
                     help="run one extra workload alone and print its result (for profiling)")
+    ap.add_argument("--grid-tiles", default="2,4,8,16,32,64",
+                    help="tile sizes of the B2SR grid (comma list)")
+    ap.add_argument("--grid-widths", default="1,32,64,128",
+                    help="widths N of the B2SR grid (comma list)")
+    ap.add_argument("--grid-no-tune", action="store_true",
+                    help="B2SR grid: the plan's default lane mapping only (profiling one kernel)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
