@@ -41,13 +41,17 @@ __device__ __forceinline__ T ld_once(const T *p) {
 // of G entries at stride `chunk_stride`, starting at chunk `first`. With
 // kPerm the value of entry e is val[perm[e]] (a transposed structure whose
 // values stay in the source order, so learnable values need no re-gather).
-template <typename TB, int G, int EPL, bool kPerm = false, bool kNT = false>
+// kUF: B-row loads in flight per lane (0: 8, or 4 at more than 16 elements a
+// lane). Fewer, 4, suit the column-sliced launches, whose slice of B mostly
+// hits L2: at 4 the 233k-row products run 1.5% (FP32) to 4% (BF16) faster
+// than at 8, while the unsliced kernel on an 89k-row graph runs 4% slower.
+template <typename TB, int G, int EPL, bool kPerm = false, bool kNT = false, int kUF = 0>
 __device__ __forceinline__ void accumulate_range(
     int e_begin, int e_end, int first_chunk_off, int chunk_stride, int lane_sub,
     const int *__restrict__ colind, const float *__restrict__ val,
     const TB *__restrict__ Bc, int ldb, float (&acc)[EPL],
     const int *__restrict__ perm = nullptr) {
-  constexpr int U0 = EPL > 16 ? 4 : 8;  // loads in flight per lane (8 x 32 B for BF16 at 2 vectors)
+  constexpr int U0 = kUF ? kUF : (EPL > 16 ? 4 : 8);  // (8 x 32 B for BF16 at 2 vectors)
   constexpr int U = G < U0 ? G : U0;
   constexpr int V = EPL / Vec16<TB>::kElems;
   int e = e_begin + first_chunk_off;
@@ -232,7 +236,8 @@ class_list_kernel(int n, const int *__restrict__ ptr, int a_limit, int c_limit,
 // One row of each class (the per-row code of the planned kernel and of the
 // plan-free kernel, so both give the same bits at the same lane geometry).
 // Class A: one subgroup of G lanes sums the row and stores it.
-template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false,
+          int kUF = 0>
 __device__ __forceinline__ void csr_row_sub(int row, int2 sp, int lane_sub, int col0,
                                             const int *__restrict__ colind,
                                             const float *__restrict__ val,
@@ -244,7 +249,7 @@ __device__ __forceinline__ void csr_row_sub(int row, int2 sp, int lane_sub, int 
   float acc[EPL];
 #pragma unroll
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-  accumulate_range<TB, G, EPL, kPerm, kNT>(sp.x, sp.y, 0, G, lane_sub, colind, val, Bc, ldb, acc,
+  accumulate_range<TB, G, EPL, kPerm, kNT, kUF>(sp.x, sp.y, 0, G, lane_sub, colind, val, Bc, ldb, acc,
                                            perm);
   put_row<G, E, EPL, kAcc>(C + static_cast<size_t>(row) * ldc + col0, acc);
 }
@@ -252,7 +257,8 @@ __device__ __forceinline__ void csr_row_sub(int row, int2 sp, int lane_sub, int 
 // Class B: the S subgroups of one wave split the row's chunks of G entries
 // (subgroup s takes chunks s, s + S, ...), then a butterfly; subgroup 0
 // stores.
-template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false,
+          int kUF = 0>
 __device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane_sub, int col0,
                                              const int *__restrict__ colind,
                                              const float *__restrict__ val,
@@ -265,7 +271,7 @@ __device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane
   float acc[EPL];
 #pragma unroll
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
-  accumulate_range<TB, G, EPL, kPerm, kNT>(sp.x, sp.y, sub * G, S * G, lane_sub, colind, val, Bc,
+  accumulate_range<TB, G, EPL, kPerm, kNT, kUF>(sp.x, sp.y, sub * G, S * G, lane_sub, colind, val, Bc,
                                            ldb, acc, perm);
   wave_reduce_subgroups<G, EPL>(acc);
   if (sub == 0) put_row<G, E, EPL, kAcc>(C + static_cast<size_t>(row) * ldc + col0, acc);
@@ -274,7 +280,8 @@ __device__ __forceinline__ void csr_row_wave(int row, int2 sp, int sub, int lane
 // Class C: the 4 S subgroups of one workgroup split the row, the waves meet
 // in LDS in a fixed order; wave 0's subgroup 0 stores. Every thread of the
 // workgroup must call it (two barriers).
-template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false>
+template <typename TB, int G, int VEC, bool kPerm, bool kAcc = false, bool kNT = false,
+          int kUF = 0>
 __device__ __forceinline__ void csr_row_wg(int row, int2 sp, int wave, int sub, int lane_sub,
                                            int col0, const int *__restrict__ colind,
                                            const float *__restrict__ val,
@@ -290,7 +297,7 @@ __device__ __forceinline__ void csr_row_wg(int row, int2 sp, int wave, int sub, 
 #pragma unroll
   for (int q = 0; q < EPL; ++q) acc[q] = 0.f;
   const int gsub = wave * S + sub;
-  accumulate_range<TB, G, EPL, kPerm, kNT>(sp.x, sp.y, gsub * G, kWavesPerWG * S * G, lane_sub,
+  accumulate_range<TB, G, EPL, kPerm, kNT, kUF>(sp.x, sp.y, gsub * G, kWavesPerWG * S * G, lane_sub,
                                            colind, val, Bc, ldb, acc, perm);
   wave_reduce_subgroups<G, EPL>(acc);
   if (wave > 0 && sub == 0) {

@@ -216,7 +216,9 @@ slice_offsets_kernel(int M, int S, int W, const int2 *__restrict__ span,
 // ((p_0 + p_1) + p_2) + ... in slice order, the same at every call. Launches
 // run in slice order on one stream, so no two of them touch C at once.
 // colind, val and C are read (and C written) once per launch, with the
-// non-temporal policy, so the slice of B keeps the caches.
+// non-temporal policy, so the slice of B keeps the caches. Four B rows are
+// in flight per lane rather than the unsliced kernel's eight.
+constexpr int kSlicedInFlight = 4;
 template <typename TB, int G, int VEC, bool kAcc>
 __global__ void __launch_bounds__(kThreads)
 csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
@@ -253,7 +255,7 @@ csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
         sp_n = make_int2(lo[pos_n], hi[pos_n]);
       }
       if (!kAcc || sp.x < sp.y)
-        csr_row_sub<TB, G, VEC, false, kAcc, true>(row, sp, lane_sub, col0, colind, val, Bc, ldb, C,
+        csr_row_sub<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(row, sp, lane_sub, col0, colind, val, Bc, ldb, C,
                                              ldc, nullptr);
       row = row_n;
       sp = sp_n;
@@ -264,13 +266,13 @@ csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
     if (pos >= sg.list_off + sg.count) return;  // whole wave
     const int2 sp = make_int2(lo[pos], hi[pos]);
     if (kAcc && sp.x == sp.y) return;
-    csr_row_wave<TB, G, VEC, false, kAcc, true>(rows[pos], sp, sub, lane_sub, col0, colind, val, Bc,
+    csr_row_wave<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(rows[pos], sp, sub, lane_sub, col0, colind, val, Bc,
                                           ldb, C, ldc, nullptr);
   } else {
     const int pos = sg.list_off + local_wg;
     const int2 sp = make_int2(lo[pos], hi[pos]);
     if (kAcc && sp.x == sp.y) return;  // whole workgroup
-    csr_row_wg<TB, G, VEC, false, kAcc, true>(rows[pos], sp, wave, sub, lane_sub, col0, colind, val, Bc,
+    csr_row_wg<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(rows[pos], sp, wave, sub, lane_sub, col0, colind, val, Bc,
                                         ldb, C, ldc, nullptr);
   }
 }
