@@ -117,6 +117,12 @@ int spmm_csr_plan_bins(spmm_csr_plan_t plan, int *counts32);
  * in state the handle owns), then one kernel runs them with the planned
  * kernel's per-row code, so the result equals the planned form's at the same
  * lane geometry bit for bit (the default geometry when neither is tuned).
+ * Large graphs (one column tile of B of about 15 MB or more, rows averaging
+ * at least 16 entries a slice) run column-sliced: per-row slice offsets
+ * built on the device each call, then one launch per slice of about 10 MB
+ * of B; the result equals the planned form at spmm_csr_dropin_geometry's
+ * geometry bit for bit, and rows whose columns do not ascend are detected
+ * on the device and summed whole in the first launch instead.
  * The handle's state grows (hipMalloc) on the first call at a larger M;
  * growing while the stream is being captured returns SPMM_ERR_UNSUPPORTED,
  * so a caller that captures makes one call at the largest size first. Calls
@@ -129,6 +135,14 @@ int spmm_csr_f32(spmm_handle_t h, int M, int K, int N, int nnz,
 int spmm_csr_bf16(spmm_handle_t h, int M, int K, int N, int nnz,
                   const int *rowptr, const int *colind, const float *val,
                   const uint16_t *B, int ldb, float *C, int ldc, void *stream);
+
+/* The lane geometry the drop-in above uses for these sizes, with B
+ * elements of elem_bytes (4: FP32, 2: BF16) and 16-byte aligned B, C, ldb and
+ * ldc: lanes_per_row and vec (0 when N is not a multiple of one 16-byte
+ * vector: the narrow kernel), rows_scale 1, slices (0: one launch). Host
+ * only. No counterpart in the reference, whose SpMM is cuSPARSE's; it lets a
+ * caller reproduce a drop-in result with the planned form. */
+int spmm_csr_dropin_geometry(int M, int K, int N, int nnz, int elem_bytes, spmm_tune_t *out);
 
 /* Planned form: fully asynchronous, no allocation. `tune` may be NULL. */
 int spmm_csr_f32_planned(spmm_csr_plan_t plan, int M, int K, int N, int nnz,

@@ -49,6 +49,7 @@ _SIGS = {
     "spmm_csr_plan_bins": [_vp, _ip],
     "spmm_csr_f32": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "spmm_csr_bf16": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
+    "spmm_csr_dropin_geometry": [_i, _i, _i, _i, _i, C.POINTER(Tune)],
     "spmm_csr_f32_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_csr_bf16_planned": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp],
     "spmm_csr_autotune": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, C.POINTER(Tune), _vp],

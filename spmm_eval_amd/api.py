@@ -178,6 +178,15 @@ def spmm_csr(h: Handle, rowptr, colind, val, B, K: int | None = None, C_out=None
     return C_out
 
 
+def dropin_geometry(M: int, K: int, N: int, nnz: int, bf16: bool = False) -> tuple:
+    """The (lanes_per_row, vec, rows_scale, slices) that the plan-free
+    spmm_csr runs at for these sizes with aligned operands; slices 0 is one
+    launch, lanes 0 the narrow kernel."""
+    t = Tune()
+    call("spmm_csr_dropin_geometry", M, K, N, nnz, 2 if bf16 else 4, C.byref(t))
+    return (t.lanes_per_row, t.vec, t.rows_scale, t.slices)
+
+
 def csr2b2sr(h: Handle, rowptr, colind, M: int, K: int, T: int, stream=None):
     """Device CSR -> B2SR-T (bsrRowPtr, bsrColInd, bsrVal words)."""
     MB = (M + T - 1) // T

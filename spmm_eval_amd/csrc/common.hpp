@@ -70,6 +70,9 @@ struct spmm_handle_s {
   // the plan-free transposed SpMM's structure: trowptr | tcol | perm
   int *tstruct = nullptr;
   size_t tstruct_cap = 0;  // ints
+  // the plan-free column-sliced SpMM's per-row slice offsets, (S + 1) x M
+  int *slice_off = nullptr;
+  size_t slice_off_cap = 0;  // ints
   // buffers replaced by larger ones, freed by spmm_destroy: work queued
   // earlier, or a graph captured from an earlier call, may still use them
   std::vector<void *> retired;

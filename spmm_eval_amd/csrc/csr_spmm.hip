@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <new>
 #include <vector>
 
@@ -173,6 +174,189 @@ csr_dropin_kernel(int M, int E, int a_limit, const int *__restrict__ rowptr,
   }
 }
 
+// ------------------------------------------------ plan-free column slices
+// The plan-free form of the column-sliced product (csr_sliced_kernel's
+// launches without a plan): per-call slice offsets, then S launches over the
+// classes class_list_kernel found, slice 0 storing C and the others adding
+// into it.
+constexpr int kSlicedInFlight = 4;  // B-row loads in flight per lane, sliced launches
+constexpr int kUnsortedFlag = 4;  // list-state slot: a row's columns descend
+
+// Slice offsets of every row. A wave takes blocks of kOffRows rows (lane l
+// reads row r0 + l's span once); row by row, the lanes stream the row's
+// colind 8 x 64 entries a round, the next row's first round requested
+// before this row's is scanned. Entry e is the first of slice t for every t
+// in (slice(e - 1), slice(e)], slice(c) = min(c / W, S - 1); so
+// off[t * M + r] is the first entry of row r in slice t, off[0 * M + r] and
+// off[S * M + r] its span, and slices after the row's last entry start at
+// its end. Every (t, r) is set once when the columns ascend; a descending
+// pair sets state[kUnsortedFlag] instead (the slice launches then run whole
+// rows in slice 0). The block's offsets gather in LDS and leave as one
+// coalesced store of kOffRows ints per slice (scattered 4-byte stores, one
+// per (row, slice), made the kernel three times slower).
+constexpr int kOffChunks = 8;
+constexpr int kOffRows = 16;
+constexpr int kOffMaxS = 32;  // slices of the plan-free form at most
+
+constexpr int kDppWaveShr1 = 0x138;  // DPP wave_shr:1: lane l reads lane l - 1
+
+// min(c / W, S - 1) for c >= 0 (negatives as 0) by a float reciprocal and a
+// one-step correction: below S <= 32 slices the float quotient is within
+// 32 * 2^-22 of the exact one, so one step up or down makes it exact, and a
+// larger quotient only has to stay >= S - 1.
+__device__ __forceinline__ int slice_of(int c, int W, float invW, int S) {
+  const unsigned cc = static_cast<unsigned>(max(c, 0));
+  int q = static_cast<int>(static_cast<float>(cc) * invW);
+  q = min(q, S);  // the correction below stays in 32 bits
+  if (static_cast<unsigned>(q) * static_cast<unsigned>(W) > cc) --q;
+  else if (static_cast<unsigned>(q + 1) * static_cast<unsigned>(W) <= cc) ++q;
+  return min(q, S - 1);
+}
+
+__device__ __forceinline__ void off_round(const int *__restrict__ colind, int e, int e1,
+                                          int lane, int (&c)[kOffChunks]) {
+#pragma unroll
+  for (int u = 0; u < kOffChunks; ++u) {
+    const int ei = e + u * kWave + lane;
+    c[u] = ei < e1 ? __builtin_nontemporal_load(colind + ei) : 0;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+dropin_slice_offsets_kernel(int M, int S, int W, const int *__restrict__ rowptr,
+                            const int *__restrict__ colind, int *__restrict__ off,
+                            int *__restrict__ state) {
+  __shared__ int soff[kWavesPerWG][kOffMaxS + 1][kOffRows];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int waves = gridDim.x * kWavesPerWG;
+  const size_t Ms = static_cast<size_t>(M);
+  const float invW = 1.0f / static_cast<float>(W);
+  int(*const so)[kOffRows] = soff[wave];
+  bool bad = false;
+  for (int r0 = (blockIdx.x * kThreads + threadIdx.x) / kWave * kOffRows; r0 < M;
+       r0 += waves * kOffRows) {
+    const int nr = min(kOffRows, M - r0);
+    const int my0 = lane < nr ? rowptr[r0 + lane] : 0;
+    const int my1 = lane < nr ? rowptr[r0 + lane + 1] : 0;
+    int c[kOffChunks], cn[kOffChunks];
+    off_round(colind, __shfl(my0, 0), __shfl(my1, 0), lane, cn);
+    for (int j = 0; j < nr; ++j) {
+      const int e0 = __shfl(my0, j), e1 = __shfl(my1, j);
+#pragma unroll
+      for (int u = 0; u < kOffChunks; ++u) c[u] = cn[u];
+      if (j + 1 < nr) off_round(colind, __shfl(my0, j + 1), __shfl(my1, j + 1), lane, cn);
+      if (lane == 0) so[0][j] = e0;
+      int prev_c = INT_MIN, prev_s = 0;  // the entry before this chunk's first
+      for (int e = e0; e < e1; e += kOffChunks * kWave) {
+        if (e != e0) off_round(colind, e, e1, lane, c);  // rows above one round
+#pragma unroll
+        for (int u = 0; u < kOffChunks; ++u) {
+          const int eb = e + u * kWave;
+          if (eb >= e1) break;  // uniform
+          const int ei = eb + lane;
+          const int sc = slice_of(c[u], W, invW, S);
+          // lane l - 1's column and slice (lane 0: the previous chunk's last)
+          const int cb = __builtin_amdgcn_update_dpp(prev_c, c[u], kDppWaveShr1, 0xf, 0xf, false);
+          const int sb = __builtin_amdgcn_update_dpp(prev_s, sc, kDppWaveShr1, 0xf, 0xf, false);
+          if (ei < e1) {
+            bad |= c[u] < cb;
+            for (int t = sb + 1; t <= sc; ++t) so[t][j] = ei;
+          }
+          const int last = min(kWave - 1, e1 - 1 - eb);  // uniform
+          prev_c = __builtin_amdgcn_readlane(c[u], last);
+          prev_s = __builtin_amdgcn_readlane(sc, last);
+        }
+      }
+      // slices after the last entry's (all of them for an empty row)
+      for (int t = prev_s + 1 + lane; t <= S; t += kWave) so[t][j] = e1;
+    }
+    // the block's offsets, one store of nr consecutive ints per slice (LDS
+    // written and read by this wave only: a wave-scope fence orders them)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int x = lane; x < (S + 1) * kOffRows; x += kWave) {
+      const int t = x / kOffRows, j = x % kOffRows;
+      if (j < nr) off[t * Ms + r0 + j] = so[t][j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (__ballot(bad) && lane == 0) atomicOr(&state[kUnsortedFlag], 1);
+}
+
+// One slice launch of the plan-free sliced product: csr_dropin_kernel's
+// shares and classes (on degrees at the plan's sliced bin limits, so a row
+// is summed as the planned sliced launch sums it at the same geometry and
+// slice count), with each row's range [lo[r], hi[r]) of slice s. Rows with
+// no entry in a later slice are skipped. The last workgroup of each launch
+// zeroes the done count; that of the last slice also the list counts and
+// the unsorted flag.
+template <typename TB, int G, int VEC, bool kAcc>
+__global__ void __launch_bounds__(kThreads)
+csr_dropin_sliced_kernel(int M, int E, int a_limit, int rounds, const int *__restrict__ lo,
+                         const int *__restrict__ hi, const int *__restrict__ rowptr,
+                         const int *__restrict__ colind, const float *__restrict__ val,
+                         const TB *__restrict__ B, int ldb, float *__restrict__ C, int ldc,
+                         int *__restrict__ state, int last) {
+  constexpr int S = kWave / G, SW = kWavesPerWG * S;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int lane_sub = lane & (G - 1), sub = lane / G;
+  const int col0 = lane_sub * Vec16<TB>::kElems;
+  const TB *Bc = B + col0;
+  const int x = blockIdx.x;
+  // descending columns somewhere: slice 0 runs whole rows, the others nothing
+  const bool whole = state[kUnsortedFlag] != 0;
+  auto span = [&](int row) {
+    return whole ? (kAcc ? make_int2(0, 0) : make_int2(rowptr[row], rowptr[row + 1]))
+                 : make_int2(lo[row], hi[row]);
+  };
+  if (!(kAcc && whole)) {
+    if (x < E) {
+      const int n_wg = state[0], n_wave = state[2];
+      for (int i = x; i < n_wg; i += E) {  // uniform over the workgroup
+        const int row = state[kStateHead + i];
+        const int2 sp = span(row);
+        if (kAcc && sp.x == sp.y) continue;
+        csr_row_wg<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(row, sp, wave, sub, lane_sub,
+                                                                   col0, colind, val, Bc, ldb, C,
+                                                                   ldc, nullptr);
+      }
+      for (int j = (E - 1 - x) * kWavesPerWG + wave; j < n_wave; j += E * kWavesPerWG) {
+        const int row = state[kStateHead + M + j];
+        const int2 sp = span(row);
+        if (kAcc && sp.x == sp.y) continue;
+        csr_row_wave<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(
+            row, sp, sub, lane_sub, col0, colind, val, Bc, ldb, C, ldc, nullptr);
+      }
+    } else {
+      const int base = (x - E) * SW * rounds + wave * S + sub;
+      for (int k = 0; k < rounds; ++k) {
+        const int row = base + k * SW;
+        if (row < M && rowptr[row + 1] - rowptr[row] < a_limit) {  // else listed
+          const int2 sp = span(row);
+          if (!kAcc || sp.x < sp.y)
+            csr_row_sub<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(
+                row, sp, lane_sub, col0, colind, val, Bc, ldb, C, ldc, nullptr);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // every workgroup has read the counts, lists and flag before it counts itself
+    if (atomicAdd(&state[1], 1) == static_cast<int>(gridDim.x) - 1) {
+      if (last) {
+        atomicExch(&state[0], 0);
+        atomicExch(&state[2], 0);
+        atomicExch(&state[kUnsortedFlag], 0);
+      }
+      atomicExch(&state[1], 0);
+    }
+  }
+}
+
 // ------------------------------------------------- column-sliced launches
 // Per-row slice offsets of a plan, one wave per row (plan position i): lane
 // s in [1, S) finds by binary search the first entry whose column is at
@@ -217,8 +401,8 @@ slice_offsets_kernel(int M, int S, int W, const int2 *__restrict__ span,
 // run in slice order on one stream, so no two of them touch C at once.
 // colind, val and C are read (and C written) once per launch, with the
 // non-temporal policy, so the slice of B keeps the caches. Four B rows are
-// in flight per lane rather than the unsliced kernel's eight.
-constexpr int kSlicedInFlight = 4;
+// in flight per lane (kSlicedInFlight) rather than the unsliced kernel's
+// eight.
 template <typename TB, int G, int VEC, bool kAcc>
 __global__ void __launch_bounds__(kThreads)
 csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
@@ -873,9 +1057,110 @@ static void dispatch_dropin(int G, int M, int col_tiles, int dev, const int *row
 #undef SPMM_DL
 }
 
+// Column slices of the plan-free form: one column tile of B (N / 16-byte
+// vector lanes, 4 to 32) cut into slices of about 10 MB, when one tile of B
+// takes at least 16 MB and rows hold at least 16 entries a slice on average;
+// 1 (no slices) otherwise. On the 232,965-row graph this gives the
+// autotuner's counts: 12 at FP32 N = 128, 6 at FP32 N = 64 and BF16 N = 128.
+static int dropin_slice_count(int M, int K, int N, long long nnz, int elem_bytes) {
+  const int e16 = 16 / elem_bytes, lanes = N / e16;
+  if (M <= 0 || N % e16 != 0 || lanes < 4 || lanes > 32 || (lanes & (lanes - 1))) return 1;
+  const double tile = static_cast<double>(K) * N * elem_bytes;
+  const int S = std::min(32, static_cast<int>(tile / 10e6 + 0.5));
+  if (S < 2 || nnz < 16ll * S * M) return 1;
+  return S;
+}
+
+// Rows per subgroup of the sliced drop-in's streamed (short-row) class, in
+// the natural row order: about 160 entries per subgroup and launch, 2 to 8
+// rows. Measured on the 232,965-row graph, ms per product at 1 / 2 / 3 / 4 /
+// 6 / 8 rows: FP32 N = 128 (12 slices, 41 entries a row a slice) 6.21 /
+// 4.43 / 3.92 / 3.86 / 3.92 / 3.97; FP32 N = 64 (6 slices) 2.41 / 2.19 /
+// 2.22 / 2.25 / 2.32 / 2.36; BF16 N = 128 (6 slices) 2.49 / 2.36 / 2.39 /
+// 2.42 / 2.53 / 2.54; BF16 N = 64 (3 slices) 1.58 / 1.51 / 1.51 / 1.52 /
+// 1.53 / 1.52.
+static int dropin_sliced_rounds(int M, long long nnz, int S) {
+  const double per_slice = static_cast<double>(nnz) / (static_cast<double>(M) * S);
+  return std::max(2, std::min(8, static_cast<int>(160.0 / std::max(per_slice, 1.0) + 0.5)));
+}
+
+// The planned sliced launch's class limits as per-row degree limits: a bin
+// of degrees [2^(b-1), 2^b) runs a subgroup per row when 2^b <= a_limit S
+// (degree below the largest power of two <= a_limit S) and a workgroup per
+// row when 2^(b-1) >= c_limit S (degree at least the smallest power of two
+// >= c_limit S).
+static void sliced_row_limits(int G, int S, int *a_row, int *c_row) {
+  int a_limit, c_limit;
+  csr_class_limits(G, &a_limit, &c_limit);
+  const long long a = static_cast<long long>(a_limit) * S, c = static_cast<long long>(c_limit) * S;
+  long long pa = 1, pc = 1;
+  while (2 * pa <= a) pa *= 2;
+  while (pc < c) pc *= 2;
+  *a_row = static_cast<int>(pa);
+  *c_row = static_cast<int>(std::min<long long>(pc, INT_MAX));
+}
+
+template <typename TB, int G>
+static void launch_dropin_sliced(int M, int S, int W, int rounds, int dev, const int *rowptr,
+                                 const int *colind, const float *val, const TB *B, int ldb,
+                                 float *C, int ldc, int *state, int *off, hipStream_t s) {
+  constexpr int SW = kWavesPerWG * (kWave / G);
+  int a_row, c_row;
+  sliced_row_limits(G, S, &a_row, &c_row);
+  hipLaunchKernelGGL(class_list_kernel<kThreads>, dim3(ceil_div(M, kThreads)), dim3(kThreads), 0,
+                     s, M, rowptr, a_row, c_row, state);
+  hipLaunchKernelGGL(dropin_slice_offsets_kernel,
+                     dim3(std::min(ceil_div(M, kWavesPerWG * kOffRows), 8192)),
+                     dim3(kThreads), 0, s, M, S, W, rowptr, colind, off, state);
+  auto *const k0 = &csr_dropin_sliced_kernel<TB, G, 1, false>;
+  auto *const k1 = &csr_dropin_sliced_kernel<TB, G, 1, true>;
+  static std::atomic<int> per_cu;
+  const int E = std::max(
+      1, std::min(resident_wgs(reinterpret_cast<const void *>(k1), dev, &per_cu), ceil_div(M, 16)));
+  const dim3 grid(E + ceil_div(M, SW * rounds));
+  const size_t Ms = static_cast<size_t>(M);
+  for (int sl = 0; sl < S; ++sl) {
+    const int *lo = off + sl * Ms, *hi = off + (sl + 1) * Ms;
+    const int last = sl == S - 1;
+    if (sl == 0)
+      hipLaunchKernelGGL(k0, grid, dim3(kThreads), 0, s, M, E, a_row, rounds, lo, hi, rowptr, colind, val,
+                         B, ldb, C, ldc, state, last);
+    else
+      hipLaunchKernelGGL(k1, grid, dim3(kThreads), 0, s, M, E, a_row, rounds, lo, hi, rowptr, colind, val,
+                         B, ldb, C, ldc, state, last);
+  }
+}
+
+template <typename TB>
+static int run_dropin_sliced(spmm_handle_s *h, int M, int K, int N, int nnz, int S,
+                             const int *rowptr,
+                             const int *colind, const float *val, const TB *B, int ldb, float *C,
+                             int ldc, int *state, hipStream_t s) {
+  SPMM_CHECK_ARG(S >= 2 && S <= kOffMaxS, "plan-free slice count out of range");
+  const size_t n = (static_cast<size_t>(S) + 1) * static_cast<size_t>(M);
+  const int st = grow_ints(h, &h->slice_off, &h->slice_off_cap, n, s);
+  if (st) return st;
+  const int W = ceil_div(K, S);
+  const int rounds = dropin_sliced_rounds(M, nnz, S);
+#define SPMM_DS(g)                                                                             \
+  launch_dropin_sliced<TB, g>(M, S, W, rounds, h->device, rowptr, colind, val, B, ldb, C, ldc, state, \
+                              h->slice_off, s)
+  switch (N / Vec16<TB>::kElems) {
+    case 4: SPMM_DS(4); break;
+    case 8: SPMM_DS(8); break;
+    case 16: SPMM_DS(16); break;
+    default: SPMM_DS(32); break;
+  }
+#undef SPMM_DS
+  SPMM_LAUNCHED();
+  return SPMM_OK;
+}
+
 // The plan-free form: no host analysis, no allocation after the first call
 // at a given size, no synchronisation (the narrow kernel's case aside, which
 // analyses per call). perm != nullptr: entry e's value is val[perm[e]].
+// Large graphs (dropin_slice_count) run column-sliced, each row summed as
+// the planned sliced launch sums it at the same geometry and slice count.
 template <typename TB>
 static int run_dropin(spmm_handle_s *h, int M, int K, int N, int nnz,
                       const int *rowptr, const int *colind, const float *val,
@@ -904,6 +1189,9 @@ static int run_dropin(spmm_handle_s *h, int M, int K, int N, int nnz,
   int *state = nullptr;
   const int st = list_state(h, M, s, &state);
   if (st) return st;
+  const int S = perm ? 1 : dropin_slice_count(M, K, N, nnz, sizeof(TB));
+  if (S > 1)
+    return run_dropin_sliced<TB>(h, M, K, N, nnz, S, rowptr, colind, val, B, ldb, C, ldc, state, s);
   if (g.VEC == 2)
     dispatch_dropin<TB, 2>(g.G, M, g.col_tiles, h->device, rowptr, colind, val, perm, B, ldb, C,
                            ldc, state, s);
@@ -1179,6 +1467,21 @@ int spmm_csr_autotune(spmm_csr_plan_t plan, int M, int K, int N, int nnz, const 
 int spmm_csr_plan_bins(spmm_csr_plan_t plan, int *counts32) {
   SPMM_CHECK_ARG(plan && counts32, "null plan or counts");
   for (int b = 0; b < 32; ++b) counts32[b] = plan->counts[b];
+  return SPMM_OK;
+}
+
+int spmm_csr_dropin_geometry(int M, int K, int N, int nnz, int elem_bytes, spmm_tune_t *out) {
+  SPMM_CHECK_ARG(out != nullptr, "out is null");
+  SPMM_CHECK_ARG(elem_bytes == 4 || elem_bytes == 2, "elem_bytes must be 4 (FP32) or 2 (BF16)");
+  SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0 && nnz >= 0, "negative size");
+  const int S = spmm::dropin_slice_count(M, K, N, nnz, elem_bytes);
+  if (S > 1) {
+    *out = {N / (16 / elem_bytes), 1, 1, S};
+    return SPMM_OK;
+  }
+  alignas(16) static const float probe[4] = {0.f, 0.f, 0.f, 0.f};  // aligned operands
+  const spmm::Geometry g = spmm::pick_geometry(nullptr, N, elem_bytes, probe, N, probe, N, nullptr);
+  *out = {g.G, g.G ? g.VEC : 0, 1, 0};
   return SPMM_OK;
 }
 

@@ -97,6 +97,7 @@ int spmm_destroy(spmm_handle_t h) {
   if (h->pinned) (void)hipHostFree(h->pinned);
   if (h->list_state) (void)hipFree(h->list_state);
   if (h->tstruct) (void)hipFree(h->tstruct);
+  if (h->slice_off) (void)hipFree(h->slice_off);
   for (void *p : h->retired) (void)hipFree(p);
   delete h;
   return SPMM_OK;

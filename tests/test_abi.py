@@ -66,3 +66,30 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(RuntimeError, match="missing"):
         _lib.lib()
+
+
+def test_dropin_geometry_on_the_host():
+    """spmm_csr_dropin_geometry (host only): the plan-free CSR SpMM slices a
+    column tile of B of about 15 MB or more into slices of about 10 MB when
+    rows average at least 16 entries a slice -- the autotuner's slice counts
+    on the 232,965-row graph -- and otherwise runs the default geometry."""
+    from spmm_eval_amd import api
+
+    M, nnz = 232_965, 114_506_280
+    assert api.dropin_geometry(M, M, 128, nnz) == (32, 1, 1, 12)
+    assert api.dropin_geometry(M, M, 64, nnz) == (16, 1, 1, 6)
+    assert api.dropin_geometry(M, M, 128, nnz, bf16=True) == (16, 1, 1, 6)
+    assert api.dropin_geometry(M, M, 64, nnz, bf16=True) == (8, 1, 1, 3)
+    # short rows (mean degree 10) or a small B: one launch at the default geometry
+    assert api.dropin_geometry(89_250, 89_250, 128, 900_036) == (8, 1, 1, 0)
+    assert api.dropin_geometry(16_384, 16_384, 32, 259_972) == (8, 1, 1, 0)
+    # 15 MB tile: two slices; just below: none
+    assert api.dropin_geometry(30_000, 30_000, 128, 1_500_000)[3] == 2
+    assert api.dropin_geometry(29_000, 29_000, 128, 1_500_000)[3] == 0
+    # a tile wider than 32 lanes, or N not a whole number of 16-byte vectors
+    assert api.dropin_geometry(M, M, 256, nnz)[3] == 0
+    assert api.dropin_geometry(100, 100, 7, 500)[:2] == (0, 0)
+    with pytest.raises(_lib.SpmmError):  # elem_bytes neither 4 nor 2
+        _lib.call("spmm_csr_dropin_geometry", 10, 10, 64, 100, 3, C.byref(_lib.Tune()))
+    with pytest.raises(_lib.SpmmError):
+        _lib.call("spmm_csr_dropin_geometry", -1, 10, 64, 100, 4, C.byref(_lib.Tune()))

@@ -1,8 +1,11 @@
 """The plan-free CSR drop-ins (spmm_csr_f32 / _bf16, spmm_csr_t_f32 / _bf16):
 no host analysis, no allocation after the first call at a size, no
 synchronisation. Their rows run the planned kernel's per-class code, so the
-output equals the planned form at the same lane geometry bit for bit (the
-default geometry when neither is tuned); they replay from a captured graph
+output equals the planned form at the drop-in's lane geometry bit for bit
+(spmm_csr_dropin_geometry: the default geometry, or column slices when a
+column tile of B is large and rows are long, with per-row slice offsets built
+on the device each call; rows whose columns descend are summed whole, found
+on the device, with no synchronisation); they replay from a captured graph
 with eager work between replays; the handle state that grows for a larger
 graph keeps every result right; and growing during a capture is refused
 (the earlier state would be left to a graph that outgrew it)."""
@@ -30,19 +33,25 @@ def hubgraph(dev):
     return g, d
 
 
-@pytest.mark.parametrize("N,bf16", [(32, False), (64, False), (128, False), (64, True), (128, True)])
-def test_dropin_equals_planned_default_geometry(handle, dev, hubgraph, N, bf16):
+@pytest.mark.parametrize("N,bf16,sliced", [(32, False, False), (64, False, False), (128, False, True),
+                                           (64, True, False), (128, True, False), (256, True, True)])
+def test_dropin_equals_planned_at_its_geometry(handle, dev, hubgraph, N, bf16, sliced):
     g, (rp, ci, v) = hubgraph
     assert int(g.degrees().max()) > 2000
     B = graphs.uniform_pm1((g.K, N), seed=N)
     Bd = torch.from_numpy(B).to(dev)
     if bf16:
         Bd = api.to_bf16_bits(Bd)
+    geo = api.dropin_geometry(g.M, g.K, N, g.nnz, bf16)
+    assert (geo[3] > 1) == sliced, geo
     C_drop = api.spmm_csr(handle, rp, ci, v, Bd)
+    C_drop2 = api.spmm_csr(handle, rp, ci, v, Bd)
     plan = api.CsrPlan(handle, rp, g.nnz)
-    C_plan = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan)  # nothing tuned: the default
+    C_plan = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=geo)
     torch.cuda.synchronize()
-    assert torch.equal(C_drop, C_plan)
+    assert torch.equal(C_drop, C_plan) and torch.equal(C_drop, C_drop2)
+    if not sliced:  # nothing tuned: the planned default is the same geometry
+        assert torch.equal(C_drop, api.spmm_csr(handle, rp, ci, v, Bd, plan=plan))
     ref = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, Bd.view(torch.int16).cpu().numpy().view(np.uint16)) \
         if bf16 else oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
     assert rel(C_drop.cpu().numpy(), ref) <= 1e-5
@@ -69,26 +78,32 @@ def test_transposed_dropin_equals_planned(handle, dev, hubgraph, bf16):
 
 
 def test_dropins_replay_from_a_graph(dev, hubgraph):
-    """Three drop-in calls (CSR, transposed CSR, BF16 CSR) captured once and
-    replayed three times, with eager drop-in calls on another graph through
-    the same handle between replays; every replay equals the eager result."""
+    """Four drop-in calls (CSR, transposed CSR, BF16 CSR, column-sliced CSR)
+    captured once and replayed three times, with eager drop-in calls on
+    another graph through the same handle between replays; every replay
+    equals the eager result."""
     g, (rp, ci, v) = hubgraph
     h = api.Handle()
     N = 64
     Bd = torch.from_numpy(graphs.uniform_pm1((g.K, N), seed=8)).to(dev)
     Bh = api.to_bf16_bits(Bd)
+    Bw = torch.from_numpy(graphs.uniform_pm1((g.K, 128), seed=10)).to(dev)
+    assert api.dropin_geometry(g.M, g.K, 128, g.nnz)[3] > 1  # sliced
     C1 = torch.empty((g.M, N), device=dev)
     C2 = torch.empty((g.K, N), device=dev)
     C3 = torch.empty((g.M, N), device=dev)
+    C4 = torch.empty((g.M, 128), device=dev)
+    outs = (C1, C2, C3, C4)
 
     def calls():
         api.spmm_csr(h, rp, ci, v, Bd, C_out=C1)
         api.spmm_csr_t(h, rp, ci, v, Bd, K=g.K, C_out=C2)
         api.spmm_csr(h, rp, ci, v, Bh, C_out=C3)
+        api.spmm_csr(h, rp, ci, v, Bw, C_out=C4)
 
     calls()  # grows the handle's state and workspace
     torch.cuda.synchronize()
-    ref = [C.clone() for C in (C1, C2, C3)]
+    ref = [C.clone() for C in outs]
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     gr = torch.cuda.CUDAGraph()
@@ -102,17 +117,51 @@ def test_dropins_replay_from_a_graph(dev, hubgraph):
     srp, sci, sv = (torch.from_numpy(x).to(dev) for x in (small.rowptr, small.colind, small.val))
     Bs = torch.from_numpy(graphs.uniform_pm1((small.K, N), seed=9)).to(dev)
     for rep in range(3):
-        for C in (C1, C2, C3):
+        for C in outs:
             C.fill_(0)
         gr.replay()
         torch.cuda.synchronize()
-        for C, R in zip((C1, C2, C3), ref):
+        for C, R in zip(outs, ref):
             assert torch.equal(C, R), rep
         Cs = api.spmm_csr(h, srp, sci, sv, Bs)  # eager, same handle state
         torch.cuda.synchronize()
         assert rel(Cs.cpu().numpy(), oracle.csr_spmm_f64(small.rowptr, small.colind, small.val,
                                                           Bs.cpu().numpy())) <= 1e-5
     h.close()
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_sliced_dropin_rows_not_ascending(handle, dev, hubgraph, bf16):
+    """A row whose columns descend cannot be split by column ranges: the
+    offsets kernel flags it on the device and the first slice launch sums
+    every row whole (the later ones do nothing). The result stays within the
+    oracle's bound, and the next call on ascending rows, through the same
+    handle state, is again bitwise the planned sliced form."""
+    g, (rp, ci, v) = hubgraph
+    N = 128 if not bf16 else 256
+    geo = api.dropin_geometry(g.M, g.K, N, g.nnz, bf16)
+    assert geo[3] > 1
+    r = int(np.argmax(g.degrees()))
+    cu = g.colind.copy()
+    cu[g.rowptr[r]:g.rowptr[r + 1]] = cu[g.rowptr[r]:g.rowptr[r + 1]][::-1]
+    ciu = torch.from_numpy(cu).to(dev)
+    B = graphs.uniform_pm1((g.K, N), seed=21)
+    Bd = torch.from_numpy(B).to(dev)
+    if bf16:
+        Bd = api.to_bf16_bits(Bd)
+        Bf = oracle.bf16_to_f32(Bd.view(torch.int16).cpu().numpy().view(np.uint16))
+    else:
+        Bf = B
+    for _ in range(2):
+        Cu = api.spmm_csr(handle, rp, ciu, v, Bd)
+        torch.cuda.synchronize()
+        assert rel(Cu.cpu().numpy(), oracle.csr_spmm_f64(g.rowptr, cu, g.val, Bf)) <= 1e-5
+    Cs = api.spmm_csr(handle, rp, ci, v, Bd)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    Cp = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=geo)
+    torch.cuda.synchronize()
+    assert torch.equal(Cs, Cp)
+    plan.close()
 
 
 def test_state_regrowth_keeps_results(dev):

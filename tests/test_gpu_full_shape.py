@@ -2,7 +2,8 @@
 
 1. The 232,965-row power-law graph (~114.5M nnz, heaviest row ~17k entries,
    values U(0,1] row-normalised), N = 64 and 128, FP32 and BF16 B, through
-   the planned (autotuned) form and the plan-free drop-in: FP32 within 1e-5
+   the planned (autotuned) form and the plan-free drop-in (column-sliced at
+   these sizes, bitwise equal to the planned form at its geometry): FP32 within 1e-5
    relative Frobenius of the FP64 oracle and every element within its
    rounding bound nnz_row * 2^-23 * sum|a||b|; BF16 within 1e-5 of FP64 of
    the BF16-rounded B (kernel exactness) and within 1e-2 of FP64 of the FP32
@@ -49,6 +50,11 @@ def test_large_graph_f32_every_row(handle, dev, large, N):
     plan.autotune(rp, ci, v, Bd, C)
     Cp = api.spmm_csr(handle, rp, ci, v, Bd, C_out=C, plan=plan).cpu().numpy()
     Cd = api.spmm_csr(handle, rp, ci, v, Bd).cpu().numpy()  # plan-free drop-in
+    # the drop-in runs column-sliced here; the planned form at its geometry gives the same bits
+    geo = api.dropin_geometry(g.M, g.K, N, g.nnz)
+    assert geo[3] > 1, geo
+    Cg = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=geo).cpu().numpy()
+    np.testing.assert_array_equal(Cd, Cg)
     ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
     # sum |a||b| per element (values are positive), the rounding bound's scale
     mag = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, np.abs(B))
@@ -71,6 +77,10 @@ def test_large_graph_bf16_every_row(handle, dev, large, N):
     plan.autotune(rp, ci, v, Bd, C)
     Cp = api.spmm_csr(handle, rp, ci, v, Bd, C_out=C, plan=plan).cpu().numpy()
     Cd = api.spmm_csr(handle, rp, ci, v, Bd).cpu().numpy()
+    geo = api.dropin_geometry(g.M, g.K, N, g.nnz, bf16=True)
+    assert geo[3] > 1, geo
+    Cg = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=geo).cpu().numpy()
+    np.testing.assert_array_equal(Cd, Cg)
     ref_q = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, Bh)
     ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
     for name, Cx in (("planned", Cp), ("drop-in", Cd)):
