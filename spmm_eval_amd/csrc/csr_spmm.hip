@@ -176,9 +176,9 @@ csr_dropin_kernel(int M, int E, int a_limit, const int *__restrict__ rowptr,
 
 // ------------------------------------------------ plan-free column slices
 // The plan-free form of the column-sliced product (csr_sliced_kernel's
-// launches without a plan): per-call slice offsets, then S launches over the
-// classes class_list_kernel found, slice 0 storing C and the others adding
-// into it.
+// launches without a plan): the rows sorted by degree bin and the slice
+// offsets built on the device per call, then S launches, slice 0 storing C
+// and the others adding into it.
 constexpr int kSlicedInFlight = 4;  // B-row loads in flight per lane, sliced launches
 constexpr int kUnsortedFlag = 4;  // list-state slot: a row's columns descend
 
@@ -286,16 +286,49 @@ dropin_slice_offsets_kernel(int M, int S, int W, const int *__restrict__ rowptr,
   if (__ballot(bad) && lane == 0) atomicOr(&state[kUnsortedFlag], 1);
 }
 
-// One slice launch of the plan-free sliced product: csr_dropin_kernel's
-// shares and classes (on degrees at the plan's sliced bin limits, so a row
-// is summed as the planned sliced launch sums it at the same geometry and
-// slice count), with each row's range [lo[r], hi[r]) of slice s. Rows with
-// no entry in a later slice are skipped. The last workgroup of each launch
-// zeroes the done count; that of the last slice also the list counts and
-// the unsorted flag.
+constexpr int kDropinHead = 2 * kBins + 16;  // counts | cursors | class sizes, 64-byte aligned
+
+// Descending-bin cursors of the plan-free sliced product's row sort: from
+// the per-bin counts of bin_count_kernel, cursor[b] = the rows of bins above
+// b, meta[0] = workgroup-class rows (bins >= b_wg), meta[1] = wave-class rows
+// (bins b_sub < b < b_wg); the counts are zeroed for the next call.
+__global__ void __launch_bounds__(kWave)
+dropin_bins_kernel(int *__restrict__ counts, int *__restrict__ cursor, int *__restrict__ meta,
+                   int b_wg, int b_sub) {
+  __shared__ int cnt[kBins];
+  if (threadIdx.x < kBins) {
+    cnt[threadIdx.x] = counts[threadIdx.x];
+    counts[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int off = 0, n_wg = 0, n_wave = 0;
+    for (int b = kBins - 1; b >= 0; --b) {
+      cursor[b] = off;
+      off += cnt[b];
+      if (b >= b_wg) n_wg += cnt[b];
+      else if (b > b_sub) n_wave += cnt[b];
+    }
+    meta[0] = n_wg;
+    meta[1] = n_wave;
+  }
+}
+
+// One slice launch of the plan-free sliced product, over the rows sorted by
+// degree bin, heaviest first (rows[0, n_wg): workgroup class, then n_wave
+// rows of the wave class, then the subgroup class; the classes of the
+// planned sliced launch at the same geometry and slice count, so every row
+// is summed as that launch sums it), each row's range [lo[r], hi[r]) of
+// slice s. The first E workgroups (as many as the chip holds) deal the
+// workgroup-class rows (x, x + E, ...) and then the wave-class rows to their
+// waves, heaviest first; workgroup E + k takes
+// `rounds` subgroup-class rows per subgroup in sorted order. Rows with no
+// entry in a later slice are skipped. The last workgroup of each launch
+// zeroes the done count; that of the last slice also the unsorted flag.
 template <typename TB, int G, int VEC, bool kAcc>
 __global__ void __launch_bounds__(kThreads)
-csr_dropin_sliced_kernel(int M, int E, int a_limit, int rounds, const int *__restrict__ lo,
+csr_dropin_sliced_kernel(int M, int E, int rounds, const int *__restrict__ rows,
+                         const int *__restrict__ meta, const int *__restrict__ lo,
                          const int *__restrict__ hi, const int *__restrict__ rowptr,
                          const int *__restrict__ colind, const float *__restrict__ val,
                          const TB *__restrict__ B, int ldb, float *__restrict__ C, int ldc,
@@ -313,10 +346,10 @@ csr_dropin_sliced_kernel(int M, int E, int a_limit, int rounds, const int *__res
                  : make_int2(lo[row], hi[row]);
   };
   if (!(kAcc && whole)) {
+    const int n_wg = meta[0], n_wave = meta[1];
     if (x < E) {
-      const int n_wg = state[0], n_wave = state[2];
       for (int i = x; i < n_wg; i += E) {  // uniform over the workgroup
-        const int row = state[kStateHead + i];
+        const int row = rows[i];
         const int2 sp = span(row);
         if (kAcc && sp.x == sp.y) continue;
         csr_row_wg<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(row, sp, wave, sub, lane_sub,
@@ -324,17 +357,19 @@ csr_dropin_sliced_kernel(int M, int E, int a_limit, int rounds, const int *__res
                                                                    ldc, nullptr);
       }
       for (int j = (E - 1 - x) * kWavesPerWG + wave; j < n_wave; j += E * kWavesPerWG) {
-        const int row = state[kStateHead + M + j];
+        const int row = rows[n_wg + j];
         const int2 sp = span(row);
         if (kAcc && sp.x == sp.y) continue;
         csr_row_wave<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(
             row, sp, sub, lane_sub, col0, colind, val, Bc, ldb, C, ldc, nullptr);
       }
     } else {
-      const int base = (x - E) * SW * rounds + wave * S + sub;
+      const int first = n_wg + n_wave;
+      const int base = first + (x - E) * SW * rounds + wave * S + sub;
       for (int k = 0; k < rounds; ++k) {
-        const int row = base + k * SW;
-        if (row < M && rowptr[row + 1] - rowptr[row] < a_limit) {  // else listed
+        const int p = base + k * SW;
+        if (p < M) {
+          const int row = rows[p];
           const int2 sp = span(row);
           if (!kAcc || sp.x < sp.y)
             csr_row_sub<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(
@@ -345,13 +380,9 @@ csr_dropin_sliced_kernel(int M, int E, int a_limit, int rounds, const int *__res
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    // every workgroup has read the counts, lists and flag before it counts itself
+    // every workgroup has read the flag before it counts itself
     if (atomicAdd(&state[1], 1) == static_cast<int>(gridDim.x) - 1) {
-      if (last) {
-        atomicExch(&state[0], 0);
-        atomicExch(&state[2], 0);
-        atomicExch(&state[kUnsortedFlag], 0);
-      }
+      if (last) atomicExch(&state[kUnsortedFlag], 0);
       atomicExch(&state[1], 0);
     }
   }
@@ -1071,17 +1102,18 @@ static int dropin_slice_count(int M, int K, int N, long long nnz, int elem_bytes
   return S;
 }
 
-// Rows per subgroup of the sliced drop-in's streamed (short-row) class, in
-// the natural row order: about 160 entries per subgroup and launch, 2 to 8
-// rows. Measured on the 232,965-row graph, ms per product at 1 / 2 / 3 / 4 /
-// 6 / 8 rows: FP32 N = 128 (12 slices, 41 entries a row a slice) 6.21 /
-// 4.43 / 3.92 / 3.86 / 3.92 / 3.97; FP32 N = 64 (6 slices) 2.41 / 2.19 /
-// 2.22 / 2.25 / 2.32 / 2.36; BF16 N = 128 (6 slices) 2.49 / 2.36 / 2.39 /
-// 2.42 / 2.53 / 2.54; BF16 N = 64 (3 slices) 1.58 / 1.51 / 1.51 / 1.52 /
-// 1.53 / 1.52.
+// Rows per subgroup of the sliced drop-in's subgroup class (rows sorted by
+// degree bin): about 240 entries per subgroup and launch, 3 to 8 rows.
+// Measured on the 232,965-row graph, ms per product at 2 / 3 / 4 / 6 rows:
+// FP32 N = 128 (12 slices, 41 entries a row a slice) 4.51 / 4.04 / 3.90 /
+// 3.81; FP32 N = 64 (6 slices) 2.15 / 2.11 / 2.11 / 2.23; BF16 N = 128
+// (6 slices) 2.28 / 2.23 / 2.26 / 2.39; BF16 N = 64 (3 slices) 1.47 / 1.44 /
+// 1.43 / 1.44. One row was slower everywhere (FP32 N = 128: 6.3 ms),
+// probably because the listed rows hold every resident slot first and the
+// short workgroups then queue behind them.
 static int dropin_sliced_rounds(int M, long long nnz, int S) {
   const double per_slice = static_cast<double>(nnz) / (static_cast<double>(M) * S);
-  return std::max(2, std::min(8, static_cast<int>(160.0 / std::max(per_slice, 1.0) + 0.5)));
+  return std::max(3, std::min(8, static_cast<int>(240.0 / std::max(per_slice, 1.0) + 0.5)));
 }
 
 // The planned sliced launch's class limits as per-row degree limits: a bin
@@ -1103,12 +1135,23 @@ static void sliced_row_limits(int G, int S, int *a_row, int *c_row) {
 template <typename TB, int G>
 static void launch_dropin_sliced(int M, int S, int W, int rounds, int dev, const int *rowptr,
                                  const int *colind, const float *val, const TB *B, int ldb,
-                                 float *C, int ldc, int *state, int *off, hipStream_t s) {
+                                 float *C, int ldc, int *state, int *scratch, hipStream_t s) {
   constexpr int SW = kWavesPerWG * (kWave / G);
   int a_row, c_row;
   sliced_row_limits(G, S, &a_row, &c_row);
-  hipLaunchKernelGGL(class_list_kernel<kThreads>, dim3(ceil_div(M, kThreads)), dim3(kThreads), 0,
-                     s, M, rowptr, a_row, c_row, state);
+  // scratch: the row sort's bin counts (zero between calls), cursors and
+  // class sizes at a fixed head, then the slice offsets
+  int *const counts = scratch, *const cursor = counts + kBins, *const meta = cursor + kBins;
+  int *const off = scratch + kDropinHead;
+  int *const rows = state + kStateHead;
+  const int b_sub = 31 - __builtin_clz(static_cast<unsigned>(a_row));      // last subgroup-class bin
+  const int b_wg = 32 - __builtin_clz(static_cast<unsigned>(c_row));       // first workgroup-class bin
+  const int grid_r = std::min(ceil_div(M, kThreads), 2048);
+  hipLaunchKernelGGL(bin_count_kernel, dim3(grid_r), dim3(kThreads), 0, s, rowptr, M, 0, counts);
+  hipLaunchKernelGGL(dropin_bins_kernel, dim3(1), dim3(kWave), 0, s, counts, cursor, meta, b_wg,
+                     b_sub);
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid_r), dim3(kThreads), 0, s, rowptr, M, 0, cursor,
+                     rows);
   hipLaunchKernelGGL(dropin_slice_offsets_kernel,
                      dim3(std::min(ceil_div(M, kWavesPerWG * kOffRows), 8192)),
                      dim3(kThreads), 0, s, M, S, W, rowptr, colind, off, state);
@@ -1123,11 +1166,11 @@ static void launch_dropin_sliced(int M, int S, int W, int rounds, int dev, const
     const int *lo = off + sl * Ms, *hi = off + (sl + 1) * Ms;
     const int last = sl == S - 1;
     if (sl == 0)
-      hipLaunchKernelGGL(k0, grid, dim3(kThreads), 0, s, M, E, a_row, rounds, lo, hi, rowptr, colind, val,
-                         B, ldb, C, ldc, state, last);
+      hipLaunchKernelGGL(k0, grid, dim3(kThreads), 0, s, M, E, rounds, rows, meta, lo, hi, rowptr,
+                         colind, val, B, ldb, C, ldc, state, last);
     else
-      hipLaunchKernelGGL(k1, grid, dim3(kThreads), 0, s, M, E, a_row, rounds, lo, hi, rowptr, colind, val,
-                         B, ldb, C, ldc, state, last);
+      hipLaunchKernelGGL(k1, grid, dim3(kThreads), 0, s, M, E, rounds, rows, meta, lo, hi, rowptr,
+                         colind, val, B, ldb, C, ldc, state, last);
   }
 }
 
@@ -1137,7 +1180,7 @@ static int run_dropin_sliced(spmm_handle_s *h, int M, int K, int N, int nnz, int
                              const int *colind, const float *val, const TB *B, int ldb, float *C,
                              int ldc, int *state, hipStream_t s) {
   SPMM_CHECK_ARG(S >= 2 && S <= kOffMaxS, "plan-free slice count out of range");
-  const size_t n = (static_cast<size_t>(S) + 1) * static_cast<size_t>(M);
+  const size_t n = kDropinHead + (static_cast<size_t>(S) + 1) * static_cast<size_t>(M);
   const int st = grow_ints(h, &h->slice_off, &h->slice_off_cap, n, s);
   if (st) return st;
   const int W = ceil_div(K, S);
