@@ -131,6 +131,33 @@ def test_dropins_replay_from_a_graph(dev, hubgraph):
 
 
 @pytest.mark.parametrize("bf16", [False, True])
+def test_sliced_dropin_strided_operands(handle, dev, hubgraph, bf16):
+    """B and C as column slices of wider row-major arrays (ldb, ldc > N): the
+    sliced drop-in equals the planned form at its geometry on the same views,
+    and C's columns outside the view are untouched."""
+    g, (rp, ci, v) = hubgraph
+    N = 128 if not bf16 else 256
+    geo = api.dropin_geometry(g.M, g.K, N, g.nnz, bf16)
+    assert geo[3] > 1
+    Bw = torch.from_numpy(graphs.uniform_pm1((g.K, N + 64), seed=33)).to(dev)
+    if bf16:
+        Bw = api.to_bf16_bits(Bw)
+    B = Bw[:, 32:32 + N]  # 16-byte aligned view, ldb = N + 64
+    Cw = torch.full((g.M, N + 32), 7.0, device=dev)
+    Cp = torch.full((g.M, N + 32), 7.0, device=dev)
+    api.spmm_csr(handle, rp, ci, v, B, C_out=Cw[:, 16:16 + N])
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    api.spmm_csr(handle, rp, ci, v, B, C_out=Cp[:, 16:16 + N], plan=plan, tune=geo)
+    torch.cuda.synchronize()
+    assert torch.equal(Cw, Cp)
+    assert bool((Cw[:, :16] == 7.0).all()) and bool((Cw[:, 16 + N:] == 7.0).all())
+    Bf = oracle.bf16_to_f32(B.contiguous().view(torch.int16).cpu().numpy().view(np.uint16)) if bf16 \
+        else B.cpu().numpy()
+    assert rel(Cw[:, 16:16 + N].cpu().numpy(), oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, Bf)) <= 1e-5
+    plan.close()
+
+
+@pytest.mark.parametrize("bf16", [False, True])
 def test_sliced_dropin_rows_not_ascending(handle, dev, hubgraph, bf16):
     """A row whose columns descend cannot be split by column ranges: the
     offsets kernel flags it on the device and the first slice launch sums
