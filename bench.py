@@ -910,7 +910,9 @@ def main():
         "checks": {"rel_err_fp64_sampled_rows_incl_heaviest": hd["err"], "bar": 1e-5},
         "analyze_ms_host_clock": round(hd["analyze_ms"], 2),
         "dropin_plan_free": {"ms_median_of_5": round(hd["dropin_ms"], 4) if hd["dropin_ms"] else None,
-                             "rel_err_fp64_sampled": hd["dropin_err"]},
+                             "rel_err_fp64_sampled": hd["dropin_err"],
+                             "geometry_lanes_vec_rows_slices": list(api.dropin_geometry(g.M, g.K, N, g.nnz)),
+                             "form": "rows sorted and slice offsets built on the device per call"},
     }
     if not args.no_cpu:
         log("cpu baseline")
