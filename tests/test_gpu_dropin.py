@@ -157,6 +157,42 @@ def test_sliced_dropin_strided_operands(handle, dev, hubgraph, bf16):
     plan.close()
 
 
+def test_sliced_dropin_edge_rows(handle, dev, hubgraph):
+    """Empty rows (every seventh), rows whose entries all lie in one slice (the
+    columns of every fifth row clipped into the first 1,000), and K not a
+    multiple of the slice count: the sliced drop-in equals the planned form at
+    its geometry, and empty rows come out as zeros."""
+    g0, _ = hubgraph
+    keep = []
+    for r in range(g0.M):
+        seg = g0.colind[g0.rowptr[r]:g0.rowptr[r + 1]]
+        if r % 7 == 3:
+            seg = seg[:0]
+        elif r % 5 == 1:
+            seg = np.unique(seg % 1000)
+        keep.append(seg)
+    rowptr = np.zeros(g0.M + 1, np.int32)
+    rowptr[1:] = np.cumsum([len(x) for x in keep])
+    colind = np.concatenate(keep).astype(np.int32)
+    val = graphs.uniform_pm1((len(colind),), seed=41).astype(np.float32)
+    K = g0.K + 5  # B has rows no entry uses; ceil(K / S) splits unevenly
+    g = graphs.Csr(rowptr, colind, val, g0.M, K)
+    N = 128
+    geo = api.dropin_geometry(g.M, K, N, g.nnz)
+    assert geo[3] > 1 and K % geo[3] != 0, geo
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    B = graphs.uniform_pm1((K, N), seed=42)
+    Bd = torch.from_numpy(B).to(dev)
+    Cd = api.spmm_csr(handle, rp, ci, v, Bd)
+    plan = api.CsrPlan(handle, rp, g.nnz)
+    Cp = api.spmm_csr(handle, rp, ci, v, Bd, plan=plan, tune=geo)
+    torch.cuda.synchronize()
+    assert torch.equal(Cd, Cp)
+    assert bool((Cd[3::7] == 0).all())
+    assert rel(Cd.cpu().numpy(), oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)) <= 1e-5
+    plan.close()
+
+
 @pytest.mark.parametrize("bf16", [False, True])
 def test_sliced_dropin_rows_not_ascending(handle, dev, hubgraph, bf16):
     """A row whose columns descend cannot be split by column ranges: the
