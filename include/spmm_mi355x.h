@@ -348,7 +348,9 @@ int csr_transpose(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
  * weights). The drop-in forms build the transpose into buffers the handle
  * owns (kernels only: no synchronisation; they grow on the first call at a
  * larger size, refused during graph capture as for spmm_csr_f32) and run the
- * plan-free SpMM over it. */
+ * plan-free SpMM over it. The planned form takes every geometry of the CSR
+ * SpMM, column slices included (the transpose's rows ascend in A's row
+ * index); spmm_csr_t_autotune tries them as spmm_csr_autotune does. */
 int spmm_csr_t_analyze(spmm_handle_t h, int M, int K, int nnz,
                        const int *rowptr, const int *colind,
                        spmm_csrt_plan_t *plan, void *stream);

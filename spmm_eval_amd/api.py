@@ -117,11 +117,13 @@ class CsrTPlan:
         self.p = p
 
     def autotune(self, val, B, C_out, stream=None) -> tuple:
+        """Times the launch geometries of A^T B (column slices included) and
+        keeps the fastest in the plan: (lanes_per_row, vec, rows_scale, slices)."""
         best = Tune()
         call("spmm_csr_t_autotune", self.p, self.M, self.K, B.shape[1], self.nnz, _p(val), _p(B),
              int(_is_bf16(B)), B.stride(0), _p(C_out), C_out.stride(0), C.byref(best),
              _stream(stream))
-        return (best.lanes_per_row, best.vec, best.rows_scale)
+        return (best.lanes_per_row, best.vec, best.rows_scale, best.slices)
 
     def close(self):
         if self.p:

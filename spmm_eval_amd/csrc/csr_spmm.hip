@@ -433,13 +433,15 @@ slice_offsets_kernel(int M, int S, int W, const int2 *__restrict__ span,
 // colind, val and C are read (and C written) once per launch, with the
 // non-temporal policy, so the slice of B keeps the caches. Four B rows are
 // in flight per lane (kSlicedInFlight) rather than the unsliced kernel's
-// eight.
-template <typename TB, int G, int VEC, bool kAcc>
+// eight. kPerm: entry e's value is val[perm[e]] (the transposed plan, whose
+// rows ascend in A's row index, as the stable transpose builds them).
+template <typename TB, int G, int VEC, bool kAcc, bool kPerm>
 __global__ void __launch_bounds__(kThreads)
 csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
                   const int *__restrict__ lo, const int *__restrict__ hi,
                   const int *__restrict__ colind, const float *__restrict__ val,
-                  const TB *__restrict__ B, int ldb, float *__restrict__ C, int ldc) {
+                  const int *__restrict__ perm, const TB *__restrict__ B, int ldb,
+                  float *__restrict__ C, int ldc) {
   constexpr int E = Vec16<TB>::kElems;
   constexpr int S = kWave / G;
   constexpr int CW = G * VEC * E;
@@ -470,8 +472,8 @@ csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
         sp_n = make_int2(lo[pos_n], hi[pos_n]);
       }
       if (!kAcc || sp.x < sp.y)
-        csr_row_sub<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(row, sp, lane_sub, col0, colind, val, Bc, ldb, C,
-                                             ldc, nullptr);
+        csr_row_sub<TB, G, VEC, kPerm, kAcc, true, kSlicedInFlight>(row, sp, lane_sub, col0, colind,
+                                                                    val, Bc, ldb, C, ldc, perm);
       row = row_n;
       sp = sp_n;
       pos = pos_n;
@@ -481,14 +483,15 @@ csr_sliced_kernel(LaunchTable tab, int ctile, const int *__restrict__ rows,
     if (pos >= sg.list_off + sg.count) return;  // whole wave
     const int2 sp = make_int2(lo[pos], hi[pos]);
     if (kAcc && sp.x == sp.y) return;
-    csr_row_wave<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(rows[pos], sp, sub, lane_sub, col0, colind, val, Bc,
-                                          ldb, C, ldc, nullptr);
+    csr_row_wave<TB, G, VEC, kPerm, kAcc, true, kSlicedInFlight>(rows[pos], sp, sub, lane_sub, col0,
+                                                                 colind, val, Bc, ldb, C, ldc, perm);
   } else {
     const int pos = sg.list_off + local_wg;
     const int2 sp = make_int2(lo[pos], hi[pos]);
     if (kAcc && sp.x == sp.y) return;  // whole workgroup
-    csr_row_wg<TB, G, VEC, false, kAcc, true, kSlicedInFlight>(rows[pos], sp, wave, sub, lane_sub, col0, colind, val, Bc,
-                                        ldb, C, ldc, nullptr);
+    csr_row_wg<TB, G, VEC, kPerm, kAcc, true, kSlicedInFlight>(rows[pos], sp, wave, sub, lane_sub,
+                                                               col0, colind, val, Bc, ldb, C, ldc,
+                                                               perm);
   }
 }
 
@@ -749,31 +752,35 @@ static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hi
   return SPMM_OK;
 }
 
-template <typename TB, int G, int VEC>
+template <typename TB, int G, int VEC, bool kPerm>
 static void launch_sliced(const LaunchTable &t, int col_tiles, int S, const spmm_csr_plan_s *p,
-                          const int *off, const int *colind, const float *val, const TB *B,
-                          int ldb, float *C, int ldc, hipStream_t s) {
+                          const int *off, const int *colind, const float *val, const int *perm,
+                          const TB *B, int ldb, float *C, int ldc, hipStream_t s) {
   const size_t M = static_cast<size_t>(p->M);
   for (int ct = 0; ct < col_tiles; ++ct)
     for (int sl = 0; sl < S; ++sl) {
       const int *lo = off + sl * M, *hi = off + (sl + 1) * M;
       if (sl == 0)
-        hipLaunchKernelGGL((csr_sliced_kernel<TB, G, VEC, false>), dim3(t.total_wg),
-                           dim3(kThreads), 0, s, t, ct, p->d_rows, lo, hi, colind, val, B, ldb,
-                           C, ldc);
+        hipLaunchKernelGGL((csr_sliced_kernel<TB, G, VEC, false, kPerm>), dim3(t.total_wg),
+                           dim3(kThreads), 0, s, t, ct, p->d_rows, lo, hi, colind, val, perm, B,
+                           ldb, C, ldc);
       else
-        hipLaunchKernelGGL((csr_sliced_kernel<TB, G, VEC, true>), dim3(t.total_wg),
-                           dim3(kThreads), 0, s, t, ct, p->d_rows, lo, hi, colind, val, B, ldb,
-                           C, ldc);
+        hipLaunchKernelGGL((csr_sliced_kernel<TB, G, VEC, true, kPerm>), dim3(t.total_wg),
+                           dim3(kThreads), 0, s, t, ct, p->d_rows, lo, hi, colind, val, perm, B,
+                           ldb, C, ldc);
     }
 }
 
 template <typename TB>
 static void dispatch_sliced(const Geometry &g, const LaunchTable &t, const spmm_csr_plan_s *p,
-                            const int *off, const int *colind, const float *val, const TB *B,
-                            int ldb, float *C, int ldc, hipStream_t s) {
-#define SPMM_SL(gg, vv) \
-  launch_sliced<TB, gg, vv>(t, g.col_tiles, g.slices, p, off, colind, val, B, ldb, C, ldc, s)
+                            const int *off, const int *colind, const float *val,
+                            const int *perm, const TB *B, int ldb, float *C, int ldc,
+                            hipStream_t s) {
+#define SPMM_SL(gg, vv)                                                                        \
+  (perm ? launch_sliced<TB, gg, vv, true>(t, g.col_tiles, g.slices, p, off, colind, val, perm, \
+                                          B, ldb, C, ldc, s)                                   \
+        : launch_sliced<TB, gg, vv, false>(t, g.col_tiles, g.slices, p, off, colind, val,      \
+                                           nullptr, B, ldb, C, ldc, s))
   if (g.VEC == 2) {
     switch (g.G) {
       case 4: SPMM_SL(4, 2); break;
@@ -882,13 +889,13 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
     return SPMM_OK;
   }
   LaunchTable t;
-  if (g.slices > 1 && perm == nullptr && nnz > 0) {
+  if (g.slices > 1 && nnz > 0) {
     const int *off = nullptr;
     const int st = slice_offsets(p, g.slices, K, colind, s, &off);
     if (st) return st;
     build_table_sliced(p->counts, p->offs, g.G, g.rows_scale, g.slices, &t);
     if (t.total_wg == 0) return SPMM_OK;
-    dispatch_sliced<TB>(g, t, p, off, colind, val, B, ldb, C, ldc, s);
+    dispatch_sliced<TB>(g, t, p, off, colind, val, perm, B, ldb, C, ldc, s);
     SPMM_LAUNCHED();
     return SPMM_OK;
   }
@@ -914,7 +921,7 @@ static int autotune(spmm_csr_plan_s *p, int M, int K, int N, int nnz, const int 
   const int e16 = 16 / static_cast<int>(sizeof(TB));
   spmm_tune_t best = {0, 0, 0, 0};
   float best_ms = 1e30f;
-  bool slicing = perm == nullptr && nnz > 0 && p->sorted != 0;
+  bool slicing = nnz > 0 && p->sorted != 0;
   hipEvent_t e0, e1;
   SPMM_HIP(hipEventCreate(&e0));
   SPMM_HIP(hipEventCreate(&e1));

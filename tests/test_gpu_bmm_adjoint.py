@@ -258,13 +258,54 @@ def test_spmm_csr_t(handle, dev, M, K, dens, nl, ll, N):
         Ch = api.spmm_csr_t(handle, rp, ci, v, Bh, K, plan=plan)
         C3h = api.spmm_csr(handle, trp, tci, tval, Bh, K=M, plan=tplan)
         assert torch.equal(Ch, C3h)
-        G, vec, rs = plan.autotune(v, Bh, torch.empty_like(Ch))
+        G, vec, rs, sl = plan.autotune(v, Bh, torch.empty_like(Ch))
         assert G in (1, 2, 4, 8, 16, 32, 64) and vec in (1, 2) and rs in (1, 2, 4, 8)
+        assert sl in (0, 2, 3, 4, 6, 8, 12, 16, 24, 32)
         Ct = api.spmm_csr_t(handle, rp, ci, v, Bh, K, plan=plan).cpu().numpy()  # tuned entry
         Bq = oracle.bf16_to_f32(Bh.view(torch.int16).cpu().numpy().view(np.uint16))
         refq = oracle.csr_spmm_f64(cp, ri, cv, Bq)
         nq = np.linalg.norm(refq)
         assert np.linalg.norm(Ct - refq) / (nq if nq > 0 else 1.0) <= 1e-5
+
+
+@pytest.mark.parametrize("S,tune", [(2, (8, 1, 1)), (6, (32, 1, 1)), (12, (32, 1, 2)), (4, (16, 1, 1))])
+def test_spmm_csr_t_column_slices(handle, dev, S, tune):
+    """A^T B from the transposed plan in column slices (values read through
+    the permutation): bit for bit the sliced CSR SpMM on the explicit
+    transpose at the same geometry, within 1e-5 of FP64 (FP32 B) and of FP64
+    of the rounded B (BF16), two calls equal, and new values followed
+    without a new plan."""
+    g = graphs.chung_lu(12000, 600000, seed=S + 40, max_deg=5000)
+    g.val = graphs.row_normalised_uniform(g, S)
+    N = 128
+    rp, ci, v = (torch.from_numpy(x).to(dev) for x in (g.rowptr, g.colind, g.val))
+    B = graphs.uniform_pm1((g.M, N), seed=S + 2)
+    Bd = torch.from_numpy(B).to(dev)
+    plan = api.CsrTPlan(handle, rp, ci, g.K)
+    t = tune + (S,)
+    C1 = api.spmm_csr_t(handle, rp, ci, v, Bd, g.K, plan=plan, tune=t)
+    C2 = api.spmm_csr_t(handle, rp, ci, v, Bd, g.K, plan=plan, tune=t)
+    trp, tci, tval, _ = api.csr_transpose(handle, rp, ci, v, g.K)
+    tplan = api.CsrPlan(handle, trp, g.nnz)
+    C3 = api.spmm_csr(handle, trp, tci, tval, Bd, K=g.M, plan=tplan, tune=t)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2) and torch.equal(C1, C3)
+    cp, ri, cv, _ = oracle.csr2csc(g.rowptr, g.colind, g.val, g.K)
+    ref = oracle.csr_spmm_f64(cp, ri, cv, B)
+    assert np.linalg.norm(C1.cpu().numpy() - ref) / np.linalg.norm(ref) <= 1e-5
+    v2 = v * 2.0 - 0.5
+    C4 = api.spmm_csr_t(handle, rp, ci, v2, Bd, g.K, plan=plan, tune=t)
+    ref2 = oracle.csr_spmm_f64(cp, ri, (cv * 2.0 - 0.5).astype(np.float32), B)
+    assert np.linalg.norm(C4.cpu().numpy() - ref2) / np.linalg.norm(ref2) <= 1e-5
+    Bh = api.to_bf16_bits(Bd)
+    Ch = api.spmm_csr_t(handle, rp, ci, v, Bh, g.K, plan=plan, tune=t)
+    C3h = api.spmm_csr(handle, trp, tci, tval, Bh, K=g.M, plan=tplan, tune=t)
+    torch.cuda.synchronize()
+    assert torch.equal(Ch, C3h)
+    refq = oracle.csr_spmm_f64(cp, ri, cv, oracle.bf16_to_f32(Bh.view(torch.int16).cpu().numpy().view(np.uint16)))
+    assert np.linalg.norm(Ch.cpu().numpy() - refq) / np.linalg.norm(refq) <= 1e-5
+    plan.close()
+    tplan.close()
 
 
 def test_spmm_csr_t_plan_mismatch(handle, dev):
