@@ -27,7 +27,8 @@ and B on all host threads (3 warm-ups, median of 10). `extra.cpu` holds the
 same kind of baseline for every other workload (forward, CPU dX and CPU
 SDDMM for the fwd+bwd comparisons).
 
-`extra` (skipped with --quick): the same graph at N = 64 and in BF16; the
+`extra` (skipped with --quick): the same graph at N = 64 and in BF16, and its
+A^T dY (the backward's dX) through the transposed plan; the
 16,384-row N=32 workload; B2SR-4 FBF SpMM on the 89,250-row 0/1 graph
 (planned and plan-free) with the tile x N grid, the sign-packed popcount
 SpMM and the dense packers; BMM32 (XNOR/popcount matmul); GCN-style
@@ -337,6 +338,33 @@ def extra_large_variants(args, h, dev, hd):
                                reps=5)
             cpu[f"csr_{name}_233k_n{N}"] = {"ms": round(cms, 2), "gpu_speedup": round(cms / ms, 1)}
         del Bd, Bh, C
+    # the backward's dX = A^T dY on the same graph: the transposed plan (values read through
+    # its entry permutation), autotuned, column slices included; checked on sampled rows of
+    # an FP64 recompute over scipy's transpose
+    import scipy.sparse as sps
+
+    tp = api.CsrTPlan(h, rp, ci, g.K)
+    At = sps.csr_matrix((g.val, g.colind, g.rowptr), shape=(g.M, g.K)).T.tocsr()
+    gt = graphs.Csr(At.indptr.astype(np.int32), At.indices.astype(np.int32),
+                    At.data.astype(np.float32), g.K, g.M)
+    del At
+    rows_t = check_rows(gt, 9)
+    N = 128
+    dY = graphs.uniform_pm1((g.M, N), seed=graphs.SEED_BASE + 7)
+    dYd = torch.from_numpy(dY).to(dev)
+    dX = torch.empty((g.K, N), device=dev)
+    for name, dYx, sb in (("f32", dYd, 4), ("bf16", api.to_bf16_bits(dYd), 2)):
+        geo = tp.autotune(v, dYx, dX)
+        ms, _ = median_batches(lambda: api.spmm_csr_t(h, rp, ci, v, dYx, g.K, C_out=dX, plan=tp),
+                               launches=20)
+        ref = (bf16_round(dY) if name == "bf16" else dY).astype(np.float64)
+        out[f"csr_t_{name}_233k_n{N}"] = {
+            "geometry": geo, "ms_median_of_5": round(ms, 4),
+            # each array once, the permutation included
+            "compulsory_GBps": rate(csr_compulsory_bytes(g.K, g.M, g.nnz, N, sb) + 4 * g.nnz, ms),
+            "rel_err_fp64_sampled": sampled_rel_err(gt, ref, dX.cpu().numpy(), rows_t)}
+    tp.close()
+    del dYd, dX, gt
     torch.cuda.empty_cache()
     out["cpu"] = cpu
     return out
