@@ -1506,6 +1506,7 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
     default:
+      SPMM_CHECK_ARG(T == 16 || p->d_wide != nullptr, "wide-tile plan without its round lists");
       b2sr_wide_launch<TB>(T, g.G, g.tiles, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s);
       break;
   }
@@ -1663,13 +1664,64 @@ int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr, spmm_
   SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M > 0, "M must be positive");
   // block rows binned by tile count (the units of b2sr_brow_kernel and
-  // b2sr_wide_kernel), heaviest first
-  const int st = analyze(h, (M + T - 1) / T, 0, bsrRowPtr, 0, plan, stream);
-  if (st == SPMM_OK) {
-    (*plan)->M = M;
-    (*plan)->tile = T;
+  // the wide-tile kernels), heaviest first
+  const int MB = (M + T - 1) / T;
+  const int st = analyze(h, MB, 0, bsrRowPtr, 0, plan, stream);
+  if (st != SPMM_OK) return st;
+  spmm_csr_plan_s *p = *plan;
+  p->M = M;
+  p->tile = T;
+  if (T > 16) {
+    // the wide-tile kernels' rounds (wide_round_tiles(T) tiles of one block
+    // row each) numbered on the host, and their list buffer
+    hipStream_t s = as_stream(stream);
+    std::vector<int> hb(static_cast<size_t>(MB) + 1);
+    auto fail = [&](const char *what) {
+      spmm_csr_plan_destroy(p);
+      *plan = nullptr;
+      set_error(std::string("spmm_b2sr_analyze: ") + what);
+      return SPMM_ERR_ALLOC;
+    };
+    std::vector<int> units(static_cast<size_t>(MB));  // the plan's block-row order
+    if (hipMemcpyAsync(hb.data(), bsrRowPtr, sizeof(int) * hb.size(), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(units.data(), p->d_rows, sizeof(int) * units.size(),
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail("bsrRowPtr read-back failed");
+    const int NT = wide_round_tiles(T);
+    std::vector<int> first(static_cast<size_t>(MB));
+    std::vector<int2> rounds;
+    if (hb[0] < 0) return fail("bsrRowPtr[0] is negative");
+    for (int b = 0; b < MB; ++b) {
+      if (hb[b + 1] < hb[b]) return fail("bsrRowPtr is not monotone");
+      first[b] = static_cast<int>(rounds.size());
+      for (int t = hb[b]; t < hb[b + 1]; t += NT)
+        rounds.push_back(make_int2(t, std::min(t + NT, hb[b + 1])));
+    }
+    std::vector<int> rbase(static_cast<size_t>(MB));  // by plan unit
+    for (int x = 0; x < MB; ++x) rbase[x] = first[units[x]];
+    const size_t nb = static_cast<size_t>(hb[MB]);  // lists are placed by absolute tile index
+    const size_t R = rounds.size();
+    // lists | row counts | round table (2 ints a round) | first rounds
+    const size_t n_list = kWideSlots * std::max<size_t>(nb, 1), n_rc = T * std::max<size_t>(R, 1);
+    const size_t n = n_list + n_rc + 2 * std::max<size_t>(R, 1) + rbase.size();
+    if (hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device) !=
+            hipSuccess ||
+        hipMalloc(&p->d_wide, sizeof(int) * n) != hipSuccess)
+      return fail("hipMalloc of the round-list buffer failed");
+    p->d_wide_rc = p->d_wide + n_list;
+    p->d_wide_round = reinterpret_cast<int2 *>(p->d_wide_rc + n_rc);
+    p->d_wide_rbase = p->d_wide_rc + n_rc + 2 * std::max<size_t>(R, 1);
+    p->wide_rounds = static_cast<int>(R);
+    if ((R > 0 && hipMemcpyAsync(p->d_wide_round, rounds.data(), sizeof(int2) * R,
+                                 hipMemcpyHostToDevice, s) != hipSuccess) ||
+        hipMemcpyAsync(p->d_wide_rbase, rbase.data(), sizeof(int) * rbase.size(),
+                       hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail("round table upload failed");
   }
-  return st;
+  return SPMM_OK;
 }
 
 int spmm_b2sr_f32_planned(spmm_csr_plan_t plan, int T, int M, int K, int N, const int *bsrRowPtr,

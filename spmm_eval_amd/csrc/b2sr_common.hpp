@@ -36,6 +36,12 @@ constexpr int kLongChunks = 2;  // block rows above this many tile chunks go to 
 // 173 at 128; T = 32 and 64 at 512 tiles 145 and 264 us against 208 and
 // 380 at 256)
 constexpr int wide_heavy(int T) { return T <= 16 ? 256 : 512; }
+// ints per tile in a wide-tile plan's list buffer (round lists written by
+// the decode kernel, read by the gather kernel)
+constexpr int kWideSlots = 8;
+// tiles per round of the wide-tile kernels (8 / 16 / 32 KB of words for
+// T = 16 / 32 / 64)
+constexpr int wide_round_tiles(int T) { return T == 16 ? 256 : (T == 32 ? 128 : 64); }
 
 template <int EPL>
 __device__ __forceinline__ void store_row(float *__restrict__ C, int ldc, int row, int col0,

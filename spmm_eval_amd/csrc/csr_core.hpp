@@ -878,4 +878,15 @@ struct spmm_csr_plan_s {
     int *d_off = nullptr;
   } slice_sets[6];
   int sorted = -1;  // colind ascending within every row: -1 not checked yet
+  int cus = 0;  // compute units of the plan's device
+  // B2SR plans for T >= 32, one allocation at d_wide: the round lists of
+  // every call (kWideSlots ints per tile) and their row counts (T ints per
+  // round), so calls on one such plan must not overlap; then the round
+  // table (first and end tile of each round) and the first round of each
+  // plan unit's block row, built at analysis
+  int *d_wide = nullptr;
+  int *d_wide_rc = nullptr;
+  int2 *d_wide_round = nullptr;
+  int *d_wide_rbase = nullptr;
+  int wide_rounds = 0;
 };

@@ -1500,6 +1500,7 @@ int spmm_csr_plan_destroy(spmm_csr_plan_t plan) {
   if (plan->d_span) (void)hipFree(plan->d_span);
   for (auto &ss : plan->slice_sets)
     if (ss.d_off) (void)hipFree(ss.d_off);
+  if (plan->d_wide) (void)hipFree(plan->d_wide);
   delete plan;
   return SPMM_OK;
 }
