@@ -219,11 +219,15 @@ int spmm_b2sr_f32_noalloc(int T, int M, int K, int N, const int *bsrRowPtr,
 
 /* Planned FBF SpMM. spmm_b2sr_analyze bins the block rows by tile count and
  * keeps each block row's tile span beside it (SYNCHRONISES once); the
- * planned call runs in a single launch (no list state, asynchronous) and
- * reads the spans from the plan, so a plan is valid only for the bsrRowPtr
- * it was built from. bsrVal must be aligned to T bytes for T <= 8 and to 16
- * bytes for T >= 16, and K < 2^27. The plan is a spmm_csr_plan_t; it keeps
- * no pointer to bsrRowPtr. */
+ * planned call is asynchronous and reads the spans from the plan, so a plan
+ * is valid only for the bsrRowPtr it was built from. For T = 32 and 64 the
+ * plan also numbers the rounds of 128 / 64 tiles and owns a list buffer
+ * (8 ints per tile and T per round) that each call writes (a word-decode
+ * launch) and reads back (a gather launch): calls on one such plan must not
+ * overlap (other streams, other threads). Other T run one launch. bsrVal
+ * must be aligned to T bytes for T <= 8 and to 16 bytes for T >= 16, and
+ * K < 2^27. The plan is a spmm_csr_plan_t; it keeps no pointer to
+ * bsrRowPtr. */
 int spmm_b2sr_analyze(spmm_handle_t h, int T, int M, const int *bsrRowPtr,
                       spmm_csr_plan_t *plan, void *stream);
 int spmm_b2sr_f32_planned(spmm_csr_plan_t plan, int T, int M, int K, int N,

@@ -353,6 +353,56 @@ def test_fbf_planned_dense_tiles(handle, dev, T, N, bf16):
     np.testing.assert_array_equal(C, exact)
 
 
+def _mixed_rounds_graph(M, K, dense_cols, dense_p, sparse_per_row, seed):
+    """Rows with a dense column block (tiles far denser than one bit each)
+    followed by scattered columns: a block row's first round(s) hold more
+    bits than the round-list slots, later rounds one or two a tile."""
+    r = np.random.default_rng(seed)
+    rows = []
+    for i in range(M):
+        d = np.flatnonzero(r.random(dense_cols) < dense_p) if i % 5 != 2 else np.zeros(0, np.int64)
+        s = r.choice(np.arange(dense_cols, K), size=sparse_per_row if i % 9 else 0, replace=False)
+        rows.append(np.unique(np.concatenate([d, s])).astype(np.int32))
+    rowptr = np.zeros(M + 1, np.int32)
+    rowptr[1:] = np.cumsum([len(x) for x in rows])
+    colind = np.concatenate(rows).astype(np.int32)
+    return graphs.Csr(rowptr, colind, np.ones(len(colind), np.float32), M, K)
+
+
+@pytest.mark.parametrize("T", [16, 32, 64])
+def test_fbf_planned_wide_rounds(handle, dev, T):
+    """Planned wide-tile B2SR over block rows of many rounds (more than the
+    16 whose row counts the gather stages at once), with rounds whose lists
+    overflow their slots (walked from the words) between rounds merged into
+    multi-round phases: +-1 B exact against integer CSR, float and BF16 B
+    within 1e-5 of FP64, drop-in and planned forms."""
+    g = _mixed_rounds_graph(150, 140000, 3 * T, 0.6, 150, seed=T)
+    print(f"wide rounds T={T} nnz={g.nnz}", flush=True)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    for N in (32, 64):
+        S = np.random.default_rng(N + T).choice([-1, 1], size=(g.K, N)).astype(np.int32)
+        exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.float32)
+        Sd = torch.from_numpy(S.astype(np.float32)).to(dev)
+        for pl in (plan, None):
+            C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Sd, plan=pl).cpu().numpy()
+            np.testing.assert_array_equal(C, exact, err_msg=f"T={T} N={N} planned={pl is not None}")
+        B = graphs.uniform_pm1((g.K, N), seed=N * T + 1)
+        Bd = torch.from_numpy(B).to(dev)
+        ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
+        Cf = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=plan).cpu().numpy()
+        assert np.linalg.norm(Cf - ref) / np.linalg.norm(ref) <= 1e-5, (T, N)
+        Bh = api.to_bf16_bits(Bd)
+        refq = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, g.val, np_words(Bh))
+        Ch = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bh, plan=plan).cpu().numpy()
+        assert np.linalg.norm(Ch - refq) / np.linalg.norm(refq) <= 1e-5, (T, N)
+        # a second call gives the same bits (the plan's round lists are rewritten)
+        Cf2 = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=plan).cpu().numpy()
+        np.testing.assert_array_equal(Cf2, Cf)
+    plan.close()
+
+
 def test_harness_flow_from_mtx(handle, dev, tmp_path):
     """The reference harness end to end (test_spmm_full.cu:99-125, 344-369,
     412-431): a Matrix Market pattern file -> CSR -> B2SR-4 -> FBF SpMM,
