@@ -258,7 +258,11 @@ int spmm_b2sr_plan_set(spmm_csr_plan_t plan, int N, int bf16,
 
 /* Binary A x sign-packed B -> int32: C[i][k] = sum over stored (i,j) of
  * (+1 if B[j][k] >= 0 else -1), with B packed by pack_sign_rows_u32 into
- * Bw[K][ldbw] (ldbw >= ceil(N/32) words). No reference counterpart (its
+ * Bw[K][ldbw] (ldbw >= ceil(N/32) words). With a handle and 0 < K < 2^24 it
+ * expands B to +-1 BF16 in the handle's workspace (K*N*2 bytes; the first
+ * call at a larger size allocates, so capture after one such call) and runs
+ * the FBF drop-in into C's memory, converting C to int32 in place; calls on
+ * one handle must be ordered on one stream. No reference counterpart (its
  * bsrbmv.cu is absent, csr2bsr_batch_bsrbmv.cu:1). */
 int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N,
                              const int *bsrRowPtr, const int *bsrColInd,

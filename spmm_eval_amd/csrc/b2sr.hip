@@ -1126,6 +1126,46 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
     if (col0 + q < N) Crow[col0 + q] = 2 * cnt[q] - deg;
 }
 
+// Sign-packed B as +-1 BF16 (+1 where the bit is set): one thread per
+// (row, 32-column word), four 16-byte stores. Column c of a row is bit
+// 31 - c % 32 of word c / 32 (pack_sign_rows_u32's order).
+__global__ void __launch_bounds__(kB2Threads)
+sign_expand_kernel(int K, int N, const uint32_t *__restrict__ Bw, int ldbw,
+                   uint16_t *__restrict__ out) {
+  const int Wn = (N + 31) / 32;
+  const long long item = static_cast<long long>(blockIdx.x) * kB2Threads + threadIdx.x;
+  if (item >= static_cast<long long>(K) * Wn) return;
+  const int k = static_cast<int>(item / Wn), wi = static_cast<int>(item % Wn);
+  const uint32_t w = Bw[static_cast<size_t>(k) * ldbw + wi];
+  uint16_t *o = out + static_cast<size_t>(k) * N + 32 * wi;
+  const int n = min(32, N - 32 * wi);
+  if (n == 32 && N % 8 == 0) {  // 16-byte aligned rows: vector stores
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = 8 * q + 2 * j;
+        const uint32_t lo = (w >> (31 - c)) & 1u ? 0x3f80u : 0xbf80u;
+        const uint32_t hi = (w >> (30 - c)) & 1u ? 0x3f80u : 0xbf80u;
+        v[j] = lo | (hi << 16);
+      }
+      *reinterpret_cast<uint4 *>(o + 8 * q) = make_uint4(v[0], v[1], v[2], v[3]);
+    }
+  } else {
+    for (int c = 0; c < n; ++c) o[c] = (w >> (31 - c)) & 1u ? 0x3f80u : 0xbf80u;
+  }
+}
+
+// C[i][0..N) in place: FP32 values (whole numbers below 2^24) -> int32.
+__global__ void __launch_bounds__(kB2Threads)
+f32_to_i32_kernel(int M, int N, int32_t *__restrict__ C, int ldc) {
+  const long long item = static_cast<long long>(blockIdx.x) * kB2Threads + threadIdx.x;
+  if (item >= static_cast<long long>(M) * N) return;
+  int32_t *p = C + static_cast<size_t>(item / N) * ldc + item % N;
+  *p = static_cast<int32_t>(__int_as_float(*p));
+}
+
 // ------------------------------------------------------- dense packers
 // One wave per (row, 64-column chunk): a 64-bit ballot gives two words.
 // cols == false: out[h][w] (row-major, Wc words per row);
@@ -1842,7 +1882,6 @@ int spmm_b2sr_plan_set(spmm_csr_plan_t plan, int N, int bf16, const spmm_tune_t 
 int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const int *bsrRowPtr,
                              const int *bsrColInd, const void *bsrVal, const uint32_t *Bw,
                              int ldbw, int32_t *C, int ldc, void *stream) {
-  (void)h;
   SPMM_CHECK_ARG(valid_T(T), "T must be 2, 4, 8, 16, 32 or 64");
   SPMM_CHECK_ARG(M >= 0 && K >= 0 && N >= 0, "negative size");
   if (M == 0 || N == 0) return SPMM_OK;
@@ -1850,6 +1889,26 @@ int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const 
   SPMM_CHECK_ARG(ldc >= N && (K == 0 || ldbw >= (N + 31) / 32), "ldbw/ldc too small");
   SPMM_CHECK_ARG(K == 0 || Bw, "null Bw");
   hipStream_t s = as_stream(stream);
+  // The product is the FBF one with B = +-1, exact in BF16, and its sums are
+  // whole numbers of magnitude at most K, exact in FP32 below 2^24: B is
+  // expanded once into the handle's workspace, the BF16 FBF drop-in writes
+  // FP32 into C's memory, and C is converted in place.
+  if (h && K > 0 && K < (1 << 24)) {
+    auto *Bx = static_cast<uint16_t *>(workspace(h, sizeof(uint16_t) * static_cast<size_t>(K) * N, s));
+    if (!Bx) return SPMM_ERR_ALLOC;
+    const long long words = static_cast<long long>(K) * ((N + 31) / 32);
+    hipLaunchKernelGGL(sign_expand_kernel, dim3(static_cast<unsigned>(ceil_div_ll(words, kB2Threads))),
+                       dim3(kB2Threads), 0, s, K, N, Bw, ldbw, Bx);
+    SPMM_LAUNCHED();
+    const int st = run_fbf<uint16_t>(h, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, Bx, N,
+                                     reinterpret_cast<float *>(C), ldc, stream);
+    if (st != SPMM_OK) return st;
+    const long long elems = static_cast<long long>(M) * N;
+    hipLaunchKernelGGL(f32_to_i32_kernel, dim3(static_cast<unsigned>(ceil_div_ll(elems, kB2Threads))),
+                       dim3(kB2Threads), 0, s, M, N, C, ldc);
+    SPMM_LAUNCHED();
+    return SPMM_OK;
+  }
   switch (T) {
     case 2: dispatch_sign<2>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
     case 4: dispatch_sign<4>(M, N, bsrRowPtr, bsrColInd, bsrVal, Bw, ldbw, C, ldc, s); break;
