@@ -12,7 +12,7 @@ import torch
 
 import oracle
 from spmm_eval_amd import api, graphs
-from spmm_eval_amd._lib import SpmmError
+from spmm_eval_amd._lib import SpmmError, call
 
 pytestmark = pytest.mark.gpu
 HAND = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "b2sr_hand.json")))
@@ -134,7 +134,7 @@ def test_fbf_workload(handle, dev, b2sr_graph, T, N):
     Cp = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(S.astype(np.float32)).to(dev),
                        plan=plan)
     assert torch.equal(Cp, C)  # planned schedule: same known answer
-    # float B vs FP64, plain and planned (same per-row summation order)
+    # float B vs FP64, plain and planned
     B = graphs.uniform_pm1((g.K, N), seed=N * T)
     Cf = api.spmm_b2sr(handle, T, g.M, brp, bci, words, torch.from_numpy(B).to(dev)).cpu().numpy()
     ref = oracle.csr_spmm_f64(g.rowptr, g.colind, g.val, B)
@@ -203,7 +203,14 @@ def test_sign_packed_popcount(handle, dev, b2sr_graph, T, N):
     np.testing.assert_array_equal(np_u32(Bw), oracle.pack_sign_rows(X))
     C = api.spmm_b2sr_sign(handle, T, g.M, g.K, N, brp, bci, words, Bw).cpu().numpy()
     S = np.where(X >= 0, 1, -1).astype(np.int32)
-    np.testing.assert_array_equal(C, oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.int32))
+    exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.int32)
+    np.testing.assert_array_equal(C, exact)
+    if N in (64, 70):  # without a handle: the per-row popcount kernel, same integers
+        Cn = torch.empty((g.M, N), dtype=torch.int32, device=dev)
+        call("spmm_b2sr_signpacked_i32", None, T, g.M, g.K, N, api._p(brp), api._p(bci),
+             api._p(words), api._p(Bw), Bw.stride(0), api._p(Cn), Cn.stride(0), None)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(Cn.cpu().numpy(), exact)
 
 
 @pytest.mark.parametrize("H,W", [(1, 1), (5, 31), (33, 32), (100, 100), (64, 129)])
