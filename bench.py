@@ -296,7 +296,8 @@ def cpu_baseline(hd):
 
 def extra_large_variants(args, h, dev, hd):
     """The headline graph at the other widths and precision: FP32 N=64, BF16
-    N=64 and N=128 (FP32 accumulate), each autotuned; with CPU baselines."""
+    N=64 and N=128 (FP32 accumulate), each autotuned, with CPU baselines; its
+    A^T dY through the transposed plan; and B2SR-4 FBF on its pattern."""
     import torch
 
     import oracle
@@ -365,6 +366,49 @@ def extra_large_variants(args, h, dev, hd):
             "rel_err_fp64_sampled": sampled_rel_err(gt, ref, dX.cpu().numpy(), rows_t)}
     tp.close()
     del dYd, dX, gt
+    torch.cuda.empty_cache()
+    # B2SR-4 FBF on the same graph's pattern (binary A): the reddit-sized workload of the
+    # reference's only published figures for this path (backend/spmm.cu:869-870,
+    # spmm4_full_full_new2, one un-warmed launch on an unstated sm_70 GPU)
+    T = 4
+    gb = graphs.Csr(g.rowptr, g.colind, np.ones(g.nnz, np.float32), g.M, g.K)
+    torch.cuda.synchronize()
+    t0, t1 = _events()
+    t0.record()
+    brp, bci, words = api.csr2b2sr(h, rp, ci, g.M, g.K, T)
+    t1.record()
+    torch.cuda.synchronize()
+    conv_ms = t0.elapsed_time(t1)
+    nb = bci.numel()
+    bplan = api.B2srPlan(h, T, g.M, brp)
+    for N, pub in ((64, 97.027), (128, 166.101)):
+        B = graphs.uniform_pm1((g.K, N), seed=graphs.SEED_BASE + 11 + N)
+        Bd = torch.from_numpy(B).to(dev)
+        C = torch.empty((g.M, N), device=dev)
+        choice = bplan.autotune(brp, bci, words, Bd, C)
+        ms, _ = median_batches(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=C,
+                                                     plan=bplan), launches=20)
+        err = sampled_rel_err(gb, B.astype(np.float64), C.cpu().numpy(), rows)
+        ms_d, _ = median_batches(lambda: api.spmm_b2sr(h, T, g.M, brp, bci, words, Bd, C_out=C),
+                                 launches=20)
+        err_d = sampled_rel_err(gb, B.astype(np.float64), C.cpu().numpy(), rows)
+        S = np.random.default_rng(N).choice([-1, 1], size=(g.K, N)).astype(np.float32)
+        Cs = api.spmm_b2sr(h, T, g.M, brp, bci, words, torch.from_numpy(S).to(dev),
+                           plan=bplan).cpu().numpy()
+        exact = all(np.array_equal(Cs[i], S[g.colind[g.rowptr[i]:g.rowptr[i + 1]]].sum(axis=0))
+                    for i in rows)
+        out[f"b2sr4_fbf_233k_n{N}"] = {
+            "nnzb": nb, "csr2b2sr_ms_device": round(conv_ms, 2), "tuned": list(choice),
+            "planned_ms_median_of_5": round(ms, 4), "dropin_ms_median_of_5": round(ms_d, 4),
+            "compulsory_GBps": rate(b2sr_compulsory_bytes(g.M, g.K, T, nb, N), ms),
+            "effective_GBps_gather_model": rate(b2sr_gather_bytes(g.M, T, nb, g.nnz, N), ms),
+            "pm1_exact_sampled_rows": bool(exact), "rel_err_fp64_sampled": err,
+            "dropin_rel_err_fp64_sampled": err_d,
+            "reference_published_ms": pub,
+            "reference_published_on": "reddit, unstated sm_70 GPU, one un-warmed launch"}
+        del Bd, C
+    bplan.close()
+    del brp, bci, words
     torch.cuda.empty_cache()
     out["cpu"] = cpu
     return out

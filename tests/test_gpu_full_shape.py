@@ -7,8 +7,10 @@
    relative Frobenius of the FP64 oracle and every element within its
    rounding bound nnz_row * 2^-23 * sum|a||b|; BF16 within 1e-5 of FP64 of
    the BF16-rounded B (kernel exactness) and within 1e-2 of FP64 of the FP32
-   source (the precision form's contract). The reference checks its own
-   workloads at full size in the same way (test_spmm_full.cu:412-431).
+   source (the precision form's contract). Its pattern as B2SR-4: converted
+   tile for tile like the oracle, +-1 B exact on every row. The reference
+   checks its own workloads at full size in the same way
+   (test_spmm_full.cu:412-431).
 2. All 512 uneven graphs batched in one plan, FP32 and BF16: Y, dX and dval
    of every graph equal per-graph runs of the same kernels at the plan's
    geometry bit for bit, and the union is within 1e-5 of FP64 (of the rounded
@@ -88,6 +90,32 @@ def test_large_graph_bf16_every_row(handle, dev, large, N):
         err = rel_fro(Cx, ref)
         print(f"BF16 N={N} {name}: rel err vs FP64 of the FP32 source {err:.3e}", flush=True)
         assert err <= 1e-2, name
+    plan.close()
+
+
+def test_large_graph_b2sr4_every_row(handle, dev, large):
+    """B2SR-4 on the same graph's pattern (the reddit-sized workload of the
+    reference's FBF figures, backend/spmm.cu:869-870): the device csr2b2sr
+    equals the oracle's tile for tile, and FBF with +-1 B equals integer CSR
+    SpMM on every row, planned (autotuned) and plan-free, N = 64."""
+    g, (rp, ci, _) = large
+    T, N = 4, 64
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    ref = oracle.csr2b2sr(g.rowptr, g.colind, T)
+    print(f"B2SR-4: {bci.numel()} tiles", flush=True)
+    np.testing.assert_array_equal(brp.cpu().numpy(), ref[0])
+    np.testing.assert_array_equal(bci.cpu().numpy(), ref[1])
+    np.testing.assert_array_equal(words.cpu().numpy(), ref[2])
+    del ref
+    S = np.random.default_rng(5).choice([-1, 1], size=(g.K, N)).astype(np.int32)
+    exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.float32)
+    Sd = torch.from_numpy(S.astype(np.float32)).to(dev)
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    C = torch.empty((g.M, N), device=dev)
+    print(f"B2SR-4 tuned: {plan.autotune(brp, bci, words, Sd, C)}", flush=True)
+    for name, pl in (("planned", plan), ("drop-in", None)):
+        Cx = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Sd, plan=pl).cpu().numpy()
+        np.testing.assert_array_equal(Cx, exact, err_msg=name)
     plan.close()
 
 
