@@ -54,16 +54,18 @@ typedef struct spmm_batch_plan_s *spmm_batch_plan_t;
  *   lanes_per_row : lanes that share one output row (power of two, 4..64)
  *   vec           : 16-byte vectors of B loaded per lane per gather (1 or 2)
  *   rows_scale    : multiplier on rows handled per subgroup for short rows
- *   slices        : CSR SpMM only: 2 .. 64 runs the product as that
- *                   many launches per column tile, launch s taking the
- *                   entries whose column lies in the s-th equal range of B's
- *                   rows (so one launch gathers from a slice of B small
- *                   enough to stay in an XCD's L2), each adding its partial
+ *   slices        : CSR SpMM, and B2SR FBF variant 0 at T <= 8: 2 .. 64
+ *                   runs the product as that many launches per column tile,
+ *                   launch s taking the entries (tiles) whose column (block
+ *                   column) lies in the s-th equal range of B's rows (block
+ *                   rows), so one launch gathers from a slice of B small
+ *                   enough to stay in an XCD's L2, each adding its partial
  *                   sums into C in slice order; 0 or 1 = one launch. Needs
- *                   column indices ascending within each row; the plan keeps
- *                   per-row slice offsets (built on first use, which
- *                   allocates and synchronises), so a plan used with slices
- *                   is valid only for the colind it saw.
+ *                   column indices ascending within each row (bsrColInd
+ *                   within each block row); the plan keeps per-row slice
+ *                   offsets (built on first use, which allocates and
+ *                   synchronises), so a plan used with slices is valid only
+ *                   for the colind / bsrColInd it saw.
  *   variant       : B2SR FBF SpMM only (the lane mappings of the reference's
  *                   variant kernels as candidates): 0 = block-row kernels
  *                   (lanes_per_row lanes own a column segment of a block
@@ -241,8 +243,10 @@ int spmm_b2sr_bf16_planned(spmm_csr_plan_t plan, int T, int M, int K, int N,
 
 /* B2SR autotuner: times the candidate lane mappings of the planned FBF SpMM
  * (variant 0 at every lanes_per_row dividing N's 16-byte vectors x
- * rows_scale 1 / 2 / 4, variant 1, and variant 2 where N <= 8 and T <= 8) on
- * these operands (B is uint16 BF16 when bf16 != 0), records the fastest in
+ * rows_scale 1 / 2 / 4, variant 1, and variant 2 where N <= 8 and T <= 8;
+ * then, for T <= 8 when the best is variant 0 and B is 8 MB or more, that
+ * mapping at 2 / 4 / 6 / 8 / 12 / 16 column slices) on these operands (B
+ * is uint16 BF16 when bf16 != 0), records the fastest in
  * the plan for this (N, dtype) -- later planned calls use it -- and returns
  * it in *best. Writes C; SYNCHRONISES. spmm_b2sr_plan_set pins a candidate
  * for (N, dtype) instead (SPMM_ERR_INVALID_ARG for one that cannot run at

@@ -216,21 +216,22 @@ class B2srPlan(CsrPlan):
         self.p = p
 
     def autotune(self, brp, bci, words, B, C_out, stream=None) -> tuple:
-        """Times the candidate lane mappings on these operands, keeps the
-        fastest in the plan for (N, dtype) and returns it as
-        (variant, lanes_per_row, rows_scale)."""
+        """Times the candidate lane mappings (and, for T <= 8 and B of 8 MB
+        or more, column slice counts) on these operands, keeps the fastest
+        in the plan for (N, dtype) and returns it as
+        (variant, lanes_per_row, rows_scale, slices)."""
         K, N = B.shape
         best = Tune()
         call("spmm_b2sr_autotune", self.p, self.T, self.M, K, N, _p(brp), _p(bci), _p(words), _p(B),
              int(_is_bf16(B)), B.stride(0), _p(C_out), C_out.stride(0), C.byref(best),
              _stream(stream))
-        return (best.variant, best.lanes_per_row, best.rows_scale)
+        return (best.variant, best.lanes_per_row, best.rows_scale, max(best.slices, 1))
 
     def set_geometry(self, N: int, bf16: bool, variant: int, lanes_per_row: int = 0,
-                     rows_scale: int = 1):
+                     rows_scale: int = 1, slices: int = 0):
         """Pins a candidate for (N, dtype) (spmm_b2sr_plan_set)."""
         call("spmm_b2sr_plan_set", self.p, N, int(bf16),
-             C.byref(Tune(lanes_per_row, 1, rows_scale, 0, variant)))
+             C.byref(Tune(lanes_per_row, 1, rows_scale, slices, variant)))
 
 
 def spmm_b2sr(h: Handle, T: int, M: int, brp, bci, words, B, C_out=None,

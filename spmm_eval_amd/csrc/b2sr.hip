@@ -595,7 +595,7 @@ __device__ __forceinline__ int decode_rows(int t, int t1, int lane_l, int base,
 // streamed flat in groups of 8 gathers into ONE running sum, stored to C at
 // each row's last entry (a block row of ~40 bits costs ~5 gather round
 // trips). Rows without entries are zeroed.
-template <int T, typename TB, int G, class LD>
+template <int T, typename TB, int G, class LD, bool kAcc = false>
 __device__ __forceinline__ void brow_stream(int t, int t1, int lane_sub,
                                             const int *__restrict__ bci,
                                             const typename TileWord<T>::type *__restrict__ tw,
@@ -610,7 +610,7 @@ __device__ __forceinline__ void brow_stream(int t, int t1, int lane_sub,
   for (int base = 0; base < total; base += CAP) {
     int seg[T + 1];
     total = decode_rows<T, G, CAP, true>(t, t1, lane_sub, base, bci, tw, list, seg);
-    if (base == 0) {
+    if (base == 0 && !kAcc) {  // (a later column slice adds nothing to such rows)
 #pragma unroll
       for (int r = 0; r < T; ++r)
         if (seg[r] == seg[r + 1] && row0 + r < M) store_row<EPL>(C, ldc, row0 + r, col0, cur);
@@ -635,7 +635,7 @@ __device__ __forceinline__ void brow_stream(int t, int t1, int lane_sub,
           const uint32_t m = meta >> (4 * u);
           if (m & 8u) {
             const int row = row0 + static_cast<int>(m & 7u);
-            if (row < M) store_row<EPL>(C, ldc, row, col0, cur);
+            if (row < M) put_c_row<EPL, kAcc>(C, ldc, row, col0, cur);
 #pragma unroll
             for (int q = 0; q < EPL; ++q) cur[q] = 0.f;
           }
@@ -650,7 +650,7 @@ __device__ __forceinline__ void brow_stream(int t, int t1, int lane_sub,
 // once by all 64 lanes into the wave's lists; then row by row, the row's
 // entries dealt round-robin to the S subgroups (8 gathers in flight each),
 // a butterfly over the subgroups in a fixed order, subgroup 0 stores.
-template <int T, typename TB, int G, class LD>
+template <int T, typename TB, int G, class LD, bool kAcc = false>
 __device__ __forceinline__ void brow_wave(int t, int t1, int lane, const int *__restrict__ bci,
                                           const typename TileWord<T>::type *__restrict__ tw,
                                           int *wlist, const LD &ld, float *__restrict__ C,
@@ -692,7 +692,8 @@ __device__ __forceinline__ void brow_wave(int t, int t1, int lane, const int *__
 #pragma unroll
           for (int q = 0; q < EPL; ++q) cur[q] += __shfl_xor(cur[q], m);
         }
-        if (sub == 0 && row0 + r < M) store_row<EPL>(C, ldc, row0 + r, col0, cur);
+        if (sub == 0 && row0 + r < M && !(kAcc && seg[r] == seg[r + 1]))
+          put_c_row<EPL, kAcc>(C, ldc, row0 + r, col0, cur);
 #pragma unroll
         for (int q = 0; q < EPL; ++q) cur[q] = 0.f;
       }
@@ -701,15 +702,20 @@ __device__ __forceinline__ void brow_wave(int t, int t1, int lane, const int *__
   }
 }
 
-// The work of virtual workgroup `vwg` of the block-row launch table.
-template <int T, typename TB, int G, bool kBuf>
+// The work of virtual workgroup `vwg` of the block-row launch table. With
+// lo != nullptr, plan unit i's tiles are [lo[i], hi[i]) (one column slice of
+// the block row: the plan's classes, over spans no longer than the whole
+// block row's), and kAcc adds the slice's sums into C.
+template <int T, typename TB, int G, bool kBuf, bool kAcc = false>
 __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *lds,
                                         const int *__restrict__ brows, int M,
                                         const int2 *__restrict__ span,
+                                        const int *__restrict__ lo, const int *__restrict__ hi,
                                         const int *__restrict__ bci,
                                         const typename TileWord<T>::type *__restrict__ tw,
                                         const TB *__restrict__ B, int ldb, int b_bytes,
                                         float *__restrict__ C, int ldc) {
+  auto span_of = [&](int pos) { return lo ? make_int2(lo[pos], hi[pos]) : span[pos]; };
   constexpr int EPL = Vec16<TB>::kElems;
   constexpr int S = kWave / G;
   constexpr int CW = G * EPL;
@@ -734,7 +740,7 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
     int2 sp = make_int2(0, 0);
     if (pos < end) {  // the block row and its tile span arrive together
       br = brows[pos];
-      sp = span[pos];
+      sp = span_of(pos);
     }
     for (int k = 0; k < R && br >= 0; ++k) {
       const int pos_n = pos + subs_per_wg;
@@ -742,9 +748,10 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
       int2 sp_n = make_int2(0, 0);
       if (k + 1 < R && pos_n < end) {
         br_n = brows[pos_n];
-        sp_n = span[pos_n];
+        sp_n = span_of(pos_n);
       }
-      brow_stream<T, TB, G>(sp.x, sp.y, lane_sub, bci, tw, list, ld, C, ldc, br * T, M, col0);
+      brow_stream<T, TB, G, RowLoader<TB, kBuf>, kAcc>(sp.x, sp.y, lane_sub, bci, tw, list, ld, C,
+                                                       ldc, br * T, M, col0);
       br = br_n;
       sp = sp_n;
       pos = pos_n;
@@ -755,9 +762,9 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
     const int pos = sg.list_off + local_wg * kB2Waves + wave;
     if (pos >= sg.list_off + sg.count) return;  // whole wave
     const int br = brows[pos];
-    const int2 sp = span[pos];
-    brow_wave<T, TB, G>(sp.x, sp.y, lane, bci, tw, lds + wave * S * CAP, ld, C, ldc, br * T, M,
-                        col0);
+    const int2 sp = span_of(pos);
+    brow_wave<T, TB, G, RowLoader<TB, kBuf>, kAcc>(sp.x, sp.y, lane, bci, tw, lds + wave * S * CAP,
+                                                   ld, C, ldc, br * T, M, col0);
     return;
   }
   // class C: one workgroup per (block row, row): its 4S subgroups take the
@@ -765,7 +772,7 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
   // wave, and the waves' sums are added in a fixed order through LDS (the
   // list area, free once every subgroup is done).
   const int br = brows[sg.list_off + local_wg / T];
-  const int2 sp = span[sg.list_off + local_wg / T];
+  const int2 sp = span_of(sg.list_off + local_wg / T);
   const int r = local_wg % T;
   const int row = br * T + r;
   if (row >= M) return;  // whole workgroup
@@ -797,22 +804,25 @@ __device__ __forceinline__ void brow_wg(const LaunchTable &tab, int vwg, int *ld
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[q] += red[w * CW + lane_sub * EPL + q];
     }
-    store_row<EPL>(C, ldc, row, col0, acc);
+    put_c_row<EPL, kAcc>(C, ldc, row, col0, acc);
   }
 }
 
-template <int T, typename TB, int G, bool kBuf>
+// lo == nullptr: the plan's whole block rows; else column slice [lo, hi)
+// of each (kAcc: the slices after the first add into C).
+template <int T, typename TB, int G, bool kBuf, bool kAcc>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_brow_kernel(LaunchTable tab, const int *__restrict__ brows, int M,
-                 const int2 *__restrict__ span, const int *__restrict__ bci,
+                 const int2 *__restrict__ span, const int *__restrict__ lo,
+                 const int *__restrict__ hi, const int *__restrict__ bci,
                  const typename TileWord<T>::type *__restrict__ tw, const TB *__restrict__ B,
                  int ldb, int b_bytes, float *__restrict__ C, int ldc) {
   constexpr int S = kWave / G;
   constexpr int CAP = BrowChunk<G>::kCap;
   static_assert(Chunk<T, G>::kCap == CAP, "all paths share the list area");
   __shared__ int lds[kB2Waves * S][CAP];
-  brow_wg<T, TB, G, kBuf>(tab, blockIdx.x, &lds[0][0], brows, M, span, bci, tw, B, ldb, b_bytes,
-                          C, ldc);
+  brow_wg<T, TB, G, kBuf, kAcc>(tab, blockIdx.x, &lds[0][0], brows, M, span, lo, hi, bci, tw, B,
+                                ldb, b_bytes, C, ldc);
 }
 
 // ------------------------------------------- plan-free form, T <= 8
@@ -1243,7 +1253,7 @@ static void convert(int T, int M, int K, const int *rowptr, const int *colind, i
 }
 
 struct Geo {
-  int G = 0, VEC = 1, tiles = 1, rows_scale = 1;
+  int G = 0, VEC = 1, tiles = 1, rows_scale = 1, slices = 1;
 };
 
 static Geo geometry(int N, int elem_bytes, const void *B, int ldb, const void *C, int ldc) {
@@ -1421,44 +1431,52 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
   return SPMM_OK;
 }
 
+// off != nullptr: S column-slice launches over the plan's slice offsets
+// (off[s MB + i], plan order), slice s > 0 adding into C.
 template <int T, typename TB, int G>
 static void launch_brow(const spmm_csr_plan_s *p, int M, int K, const int *brp, const int *bci,
                         const void *val, const TB *B, int ldb, float *C, int ldc, int tiles,
-                        int rows_scale, hipStream_t s) {
-  constexpr int S = kWave / G;
+                        int rows_scale, const int *off, int S, hipStream_t s) {
+  constexpr int SG = kWave / G;
   constexpr int TC = BrowChunk<G>::kTiles;
   LaunchTable t;
   // bins wholly below one subgroup chunk of tiles: a subgroup per block row
   // (several only below 8 tiles); wholly below one wave chunk: a wave (the
   // wave path decodes exactly one chunk); the rest: a workgroup per row
-  build_table_ex(p->counts, p->offs, S, TC, BrowChunk<G>::kWaveTiles, 8, 16, rows_scale, &t, 1, T);
+  build_table_ex(p->counts, p->offs, SG, TC, BrowChunk<G>::kWaveTiles, 8, 16, rows_scale, &t, 1,
+                 T);
   if (t.total_wg == 0) return;
   const auto *tw = static_cast<const typename TileWord<T>::type *>(val);
   // B rows are read through a buffer resource when B spans < 2 GiB
   const int b_bytes = buffer_span(K, ldb, sizeof(TB));
-  if (b_bytes)
-    hipLaunchKernelGGL((b2sr_brow_kernel<T, TB, G, true>), dim3(t.total_wg, tiles),
-                       dim3(kB2Threads), 0, s, t, p->d_rows, M, p->d_span, bci, tw, B, ldb,
-                       b_bytes, C, ldc);
-  else
-    hipLaunchKernelGGL((b2sr_brow_kernel<T, TB, G, false>), dim3(t.total_wg, tiles),
-                       dim3(kB2Threads), 0, s, t, p->d_rows, M, p->d_span, bci, tw, B, ldb, 0, C,
-                       ldc);
+  const size_t MB = static_cast<size_t>(ceil_div(M, T));  // plan units (p->M counts rows)
+  for (int sl = 0; sl < (off ? S : 1); ++sl) {
+    const int *lo = off ? off + sl * MB : nullptr;
+    const int *hi = off ? off + (sl + 1) * MB : nullptr;
+    auto *const kern =
+        b_bytes ? (sl ? &b2sr_brow_kernel<T, TB, G, true, true> : &b2sr_brow_kernel<T, TB, G, true, false>)
+                : (sl ? &b2sr_brow_kernel<T, TB, G, false, true>
+                      : &b2sr_brow_kernel<T, TB, G, false, false>);
+    hipLaunchKernelGGL(kern, dim3(t.total_wg, tiles), dim3(kB2Threads), 0, s, t, p->d_rows, M,
+                       p->d_span, lo, hi, bci, tw, B, ldb, b_bytes, C, ldc);
+  }
 }
 
 template <int T, typename TB>
 static void dispatch_brow(const Geo &g, const spmm_csr_plan_s *p, int M, int K, const int *brp,
                           const int *bci, const void *val, const TB *B, int ldb, float *C,
-                          int ldc, hipStream_t s) {
+                          int ldc, const int *off, hipStream_t s) {
+#define SPMM_BR(gg)   launch_brow<T, TB, gg>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, off, g.slices, s)
   switch (g.G) {
-    case 1: launch_brow<T, TB, 1>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
-    case 2: launch_brow<T, TB, 2>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
-    case 4: launch_brow<T, TB, 4>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
-    case 8: launch_brow<T, TB, 8>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
-    case 16: launch_brow<T, TB, 16>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
-    case 32: launch_brow<T, TB, 32>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
-    default: launch_brow<T, TB, 64>(p, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, g.rows_scale, s); break;
+    case 1: SPMM_BR(1); break;
+    case 2: SPMM_BR(2); break;
+    case 4: SPMM_BR(4); break;
+    case 8: SPMM_BR(8); break;
+    case 16: SPMM_BR(16); break;
+    case 32: SPMM_BR(32); break;
+    default: SPMM_BR(64); break;
   }
+#undef SPMM_BR
 }
 
 // The candidate a planned call runs: `tune` when given, else the plan's
@@ -1488,6 +1506,7 @@ static Geo b2sr_candidate(const spmm_csr_plan_s *p, int T, int N, int elem_bytes
     g.tiles = lanes / G;
   }
   g.rows_scale = t.rows_scale == 2 || t.rows_scale == 4 ? t.rows_scale : 1;
+  g.slices = T <= 8 && t.slices >= 2 && t.slices <= 64 ? t.slices : 1;
   return g;
 }
 
@@ -1541,10 +1560,16 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     SPMM_LAUNCHED();
     return SPMM_OK;
   }
+  const int *off = nullptr;  // column slices: the plan's per-block-row tile offsets
+  if (T <= 8 && g.slices > 1) {
+    // a B2SR plan's units are its ceil(M / T) block rows (p->M counts rows)
+    const int st = plan_slice_offsets(p, ceil_div(M, T), g.slices, ceil_div(K, T), bci, s, &off);
+    if (st != SPMM_OK) return st;
+  }
   switch (T) {
-    case 2: dispatch_brow<2, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
-    case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, s); break;
+    case 2: dispatch_brow<2, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, off, s); break;
+    case 4: dispatch_brow<4, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, off, s); break;
+    case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, off, s); break;
     default:
       SPMM_CHECK_ARG(T == 16 || p->d_wide != nullptr, "wide-tile plan without its round lists");
       b2sr_wide_launch<TB>(T, g.G, g.tiles, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s);
@@ -1819,22 +1844,35 @@ int b2sr_autotune(spmm_csr_plan_s *p, int T, int M, int K, int N, const int *brp
   spmm_tune_t best = cands[0];
   float best_ms = 1e30f;
   int st = SPMM_OK;
-  for (int c = 0; c < nc && st == SPMM_OK; ++c) {
-    st = run_planned_fbf<TB>(p, T, M, K, N, brp, bci, val, B, ldb, C, ldc, stream, &cands[c]);
-    if (st) break;
-    if (hipEventRecord(e0, s) != hipSuccess) { st = SPMM_ERR_HIP; break; }
+  auto trial = [&](const spmm_tune_t &c) {  // one warm call, then three timed
+    st = run_planned_fbf<TB>(p, T, M, K, N, brp, bci, val, B, ldb, C, ldc, stream, &c);
+    if (st) return;
+    if (hipEventRecord(e0, s) != hipSuccess) { st = SPMM_ERR_HIP; return; }
     for (int r = 0; r < 3 && st == SPMM_OK; ++r)
-      st = run_planned_fbf<TB>(p, T, M, K, N, brp, bci, val, B, ldb, C, ldc, stream, &cands[c]);
-    if (st) break;
+      st = run_planned_fbf<TB>(p, T, M, K, N, brp, bci, val, B, ldb, C, ldc, stream, &c);
+    if (st) return;
     if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
       st = SPMM_ERR_HIP;
-      break;
+      return;
     }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     if (ms < best_ms) {
       best_ms = ms;
-      best = cands[c];
+      best = c;
+    }
+  };
+  for (int c = 0; c < nc && st == SPMM_OK; ++c) trial(cands[c]);
+  // column slices for the block-row kernels when B outgrows two XCDs' L2
+  // (the offsets are built once per slice count and kept in the plan)
+  if (st == SPMM_OK && best.variant == 0 && T <= 8 && p->sorted != 0 &&
+      static_cast<long long>(K) * N * static_cast<long long>(sizeof(TB)) >= (8ll << 20)) {
+    const spmm_tune_t base = best;
+    for (const int S : {2, 4, 6, 8, 12, 16}) {
+      if (st != SPMM_OK) break;
+      spmm_tune_t c = base;
+      c.slices = S;
+      trial(c);
     }
   }
   (void)hipEventDestroy(e0);
@@ -1872,9 +1910,11 @@ int spmm_b2sr_plan_set(spmm_csr_plan_t plan, int N, int bf16, const spmm_tune_t 
                      (G <= 64 && (G & (G - 1)) == 0 && (N / e16) % G == 0),
                  "lanes_per_row must be a power of two dividing N's 16-byte vectors");
   SPMM_CHECK_ARG(rs == 0 || rs == 1 || rs == 2 || rs == 4, "rows_scale must be 1, 2 or 4");
+  SPMM_CHECK_ARG(tune->slices <= 1 || (v == 0 && plan->tile <= 8 && tune->slices <= 64),
+                 "slices (2 .. 64) need variant 0 and T <= 8");
   spmm_tune_t t = *tune;
   t.vec = 1;
-  t.slices = 0;
+  t.slices = tune->slices > 1 ? tune->slices : 0;
   record_tune(plan, N, bf16 ? 2 : 4, t);
   return SPMM_OK;
 }

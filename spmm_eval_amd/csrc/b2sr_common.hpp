@@ -52,6 +52,26 @@ __device__ __forceinline__ void store_row(float *__restrict__ C, int ldc, int ro
     *reinterpret_cast<float4 *>(Crow + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
 }
 
+// store_row, or (kAcc: a column slice after the first) C row += v.
+template <int EPL, bool kAcc>
+__device__ __forceinline__ void put_c_row(float *__restrict__ C, int ldc, int row, int col0,
+                                          const float (&v)[EPL]) {
+  if constexpr (!kAcc) {
+    store_row<EPL>(C, ldc, row, col0, v);
+  } else {
+    float *Crow = C + static_cast<size_t>(row) * ldc + col0;
+#pragma unroll
+    for (int q = 0; q < EPL; q += 4) {
+      float4 c = *reinterpret_cast<const float4 *>(Crow + q);
+      c.x += v[q];
+      c.y += v[q + 1];
+      c.z += v[q + 2];
+      c.w += v[q + 3];
+      *reinterpret_cast<float4 *>(Crow + q) = c;
+    }
+  }
+}
+
 // Workgroups of `kernel` resident on the chip at once (per-CU count cached).
 inline int resident_wgs(const void *kernel, int cus, std::atomic<int> *cache) {
   int per_cu = cache->load(std::memory_order_relaxed);

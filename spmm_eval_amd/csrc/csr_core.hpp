@@ -841,6 +841,12 @@ int run_sddmm(const spmm_csr_plan_s *plan, int M, int K, int N, int nnz, const i
 // Bins rows by degree rowptr[(r >> shift) + 1] - rowptr[r >> shift].
 int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
             spmm_csr_plan_s **out, void *stream);
+// The plan's column-slice offsets for S equal ranges of K columns over its
+// `units` plan units (colind: their column or block-column indices,
+// ascending in each unit), built on first use (synchronises) and kept in
+// the plan per (S, K).
+int plan_slice_offsets(spmm_csr_plan_s *p, int units, int S, int K, const int *colind,
+                       hipStream_t s, const int **out);
 // The stable transpose queued on `s` without a synchronisation (the public
 // csr_transpose adds one); capturable once the workspace has grown.
 int transpose_async(spmm_handle_s *h, int M, int K, int nnz, const int *rowptr, const int *colind,

@@ -703,12 +703,13 @@ static void build_table_sliced(const int *counts, const int *offs, int G, int ro
                  static_cast<long long>(c_limit) * S, 8 * S, 16, rows_scale, t);
 }
 
-// The plan's slice offsets for (S, K), built on first use: one allocation,
-// one kernel and a read-back of the sortedness flag (synchronises). Up to
-// six (S, K) sets are kept; a seventh replaces the last. SPMM_ERR_UNSUPPORTED
-// when colind is not ascending within its rows.
-static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hipStream_t s,
-                         const int **out) {
+// The plan's slice offsets for (S, K) over its `units` plan units (rows, or
+// block rows of a B2SR plan, whose M counts rows), built on first use: one
+// allocation, one kernel and a read-back of the sortedness flag
+// (synchronises). Up to six (S, K) sets are kept; a seventh replaces the
+// last. SPMM_ERR_UNSUPPORTED when colind is not ascending within its rows.
+static int slice_offsets(spmm_csr_plan_s *p, int units, int S, int K, const int *colind,
+                         hipStream_t s, const int **out) {
   auto &sets = p->slice_sets;
   for (auto &ss : sets)
     if (ss.d_off && ss.S == S && ss.K == K) { *out = ss.d_off; return SPMM_OK; }
@@ -721,7 +722,7 @@ static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hi
     (void)hipFree(sets[slot].d_off);
     sets[slot] = {};
   }
-  const size_t n = (static_cast<size_t>(S) + 1) * static_cast<size_t>(p->M);
+  const size_t n = (static_cast<size_t>(S) + 1) * static_cast<size_t>(units);
   int *d = nullptr;
   if (hipMalloc(&d, sizeof(int) * (n + 1)) != hipSuccess)
     SPMM_FAIL(SPMM_ERR_ALLOC, "hipMalloc of the slice offsets failed");
@@ -730,8 +731,8 @@ static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hi
   const int W = ceil_div(K, S);
   hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(slice_offsets_kernel, dim3(std::min(ceil_div(p->M, kWavesPerWG), 8192)),
-                       dim3(kThreads), 0, s, p->M, S, W, p->d_span, colind, d, flag);
+    hipLaunchKernelGGL(slice_offsets_kernel, dim3(std::min(ceil_div(units, kWavesPerWG), 8192)),
+                       dim3(kThreads), 0, s, units, S, W, p->d_span, colind, d, flag);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(&unsorted, flag, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -750,6 +751,11 @@ static int slice_offsets(spmm_csr_plan_s *p, int S, int K, const int *colind, hi
   sets[slot].d_off = d;
   *out = d;
   return SPMM_OK;
+}
+
+int plan_slice_offsets(spmm_csr_plan_s *p, int units, int S, int K, const int *colind,
+                       hipStream_t s, const int **out) {
+  return slice_offsets(p, units, S, K, colind, s, out);
 }
 
 template <typename TB, int G, int VEC, bool kPerm>
@@ -891,7 +897,7 @@ static int run_planned(spmm_csr_plan_s *p, int M, int K, int N, int nnz,
   LaunchTable t;
   if (g.slices > 1 && nnz > 0) {
     const int *off = nullptr;
-    const int st = slice_offsets(p, g.slices, K, colind, s, &off);
+    const int st = slice_offsets(p, p->M, g.slices, K, colind, s, &off);
     if (st) return st;
     build_table_sliced(p->counts, p->offs, g.G, g.rows_scale, g.slices, &t);
     if (t.total_wg == 0) return SPMM_OK;

@@ -482,6 +482,42 @@ def test_b2sr_every_tuner_candidate(handle, dev, T):
     plan.close()
 
 
+@pytest.mark.parametrize("T", [2, 4, 8])
+def test_b2sr_column_slices(handle, dev, T):
+    """Planned B2SR at 2, 3 and 7 column slices (pinned on the plan): each
+    launch takes the tiles of one range of block columns and the later ones
+    add into C. +-1 B exact against integer CSR; FP32 and BF16 B within 1e-5
+    of FP64; a power-law graph whose hub block rows take every row class,
+    with K not a multiple of T times the slice count."""
+    g = graphs.chung_lu(6001, 120000, seed=T + 40, max_deg=2500)
+    rp, ci = dev_csr(g, dev)
+    brp, bci, words = api.csr2b2sr(handle, rp, ci, g.M, g.K, T)
+    plan = api.B2srPlan(handle, T, g.M, brp)
+    ones = np.ones(g.nnz, np.float32)
+    for N in (32, 64):
+        S = np.random.default_rng(N + T).choice([-1, 1], size=(g.K, N)).astype(np.int32)
+        exact = oracle.csr_spmm_i64(g.rowptr, g.colind, S).astype(np.float32)
+        Sd = torch.from_numpy(S.astype(np.float32)).to(dev)
+        B = graphs.uniform_pm1((g.K, N), seed=N + T + 1)
+        Bd = torch.from_numpy(B).to(dev)
+        ref = oracle.csr_spmm_f64(g.rowptr, g.colind, ones, B)
+        Bh = api.to_bf16_bits(Bd)
+        refh = oracle.csr_spmm_f64_bf16(g.rowptr, g.colind, ones, np_words(Bh))
+        for sl in (2, 3, 7):
+            print(f"slices T={T} N={N} S={sl}", flush=True)
+            plan.set_geometry(N, False, 0, 0, 1, sl)
+            Cs = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Sd, plan=plan).cpu().numpy()
+            np.testing.assert_array_equal(Cs, exact, err_msg=f"T={T} N={N} S={sl}")
+            C = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=plan).cpu().numpy()
+            assert np.linalg.norm(C - ref) / np.linalg.norm(ref) <= 1e-5, (T, N, sl)
+            plan.set_geometry(N, True, 0, 0, 1, sl)
+            Ch = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bh, plan=plan).cpu().numpy()
+            assert np.linalg.norm(Ch - refh) / np.linalg.norm(refh) <= 1e-5, (T, N, sl)
+    with pytest.raises(SpmmError):  # slices need variant 0
+        plan.set_geometry(32, False, 1, 0, 1, 4)
+    plan.close()
+
+
 @pytest.mark.parametrize("T,N", [(4, 64), (4, 8), (16, 32), (8, 4)])
 def test_b2sr_autotune_keeps_a_candidate(handle, dev, T, N):
     g = graphs.chung_lu(20000, 400000, seed=5)
@@ -490,10 +526,11 @@ def test_b2sr_autotune_keeps_a_candidate(handle, dev, T, N):
     plan = api.B2srPlan(handle, T, g.M, brp)
     Bd = torch.from_numpy(graphs.uniform_pm1((g.K, N), seed=3)).to(dev)
     C = torch.empty((g.M, N), device=dev)
-    var, G, rs = plan.autotune(brp, bci, words, Bd, C)
+    var, G, rs, sl = plan.autotune(brp, bci, words, Bd, C)
     assert (var, G, rs) in b2sr_candidates(T, N, 4)
+    assert sl == 1 or (var == 0 and T <= 8 and sl in (2, 4, 6, 8, 12, 16)), sl
     C1 = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=plan)  # the tuned entry
-    plan.set_geometry(N, False, var, G, rs)
+    plan.set_geometry(N, False, var, G, rs, sl)
     C2 = api.spmm_b2sr(handle, T, g.M, brp, bci, words, Bd, plan=plan)
     torch.cuda.synchronize()
     assert torch.equal(C1, C2)
