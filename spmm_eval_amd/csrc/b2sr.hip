@@ -1136,44 +1136,58 @@ b2sr_sign_kernel(int M, int N, const int *__restrict__ brp, const int *__restric
     if (col0 + q < N) Crow[col0 + q] = 2 * cnt[q] - deg;
 }
 
-// Sign-packed B as +-1 BF16 (+1 where the bit is set): one thread per
-// (row, 32-column word), four 16-byte stores. Column c of a row is bit
-// 31 - c % 32 of word c / 32 (pack_sign_rows_u32's order).
+// Sign-packed B as +-1 BF16 (+1 where the bit is set). Column c of a row
+// is bit 31 - c % 32 of word c / 32 (pack_sign_rows_u32's order). kVec
+// (N a multiple of 8): one thread per 8 columns, one 16-byte store, so a
+// wave writes 1 KB of consecutive bytes; else one thread per column. The
+// item count is below 2^32 (the caller checks).
+template <bool kVec>
 __global__ void __launch_bounds__(kB2Threads)
-sign_expand_kernel(int K, int N, const uint32_t *__restrict__ Bw, int ldbw,
+sign_expand_kernel(uint32_t items, int N, const uint32_t *__restrict__ Bw, int ldbw,
                    uint16_t *__restrict__ out) {
-  const int Wn = (N + 31) / 32;
-  const long long item = static_cast<long long>(blockIdx.x) * kB2Threads + threadIdx.x;
-  if (item >= static_cast<long long>(K) * Wn) return;
-  const int k = static_cast<int>(item / Wn), wi = static_cast<int>(item % Wn);
-  const uint32_t w = Bw[static_cast<size_t>(k) * ldbw + wi];
-  uint16_t *o = out + static_cast<size_t>(k) * N + 32 * wi;
-  const int n = min(32, N - 32 * wi);
-  if (n == 32 && N % 8 == 0) {  // 16-byte aligned rows: vector stores
+  const uint32_t i = blockIdx.x * kB2Threads + threadIdx.x;
+  if (i >= items) return;
+  const uint32_t per_row = kVec ? static_cast<uint32_t>(N) / 8 : static_cast<uint32_t>(N);
+  const uint32_t k = i / per_row, j = i - k * per_row;
+  const uint32_t c = kVec ? 8 * j : j;  // first column
+  const uint32_t w = Bw[static_cast<size_t>(k) * ldbw + c / 32];
+  if constexpr (kVec) {
+    uint32_t v[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      uint32_t v[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int c = 8 * q + 2 * j;
-        const uint32_t lo = (w >> (31 - c)) & 1u ? 0x3f80u : 0xbf80u;
-        const uint32_t hi = (w >> (30 - c)) & 1u ? 0x3f80u : 0xbf80u;
-        v[j] = lo | (hi << 16);
-      }
-      *reinterpret_cast<uint4 *>(o + 8 * q) = make_uint4(v[0], v[1], v[2], v[3]);
+      const uint32_t b = 31 - (c % 32) - 2 * q;  // columns c + 2q, c + 2q + 1
+      const uint32_t lo = (w >> b) & 1u ? 0x3f80u : 0xbf80u;
+      const uint32_t hi = (w >> (b - 1)) & 1u ? 0x3f80u : 0xbf80u;
+      v[q] = lo | (hi << 16);
     }
+    *reinterpret_cast<uint4 *>(out + static_cast<size_t>(k) * N + c) =
+        make_uint4(v[0], v[1], v[2], v[3]);
   } else {
-    for (int c = 0; c < n; ++c) o[c] = (w >> (31 - c)) & 1u ? 0x3f80u : 0xbf80u;
+    out[static_cast<size_t>(k) * N + c] = (w >> (31 - c % 32)) & 1u ? 0x3f80u : 0xbf80u;
   }
 }
 
 // C[i][0..N) in place: FP32 values (whole numbers below 2^24) -> int32.
+// kVec (N and ldc multiples of 4, C 16-byte aligned): one thread per 16
+// bytes; else one per element. The item count is below 2^32.
+template <bool kVec>
 __global__ void __launch_bounds__(kB2Threads)
-f32_to_i32_kernel(int M, int N, int32_t *__restrict__ C, int ldc) {
-  const long long item = static_cast<long long>(blockIdx.x) * kB2Threads + threadIdx.x;
-  if (item >= static_cast<long long>(M) * N) return;
-  int32_t *p = C + static_cast<size_t>(item / N) * ldc + item % N;
-  *p = static_cast<int32_t>(__int_as_float(*p));
+f32_to_i32_kernel(uint32_t items, int N, int32_t *__restrict__ C, int ldc) {
+  const uint32_t i = blockIdx.x * kB2Threads + threadIdx.x;
+  if (i >= items) return;
+  const uint32_t per_row = kVec ? static_cast<uint32_t>(N) / 4 : static_cast<uint32_t>(N);
+  const uint32_t r = i / per_row, j = i - r * per_row;
+  int32_t *p = C + static_cast<size_t>(r) * ldc + (kVec ? 4 * j : j);
+  if constexpr (kVec) {
+    int4 v = *reinterpret_cast<const int4 *>(p);
+    v.x = static_cast<int32_t>(__int_as_float(v.x));
+    v.y = static_cast<int32_t>(__int_as_float(v.y));
+    v.z = static_cast<int32_t>(__int_as_float(v.z));
+    v.w = static_cast<int32_t>(__int_as_float(v.w));
+    *reinterpret_cast<int4 *>(p) = v;
+  } else {
+    *p = static_cast<int32_t>(__int_as_float(*p));
+  }
 }
 
 // ------------------------------------------------------- dense packers
@@ -1933,19 +1947,24 @@ int spmm_b2sr_signpacked_i32(spmm_handle_t h, int T, int M, int K, int N, const 
   // whole numbers of magnitude at most K, exact in FP32 below 2^24: B is
   // expanded once into the handle's workspace, the BF16 FBF drop-in writes
   // FP32 into C's memory, and C is converted in place.
-  if (h && K > 0 && K < (1 << 24)) {
-    auto *Bx = static_cast<uint16_t *>(workspace(h, sizeof(uint16_t) * static_cast<size_t>(K) * N, s));
+  const long long kn = static_cast<long long>(K) * N, mn = static_cast<long long>(M) * N;
+  if (h && K > 0 && K < (1 << 24) && kn < (1ll << 32) && mn < (1ll << 32)) {
+    auto *Bx = static_cast<uint16_t *>(workspace(h, sizeof(uint16_t) * static_cast<size_t>(kn), s));
     if (!Bx) return SPMM_ERR_ALLOC;
-    const long long words = static_cast<long long>(K) * ((N + 31) / 32);
-    hipLaunchKernelGGL(sign_expand_kernel, dim3(static_cast<unsigned>(ceil_div_ll(words, kB2Threads))),
-                       dim3(kB2Threads), 0, s, K, N, Bw, ldbw, Bx);
+    const bool ev = N % 8 == 0;  // 16-byte rows of Bx (the workspace is 256-byte aligned)
+    const uint32_t ei = static_cast<uint32_t>(ev ? kn / 8 : kn);
+    hipLaunchKernelGGL(ev ? sign_expand_kernel<true> : sign_expand_kernel<false>,
+                       dim3(static_cast<unsigned>(ceil_div_ll(ei, kB2Threads))), dim3(kB2Threads), 0,
+                       s, ei, N, Bw, ldbw, Bx);
     SPMM_LAUNCHED();
     const int st = run_fbf<uint16_t>(h, T, M, K, N, bsrRowPtr, bsrColInd, bsrVal, Bx, N,
                                      reinterpret_cast<float *>(C), ldc, stream);
     if (st != SPMM_OK) return st;
-    const long long elems = static_cast<long long>(M) * N;
-    hipLaunchKernelGGL(f32_to_i32_kernel, dim3(static_cast<unsigned>(ceil_div_ll(elems, kB2Threads))),
-                       dim3(kB2Threads), 0, s, M, N, C, ldc);
+    const bool cv = N % 4 == 0 && ldc % 4 == 0 && reinterpret_cast<uintptr_t>(C) % 16 == 0;
+    const uint32_t ci = static_cast<uint32_t>(cv ? mn / 4 : mn);
+    hipLaunchKernelGGL(cv ? f32_to_i32_kernel<true> : f32_to_i32_kernel<false>,
+                       dim3(static_cast<unsigned>(ceil_div_ll(ci, kB2Threads))), dim3(kB2Threads), 0,
+                       s, ci, N, C, ldc);
     SPMM_LAUNCHED();
     return SPMM_OK;
   }
