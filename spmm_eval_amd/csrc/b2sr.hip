@@ -837,12 +837,21 @@ b2sr_brow_kernel(LaunchTable tab, const int *__restrict__ brows, int M,
 // k SW .. k SW + SW - 1, one subgroup each, skipping the listed ones. The
 // last workgroup to finish zeroes the list counts, so consecutive or
 // graph-replayed calls need no memset.
-template <int T, typename TB, int G, bool kBuf>
+//
+// kSorted (with a handle): the block rows come sorted by tile-count bin,
+// heaviest first (rows; meta[0] workgroup-class and meta[1] wave-class
+// block rows at the head, the classes above), from dropin_bin_sort. The
+// first E workgroups deal the two listed classes as above, and workgroup
+// E + k takes the subgroup-class block rows at sorted positions
+// k SW .. k SW + SW - 1 past them, so its subgroups get block rows of
+// similar size. No list counts to reset.
+template <int T, typename TB, int G, bool kBuf, bool kSorted>
 __global__ void __launch_bounds__(kB2Threads)
 b2sr_dropin_kernel(int M, int E, const int *__restrict__ brp, const int *__restrict__ bci,
                    const typename TileWord<T>::type *__restrict__ tw, const TB *__restrict__ B,
                    int ldb, int b_bytes, float *__restrict__ C, int ldc,
-                   int *__restrict__ state) {
+                   int *__restrict__ state, const int *__restrict__ rows,
+                   const int *__restrict__ meta) {
   constexpr int EPL = Vec16<TB>::kElems;
   constexpr int S = kWave / G, SW = kB2Waves * S;
   constexpr int CW = G * EPL;
@@ -862,11 +871,15 @@ b2sr_dropin_kernel(int M, int E, const int *__restrict__ brp, const int *__restr
   const RowLoader<TB, kBuf> ld(B, ldb, col0, b_bytes);
   const int x = blockIdx.x;
 
+  // workgroup-class block row i, wave-class block row j
+  auto wg_brow = [&](int i) { return kSorted ? rows[i] : state[kStateHead + i]; };
+  auto wave_brow = [&](int j) { return kSorted ? rows[meta[0] + j] : state[kStateHead + MB + j]; };
   if (x < E) {
-    const int n_rows = state[0] * T, n_brows = state[2];
+    const int n_rows = (kSorted ? meta[0] : state[0]) * T;
+    const int n_brows = kSorted ? meta[1] : state[2];
     const auto *words = reinterpret_cast<const typename Word<T>::type *>(tw);
     for (int i = x; i < n_rows; i += E) {  // workgroup class
-      const int br = state[kStateHead + i / T];
+      const int br = wg_brow(i / T);
       const int r = i % T;
       const int row = br * T + r;
       if (row >= M) continue;  // whole workgroup
@@ -898,9 +911,16 @@ b2sr_dropin_kernel(int M, int E, const int *__restrict__ brp, const int *__restr
       __syncthreads();  // red is the next row's list area
     }
     for (int j = (E - 1 - x) * kB2Waves + wave; j < n_brows; j += E * kB2Waves) {  // wave class
-      const int br = state[kStateHead + MB + j];
+      const int br = wave_brow(j);
       brow_wave<T, TB, G>(brp[br], brp[br + 1], lane, bci, tw, lds + wave * S * CAP, ld, C, ldc,
                           br * T, M, col0);
+    }
+  } else if constexpr (kSorted) {  // subgroup class, sorted
+    const int pos = meta[0] + meta[1] + (x - E) * SW + gsub;
+    if (pos < MB) {
+      const int br = rows[pos];
+      brow_stream<T, TB, G>(brp[br], brp[br + 1], lane_sub, bci, tw, lds + gsub * CAP, ld, C, ldc,
+                            br * T, M, col0);
     }
   } else {
     const int br = (x - E) * SW + gsub;  // subgroup class
@@ -911,6 +931,7 @@ b2sr_dropin_kernel(int M, int E, const int *__restrict__ brp, const int *__restr
                               M, col0);
     }
   }
+  if constexpr (kSorted) return;
   __syncthreads();
   if (threadIdx.x == 0) {
     // every workgroup has read the counts and lists before it counts itself
@@ -1336,19 +1357,38 @@ static void dispatch_fbf(const Geo &g, int M, int N, const int *brp, const int *
 
 // Workgroups of `kernel` the chip holds at once (per-CU occupancy, cached
 // per kernel, times the CU count).
+// h != nullptr: the block rows sorted by tile-count bin (dropin_bin_sort),
+// else listed by class_list_kernel into `state` (the allocation-free form).
 template <int T, typename TB, int G>
-static void launch_dropin(int M, int K, const int *brp, const int *bci, const void *val,
-                          const TB *B, int ldb, float *C, int ldc, int tiles, int *state,
-                          int cus, hipStream_t s) {
+static int launch_dropin(spmm_handle_s *h, int M, int K, const int *brp, const int *bci,
+                         const void *val, const TB *B, int ldb, float *C, int ldc, int tiles,
+                         int *state, int cus, hipStream_t s) {
   constexpr int SW = kB2Waves * (kWave / G);
   const int MB = (M + T - 1) / T;
-  hipLaunchKernelGGL(class_list_kernel<kB2Threads>, dim3(ceil_div(MB, kB2Threads)), dim3(kB2Threads), 0, s,
-                     MB, brp, BrowChunk<G>::kTiles, BrowChunk<G>::kWaveTiles, state);
+  const int *rows = nullptr, *meta = nullptr;
+  if (h) {
+    // the classes of class_list_kernel's limits, which are powers of two: a
+    // bin b >= 1 holds [2^(b-1), 2^b) tiles
+    constexpr int b_sub = 31 - __builtin_clz(BrowChunk<G>::kTiles);
+    constexpr int b_wg = 32 - __builtin_clz(BrowChunk<G>::kWaveTiles);
+    static_assert((BrowChunk<G>::kTiles & (BrowChunk<G>::kTiles - 1)) == 0 &&
+                      (BrowChunk<G>::kWaveTiles & (BrowChunk<G>::kWaveTiles - 1)) == 0,
+                  "class limits on bin edges");
+    const int st = dropin_bin_sort(h, brp, MB, b_wg, b_sub, s, &rows, &meta);
+    if (st != SPMM_OK) return st;
+  } else {
+    hipLaunchKernelGGL(class_list_kernel<kB2Threads>, dim3(ceil_div(MB, kB2Threads)),
+                       dim3(kB2Threads), 0, s, MB, brp, BrowChunk<G>::kTiles,
+                       BrowChunk<G>::kWaveTiles, state);
+  }
   const auto *tw = static_cast<const typename TileWord<T>::type *>(val);
   const int b_bytes = buffer_span(K, ldb, sizeof(TB));
-  auto *const kern = b_bytes ? &b2sr_dropin_kernel<T, TB, G, true>
-                             : &b2sr_dropin_kernel<T, TB, G, false>;
-  static std::atomic<int> per_cu[2];
+  auto *const kern =
+      h ? (b_bytes ? &b2sr_dropin_kernel<T, TB, G, true, true>
+                   : &b2sr_dropin_kernel<T, TB, G, false, true>)
+        : (b_bytes ? &b2sr_dropin_kernel<T, TB, G, true, false>
+                   : &b2sr_dropin_kernel<T, TB, G, false, false>);
+  static std::atomic<int> per_cu[4];
   // listed work on as many workgroups as the chip holds at once, or half
   // as many for FP32 B at T <= 4 and BF16 B at T = 2 (no more than there
   // are rows to list), then one workgroup per SW block rows. On the 89k-row
@@ -1358,27 +1398,29 @@ static void launch_dropin(int M, int K, const int *brp, const int *bci, const vo
   // FP32 T = 4: 63.5 and 65.1)
   constexpr int kDiv = (T == 2 || (T == 4 && sizeof(TB) == 4)) ? 2 : 1;
   const int E = std::min(std::max(1, resident_wgs(reinterpret_cast<const void *>(kern), cus,
-                                                  &per_cu[b_bytes ? 1 : 0]) / kDiv),
+                                                  &per_cu[(b_bytes ? 1 : 0) + (h ? 2 : 0)]) /
+                                         kDiv),
                          M);
   const dim3 grid(E + ceil_div(MB, SW), tiles);
   hipLaunchKernelGGL(kern, grid, dim3(kB2Threads), 0, s, M, E, brp, bci, tw, B, ldb, b_bytes, C,
-                     ldc, state);
+                     ldc, state, rows, meta);
+  return SPMM_OK;
 }
 
 template <int T, typename TB>
-static void dispatch_dropin(const Geo &g, int M, int K, const int *brp, const int *bci,
-                            const void *val, const TB *B, int ldb, float *C, int ldc,
-                            int *state, int cus, hipStream_t s) {
+static int dispatch_dropin(spmm_handle_s *h, const Geo &g, int M, int K, const int *brp,
+                           const int *bci, const void *val, const TB *B, int ldb, float *C,
+                           int ldc, int *state, int cus, hipStream_t s) {
 #define SPMM_DI(gg) \
-  launch_dropin<T, TB, gg>(M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, state, cus, s)
+  return launch_dropin<T, TB, gg>(h, M, K, brp, bci, val, B, ldb, C, ldc, g.tiles, state, cus, s)
   switch (g.G) {
-    case 1: SPMM_DI(1); break;
-    case 2: SPMM_DI(2); break;
-    case 4: SPMM_DI(4); break;
-    case 8: SPMM_DI(8); break;
-    case 16: SPMM_DI(16); break;
-    case 32: SPMM_DI(32); break;
-    default: SPMM_DI(64); break;
+    case 1: SPMM_DI(1);
+    case 2: SPMM_DI(2);
+    case 4: SPMM_DI(4);
+    case 8: SPMM_DI(8);
+    case 16: SPMM_DI(16);
+    case 32: SPMM_DI(32);
+    default: SPMM_DI(64);
   }
 #undef SPMM_DI
 }
@@ -1423,11 +1465,16 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
     return SPMM_OK;
   }
   if (classed) {
+    // with a handle the block rows are sorted on the device; the
+    // allocation-free form (ext_state) lists its classes instead
+    spmm_handle_s *hs = ext_state ? nullptr : h;
+    int st = SPMM_OK;
     switch (T) {
-      case 2: dispatch_dropin<2, TB>(g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
-      case 4: dispatch_dropin<4, TB>(g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
-      default: dispatch_dropin<8, TB>(g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
+      case 2: st = dispatch_dropin<2, TB>(hs, g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
+      case 4: st = dispatch_dropin<4, TB>(hs, g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
+      default: st = dispatch_dropin<8, TB>(hs, g, M, K, brp, bci, val, B, ldb, C, ldc, state, cus, s); break;
     }
+    if (st != SPMM_OK) return st;
     SPMM_LAUNCHED();
     return SPMM_OK;
   }

@@ -847,6 +847,13 @@ int analyze(spmm_handle_s *h, int M, int nnz, const int *rowptr, int shift,
 // the plan per (S, K).
 int plan_slice_offsets(spmm_csr_plan_s *p, int units, int S, int K, const int *colind,
                        hipStream_t s, const int **out);
+// Plan-free: units u sorted by the bin of ptr[u + 1] - ptr[u] (bin 0 for
+// none, else floor(log2) + 1), heaviest bin first, into *rows (the handle's
+// drop-in scratch, after its counts, cursors and class sizes); meta[0] =
+// units in bins >= b_wg, meta[1] = units in bins b_sub < b < b_wg. Three
+// launches, no synchronisation; the bin counts are left at zero.
+int dropin_bin_sort(spmm_handle_s *h, const int *ptr, int units, int b_wg, int b_sub,
+                    hipStream_t s, const int **rows, const int **meta);
 // The stable transpose queued on `s` without a synchronisation (the public
 // csr_transpose adds one); capturable once the workspace has grown.
 int transpose_async(spmm_handle_s *h, int M, int K, int nnz, const int *rowptr, const int *colind,

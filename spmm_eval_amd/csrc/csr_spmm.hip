@@ -753,6 +753,24 @@ static int slice_offsets(spmm_csr_plan_s *p, int units, int S, int K, const int 
   return SPMM_OK;
 }
 
+int dropin_bin_sort(spmm_handle_s *h, const int *ptr, int units, int b_wg, int b_sub,
+                    hipStream_t s, const int **rows, const int **meta) {
+  const int st = grow_ints(h, &h->slice_off, &h->slice_off_cap,
+                           static_cast<size_t>(kDropinHead) + static_cast<size_t>(units), s);
+  if (st != SPMM_OK) return st;
+  int *counts = h->slice_off, *cursor = counts + kBins, *m = counts + 2 * kBins;
+  int *r = h->slice_off + kDropinHead;
+  const int grid = std::min(ceil_div(units, kThreads), 2048);
+  hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(kThreads), 0, s, ptr, units, 0, counts);
+  hipLaunchKernelGGL(dropin_bins_kernel, dim3(1), dim3(kWave), 0, s, counts, cursor, m, b_wg,
+                     b_sub);
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(kThreads), 0, s, ptr, units, 0, cursor,
+                     r);
+  *rows = r;
+  *meta = m;
+  return SPMM_OK;
+}
+
 int plan_slice_offsets(spmm_csr_plan_s *p, int units, int S, int K, const int *colind,
                        hipStream_t s, const int **out) {
   return slice_offsets(p, units, S, K, colind, s, out);
