@@ -1459,8 +1459,15 @@ static int run_fbf(spmm_handle_s *h, int T, int M, int K, int N, const int *brp,
     long_grid = 8 * cus;
   }
   if (classed && T > 8) {
-    b2sr_wide_launch<TB>(T, g.G, g.tiles, nullptr, brp, state, cus, M, K, bci, val, B, ldb, C, ldc,
-                         s);
+    // with a handle: one workgroup per block row, heaviest first after a
+    // device-side sort by tile count (no classes: bins above 31 are none)
+    const int *sorted = nullptr, *meta = nullptr;
+    if (!ext_state) {
+      const int st = dropin_bin_sort(h, brp, (M + T - 1) / T, kBins, kBins - 1, s, &sorted, &meta);
+      if (st != SPMM_OK) return st;
+    }
+    b2sr_wide_launch<TB>(T, g.G, g.tiles, nullptr, brp, state, sorted, cus, M, K, bci, val, B, ldb,
+                         C, ldc, s);
     SPMM_LAUNCHED();
     return SPMM_OK;
   }
@@ -1633,7 +1640,8 @@ static int run_planned_fbf(spmm_csr_plan_s *p, int T, int M, int K, int N, const
     case 8: dispatch_brow<8, TB>(g, p, M, K, brp, bci, val, B, ldb, C, ldc, off, s); break;
     default:
       SPMM_CHECK_ARG(T == 16 || p->d_wide != nullptr, "wide-tile plan without its round lists");
-      b2sr_wide_launch<TB>(T, g.G, g.tiles, p, brp, nullptr, 0, M, K, bci, val, B, ldb, C, ldc, s);
+      b2sr_wide_launch<TB>(T, g.G, g.tiles, p, brp, nullptr, nullptr, 0, M, K, bci, val, B, ldb,
+                           C, ldc, s);
       break;
   }
   SPMM_LAUNCHED();

@@ -86,11 +86,12 @@ inline int resident_wgs(const void *kernel, int cus, std::atomic<int> *cache) {
 }
 
 // The T = 16 / 32 / 64 FBF launch (b2sr_wide.hip): G lanes per row over
-// `tiles` column tiles; p == nullptr is the plan-free form, which lists the
-// heavy block rows on `state` first.
+// `tiles` column tiles; p == nullptr is the plan-free form, over the block
+// rows sorted by tile count when `sorted` is given, else listing the heavy
+// block rows on `state` first.
 template <typename TB>
 void b2sr_wide_launch(int T, int G, int tiles, const spmm_csr_plan_s *p, const int *brp,
-                      int *state, int cus, int M, int K, const int *bci, const void *val,
-                      const TB *B, int ldb, float *C, int ldc, hipStream_t s);
+                      int *state, const int *sorted, int cus, int M, int K, const int *bci,
+                      const void *val, const TB *B, int ldb, float *C, int ldc, hipStream_t s);
 
 }  // namespace spmm

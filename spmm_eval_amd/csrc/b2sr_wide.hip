@@ -658,6 +658,21 @@ b2sr_wide_planned_kernel(const int *__restrict__ brows, const int2 *__restrict__
                             C, ldc);
 }
 
+// Plan-free with a handle: workgroup x takes the x-th block row of the
+// device-side sort by tile count (heaviest first), as the planned kernel
+// takes the plan's order.
+template <int T, typename TB, int G, bool kBuf>
+__global__ void __launch_bounds__(kB2Threads)
+b2sr_wide_sorted_kernel(const int *__restrict__ rows, const int *__restrict__ brp, int M,
+                        const int *__restrict__ bci,
+                        const typename Word<T>::type *__restrict__ words,
+                        const TB *__restrict__ B, int ldb, int b_bytes, float *__restrict__ C,
+                        int ldc) {
+  const int br = rows[blockIdx.x];
+  wide_brow<T, TB, G, kBuf>(br, make_int2(brp[br], brp[br + 1]), M, bci, words, B, ldb, b_bytes,
+                            C, ldc);
+}
+
 // Plan-free: the block rows of at least heavy_tiles tiles, listed by
 // class_list_kernel (state + kStateHead + MB, count state[2]), go first to
 // the E workgroups resident from the start (x, x + E, ...); workgroup E + k
@@ -694,16 +709,23 @@ b2sr_wide_kernel(const int *__restrict__ brp, int E, int heavy_tiles, int *__res
   }
 }
 
-// p == nullptr: the plan-free form (block rows of at least wide_heavy(T)
-// tiles listed first, on `state`); else the plan's two kernels over its
-// list buffer.
+// p == nullptr: the plan-free form, over the block rows sorted by tile
+// count when `sorted` is given, else with the block rows of at least
+// wide_heavy(T) tiles listed first on `state`; else the planned form.
 template <int T, typename TB, int G>
-static void launch_wide(const spmm_csr_plan_s *p, const int *brp, int *state, int cus, int K,
-                        const int *bci, const void *val, const TB *B, int ldb, float *C,
-                        int ldc, int tiles, int M, hipStream_t s) {
+static void launch_wide(const spmm_csr_plan_s *p, const int *brp, int *state, const int *sorted,
+                        int cus, int K, const int *bci, const void *val, const TB *B, int ldb,
+                        float *C, int ldc, int tiles, int M, hipStream_t s) {
   const int MB = (M + T - 1) / T;
   const auto *w = static_cast<const typename Word<T>::type *>(val);
   const int b_bytes = buffer_span(K, ldb, sizeof(TB));
+  if (!p && sorted) {
+    auto *const kern = b_bytes ? &b2sr_wide_sorted_kernel<T, TB, G, true>
+                               : &b2sr_wide_sorted_kernel<T, TB, G, false>;
+    hipLaunchKernelGGL(kern, dim3(MB, tiles), dim3(kB2Threads), 0, s, sorted, brp, M, bci, w, B,
+                       ldb, b_bytes, C, ldc);
+    return;
+  }
   if constexpr (T == 16) {  // 16-row block rows: too short for the split to pay
     if (p) {
       auto *const kern = b_bytes ? &b2sr_wide_planned_kernel<T, TB, G, true>
@@ -738,10 +760,11 @@ static void launch_wide(const spmm_csr_plan_s *p, const int *brp, int *state, in
 // at most).
 template <int T, typename TB>
 static void dispatch_wide(int G, int tiles, const spmm_csr_plan_s *p, const int *brp, int *state,
-                          int cus, int M, int K, const int *bci, const void *val, const TB *B,
-                          int ldb, float *C, int ldc, hipStream_t s) {
+                          const int *sorted, int cus, int M, int K, const int *bci,
+                          const void *val, const TB *B, int ldb, float *C, int ldc,
+                          hipStream_t s) {
 #define SPMM_WD(gg, tl) \
-  launch_wide<T, TB, gg>(p, brp, state, cus, K, bci, val, B, ldb, C, ldc, tl, M, s)
+  launch_wide<T, TB, gg>(p, brp, state, sorted, cus, K, bci, val, B, ldb, C, ldc, tl, M, s)
   switch (G) {
     case 1: SPMM_WD(1, tiles); break;
     case 2: SPMM_WD(2, tiles); break;
@@ -755,20 +778,24 @@ static void dispatch_wide(int G, int tiles, const spmm_csr_plan_s *p, const int 
 
 template <typename TB>
 void b2sr_wide_launch(int T, int G, int tiles, const spmm_csr_plan_s *p, const int *brp,
-                      int *state, int cus, int M, int K, const int *bci, const void *val,
-                      const TB *B, int ldb, float *C, int ldc, hipStream_t s) {
+                      int *state, const int *sorted, int cus, int M, int K, const int *bci,
+                      const void *val, const TB *B, int ldb, float *C, int ldc, hipStream_t s) {
+#define SPMM_WT(t) \
+  dispatch_wide<t, TB>(G, tiles, p, brp, state, sorted, cus, M, K, bci, val, B, ldb, C, ldc, s)
   switch (T) {
-    case 16: dispatch_wide<16, TB>(G, tiles, p, brp, state, cus, M, K, bci, val, B, ldb, C, ldc, s); break;
-    case 32: dispatch_wide<32, TB>(G, tiles, p, brp, state, cus, M, K, bci, val, B, ldb, C, ldc, s); break;
-    default: dispatch_wide<64, TB>(G, tiles, p, brp, state, cus, M, K, bci, val, B, ldb, C, ldc, s); break;
+    case 16: SPMM_WT(16); break;
+    case 32: SPMM_WT(32); break;
+    default: SPMM_WT(64); break;
   }
+#undef SPMM_WT
 }
 
 template void b2sr_wide_launch<float>(int, int, int, const spmm_csr_plan_s *, const int *, int *,
-                                      int, int, int, const int *, const void *, const float *,
-                                      int, float *, int, hipStream_t);
+                                      const int *, int, int, int, const int *, const void *,
+                                      const float *, int, float *, int, hipStream_t);
 template void b2sr_wide_launch<uint16_t>(int, int, int, const spmm_csr_plan_s *, const int *,
-                                         int *, int, int, int, const int *, const void *,
-                                         const uint16_t *, int, float *, int, hipStream_t);
+                                         int *, const int *, int, int, int, const int *,
+                                         const void *, const uint16_t *, int, float *, int,
+                                         hipStream_t);
 
 }  // namespace spmm
