@@ -1389,14 +1389,15 @@ static int launch_dropin(spmm_handle_s *h, int M, int K, const int *brp, const i
         : (b_bytes ? &b2sr_dropin_kernel<T, TB, G, true, false>
                    : &b2sr_dropin_kernel<T, TB, G, false, false>);
   static std::atomic<int> per_cu[4];
-  // listed work on as many workgroups as the chip holds at once, or half
-  // as many for FP32 B at T <= 4 and BF16 B at T = 2 (no more than there
-  // are rows to list), then one workgroup per SW block rows. On the 89k-row
-  // power-law graph at N = 64 (us per call, full / half): FP32 T = 2 63.6 /
+  // listed work on as many workgroups as the chip holds at once, then one
+  // workgroup per SW block rows. Sorted, the full count is fastest at every
+  // T, N and dtype measured on the 89k-row power-law graph (FP32 N = 64, us
+  // per call, full / half / quarter: T = 2 38.6 / 42.5 / 57.6, T = 4 36.4 /
+  // 43.6 / 60.9, T = 8 45.9 / 59.5 / 87.0). With class lists, half as many
+  // for FP32 B at T <= 4 and BF16 B at T = 2 (full / half: FP32 T = 2 63.6 /
   // 57.0, T = 4 50.0 / 47.0, T = 8 47.3 / 61.9; BF16 T = 2 45.4 / 44.2,
-  // T = 4 42.6 / 47.8, T = 8 58.2 / 79.9 (a quarter or twice the count at
-  // FP32 T = 4: 63.5 and 65.1)
-  constexpr int kDiv = (T == 2 || (T == 4 && sizeof(TB) == 4)) ? 2 : 1;
+  // T = 4 42.6 / 47.8, T = 8 58.2 / 79.9).
+  const int kDiv = h ? 1 : ((T == 2 || (T == 4 && sizeof(TB) == 4)) ? 2 : 1);
   const int E = std::min(std::max(1, resident_wgs(reinterpret_cast<const void *>(kern), cus,
                                                   &per_cu[(b_bytes ? 1 : 0) + (h ? 2 : 0)]) /
                                          kDiv),
