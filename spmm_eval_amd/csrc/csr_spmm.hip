@@ -124,9 +124,11 @@ csr_t_sddmm_kernel(LaunchTable tab, const int *__restrict__ rows,
 // XCDs. list_counts_reset_kernel, launched next, zeroes the list counts, so
 // consecutive or graph-replayed calls need no memset. (Before, the last
 // workgroup to finish did it, at the cost of one atomic per workgroup on one
-// address: a 16k-row graph at N = 32 took 23.7 us a call, now 15.5, against
+// address: a 16k-row graph at N = 32 took 23.7 us a call, then 15.5, against
 // 7.1 us planned.)
-constexpr int kDropinRounds = 2;
+// R = 1: a 16k-row graph at N = 32 takes 12.8 us a call against 15.8 at
+// R = 2 and 23.9 at R = 4 (the 89k-row graph at N = 64: 45.6 / 47.0 / 51.5)
+constexpr int kDropinRounds = 1;
 
 template <typename TB, int G, int VEC, bool kPerm>
 __global__ void __launch_bounds__(kThreads)
