@@ -186,12 +186,12 @@ int csr2b2sr(spmm_handle_t h, int M, int K, int nnz, const int *rowptr,
 
 /* FBF SpMM: C[M][N] = A_bin * B, replacing spmm4_full_full_new2 and its
  * siblings (backend/spmm.cu:877-956, launched at test_spmm_full.cu:351-361).
- * Any N >= 1 (the reference requires 32 | N and N | 1024). No plan: a small
- * kernel lists the block rows that need a wave or a workgroup into device
- * state kept in the handle, and the product kernel takes those first (with
- * bsrVal aligned as the planned form requires; otherwise a row-per-subgroup
- * kernel plus a hub-row kernel). Calls on one handle must therefore be
- * ordered on one stream. Asynchronous; graph-capturable after a first call
+ * Any N >= 1 (the reference requires 32 | N and N | 1024). No plan: with
+ * bsrVal aligned as the planned form requires, small kernels sort the block
+ * rows by tile count into device state kept in the handle, and the product
+ * kernel takes the heaviest first (otherwise a row-per-subgroup kernel plus
+ * a hub-row kernel). Calls on one handle must therefore be ordered on one
+ * stream. Asynchronous; graph-capturable after a first call
  * at the largest block-row count (which sizes the state): growing it during
  * a capture returns SPMM_ERR_UNSUPPORTED, and state replaced by a larger
  * call stays allocated until spmm_destroy, so a captured graph never points
